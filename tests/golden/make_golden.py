@@ -1,0 +1,118 @@
+"""Writes the golden vectors in tests/golden/ by running the real PGMG
+reference (built by `make -f oracle/Makefile.ref` into oracle/_ref/).
+
+The reference's own test programs print and assert nothing, so these vectors
+are the pins for parity: coarse (Galerkin) operators at every level, one
+forward and one symmetric Gauss-Seidel sweep, a restriction, an
+interpolation, V-cycle residual histories and iterates with the default
+24-sweep bottom and with the bottom LU enabled, on odd, even and non-cubic
+grids of all three synthetic problem families.  Inputs are regenerated from
+(family, seed, m, n, o) by include/pgmg_problems.h; the files keep SHA-256
+digests of the operator and source the reference actually received.
+
+Run from the repository root in a container that has /root/reference and
+/opt/conda (MPICH, MKL):  python tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import os
+import struct
+import subprocess
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+REF = os.path.join(ROOT, "oracle", "_ref", "pgmg_ref")
+OUT = os.path.join(ROOT, "tests", "golden")
+
+# (name, family, seed, m, n, o)
+FIXTURES = [
+    ("f7u_17x17x17", 1, 11, 17, 17, 17),
+    ("f7u_20x17x30", 1, 12, 20, 17, 30),
+    ("f7u_16x16x16", 1, 13, 16, 16, 16),
+    ("f7u_9x9x9", 1, 17, 9, 9, 9),
+    ("f7log_33x20x11", 2, 14, 33, 20, 11),
+    ("q1_18x13x22", 3, 15, 18, 13, 22),
+    ("q1_12x12x12", 3, 16, 12, 12, 12),
+]
+# (name, family, seed, m, n, o, bottom, tol)
+SOLVES = [
+    ("solve_f7u_65_gs24", 1, 12345, 65, 65, 65, "gs24", 1e-10),
+    ("solve_f7u_65_lu", 1, 12345, 65, 65, 65, "lu", 1e-10),
+    ("solve_q1_33x29x40_gs24", 3, 77, 33, 29, 40, "gs24", 1e-10),
+]
+
+
+def read_records(path):
+    out = {}
+    with open(path, "rb") as f:
+        data = f.read()
+    pos = 0
+    while pos < len(data):
+        name = data[pos:pos + 32].split(b"\0")[0].decode()
+        dtype, count = struct.unpack_from("<iq", data, pos + 32)
+        pos += 44
+        nbytes = count * (8 if dtype == 0 else 4)
+        arr = np.frombuffer(data[pos:pos + nbytes], dtype=np.float64 if dtype == 0 else np.int32).copy()
+        pos += nbytes
+        out[name] = arr
+    return out
+
+
+def run(args):
+    env = dict(os.environ)
+    env["LD_LIBRARY_PATH"] = "/opt/conda/lib"
+    p = subprocess.run([REF] + [str(a) for a in args], capture_output=True, text=True, env=env, timeout=600)
+    if p.returncode != 0:
+        sys.stderr.write(p.stdout + p.stderr)
+        raise RuntimeError("reference run failed: %s" % args)
+    return json.loads(p.stdout.strip().splitlines()[-1])
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def main():
+    if not os.path.exists(REF):
+        raise SystemExit("build the reference first: make -f oracle/Makefile.ref")
+    tmp = os.path.join(ROOT, "_work", "golden_tmp.bin")
+    os.makedirs(os.path.dirname(tmp), exist_ok=True)
+    for name, fam, seed, m, n, o in FIXTURES:
+        run(["fixture", fam, seed, m, n, o, tmp])
+        rec = read_records(tmp)
+        dims = rec["dims"].reshape(-1, 3)
+        save = {"meta": np.array([fam, seed, m, n, o], dtype=np.int64), "dims": dims,
+                "aexact_after_ctor": rec["aexact_after_ctor"]}
+        for k, v in rec.items():
+            if k in ("dims", "aexact_after_ctor"):
+                continue
+            if k == "A0":
+                save["sha_A0"] = np.array(sha(v.reshape(o, n, m, 27)))
+                continue
+            if k == "b":
+                save["sha_b"] = np.array(sha(v.reshape(o, n, m)))
+                continue
+            if k.startswith("S"):
+                # box bounds: keep only the min/max over the level (the
+                # [-1,1]^3 claim) to stay small
+                s = v.reshape(-1, 6)
+                save[k + "_range"] = np.array([s.min(axis=0), s.max(axis=0)])
+                continue
+            save[k] = v
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), **save)
+        print(name, "levels", dims.tolist(), flush=True)
+    for name, fam, seed, m, n, o, bottom, tol in SOLVES:
+        info = run(["solve", fam, seed, m, n, o, bottom, tol, 100, tmp])
+        rec = read_records(tmp)
+        np.savez_compressed(os.path.join(OUT, name + ".npz"),
+                            meta=np.array([fam, seed, m, n, o], dtype=np.int64),
+                            tol=np.array(tol), res=rec["res"], x=rec["x"], rhs_l2norm=rec["rhs_l2norm"],
+                            info=np.array(json.dumps(info)))
+        print(name, info, flush=True)
+    os.remove(tmp)
+
+
+if __name__ == "__main__":
+    main()
