@@ -1,0 +1,161 @@
+/* pgmg_capi.h - C ABI of the MI355X (gfx950) multigrid library libpgmg_amd.so.
+ *
+ * This is the boundary the drop-in C++ headers in include/pgmg/ call, and
+ * what any FFI (ctypes, cgo, JNI) binds.  Plain C types only: opaque handles,
+ * ints, doubles and host pointers.  Host pointers are borrowed for the call's
+ * duration.  Every function returns 0 on success or a PGMG_ERR_* code;
+ * pgmg_last_error() gives the message of the calling thread's last failure.
+ * The reference signals the same conditions through p_fatal_error ->
+ * MPI_Abort (common.cc:52-55) with the codes of common.hh:19-26; here each
+ * PGMG_ERR_<X> equals that enum's value plus one.
+ *
+ * One hierarchy lives on one GPU and owns one HIP stream.  Calls on one
+ * handle must come from one thread at a time; distinct handles are
+ * independent.
+ *
+ * Layout of host arrays: point-major, p = i + m*(j + n*k).  Operators carry
+ * 27 entries per point, e = (dk+1)*9 + (dj+1)*3 + (di+1): the reference's box
+ * order (region.hh:95-101) with the box fixed to [-1,1]^3 and zero entries
+ * where a neighbour lies outside the grid.  Grids are non-periodic with zero
+ * Dirichlet data outside. */
+#ifndef PGMG_CAPI_H
+#define PGMG_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PGMG_CAPI_VERSION 1
+
+/* Status codes: reference enum (common.hh:19-26) + 1, then new ones. */
+enum {
+	PGMG_OK = 0,
+	PGMG_ERR_FILE = 1,
+	PGMG_ERR_MEMORY = 2,
+	PGMG_ERR_INTERNAL = 3,
+	PGMG_ERR_MULTIGRID = 4,
+	PGMG_ERR_MATH = 5,
+	PGMG_ERR_SETUP = 6,
+	PGMG_ERR_HIP = 7,          /* a HIP runtime call failed */
+	PGMG_ERR_ARG = 8,          /* invalid argument */
+	PGMG_ERR_UNSUPPORTED = 9,  /* e.g. periodic axes */
+	PGMG_ERR_ADMISSION = 10,   /* serving: request cannot be admitted */
+	PGMG_ERR_NOT_CONVERGED = 11,
+	PGMG_ERR_NAN = 12
+};
+
+enum { PGMG_PREC_FP64 = 0, PGMG_PREC_MIXED = 1 };
+/* Smoother: EXACT sweeps hyperplanes i+2j+4k in order and reproduces the
+ * reference's lexicographic Gauss-Seidel bit for bit; COLOUR8 updates the
+ * eight parity classes (i&1)+2(j&1)+4(k&1) in turn. */
+enum { PGMG_SMOOTH_EXACT = 0, PGMG_SMOOTH_COLOUR8 = 1 };
+/* Bottom solve: GS24 = 24 sweeps (the reference's effective default,
+ * region_tr.cc:557); LU = dense LU factor and triangular solves
+ * (region_tr.cc:483-555). */
+enum { PGMG_BOTTOM_GS24 = 0, PGMG_BOTTOM_LU = 1 };
+
+typedef struct {
+	int m, n, o;              /* global grid size, geometry.hh:12-17 */
+	int x_prd, y_prd, z_prd;  /* periodicity; must be 0 */
+} pgmg_dims;
+
+typedef struct {
+	int precision;   /* PGMG_PREC_* */
+	int smoother;    /* PGMG_SMOOTH_* */
+	int bottom;      /* PGMG_BOTTOM_* */
+	int v_down;      /* multigrid.hh:122, default 2 */
+	int v_up;        /* multigrid.hh:128, default 2 */
+	int v_bottom;    /* multigrid.hh:125, default 20 (fmg only; kept for API parity) */
+	int use_graph;   /* capture V-cycles in a hipGraph */
+} pgmg_opts;
+
+typedef struct pgmg_hier pgmg_hier;
+
+const char *pgmg_last_error(void);
+int pgmg_version(void);
+/* Default options: FP64, COLOUR8, GS24, V(2,2), graphs on. */
+void pgmg_default_opts(pgmg_opts *opts);
+/* Levels of the hierarchy for a grid, without allocating: multigrid.cc:17-85
+ * (coarse size (m+2)>>1, region.cc:14; stop at m*n*o <= 1024). dims_out holds
+ * 3*16 ints. */
+int pgmg_hierarchy_shape(const pgmg_dims *dims, int *nlevels, int *dims_out);
+/* Device bytes a hierarchy with these options would hold. */
+int pgmg_hier_footprint(const pgmg_dims *dims, const pgmg_opts *opts, uint64_t *bytes);
+
+/* multigrid<p_class>::multigrid(pr, gm, com), multigrid.hh:282-295, and
+ * setup_hierarchy, multigrid.cc:17-85: allocate every level on the device. */
+int pgmg_hier_create(const pgmg_dims *dims, const pgmg_opts *opts, pgmg_hier **out);
+/* multigrid::~multigrid, multigrid.cc:7-14. */
+int pgmg_hier_destroy(pgmg_hier *h);
+/* nlevels / mlevels, multigrid.hh:131-135. */
+int pgmg_num_levels(const pgmg_hier *h, int *nlevels);
+/* reg[level]->m,n,o, region.hh:30-35. */
+int pgmg_level_dims(const pgmg_hier *h, int level, int dims[3]);
+
+/* region::setup_matrices, region.hh:95-101 (top level): host 27-entry rows. */
+int pgmg_set_operator27(pgmg_hier *h, const double *A27);
+/* Device-side generation of a synthetic problem (include/pgmg_problems.h):
+ * the top-level operator and the source. */
+int pgmg_generate_problem(pgmg_hier *h, int family, uint64_t seed);
+/* multigrid::compute_rats, multigrid.cc:138-152 (Galerkin operators on every
+ * coarse level, region.cc:629-660), then the bottom LU when opts.bottom is
+ * LU (region_tr.cc:472-534), then FP32 copies when precision is MIXED. */
+int pgmg_compute_rats(pgmg_hier *h);
+/* Reports whether the bottom LU factor is in use (0 = it fell back to GS24
+ * by the rule of region_tr.cc:526-533). */
+int pgmg_lu_active(const pgmg_hier *h, int *active);
+
+/* region::setup_r_field / setup_x_field, region.hh:110-126, and read-back of
+ * reg[level]->r0 / x0. */
+int pgmg_set_rhs(pgmg_hier *h, int level, const double *b);
+int pgmg_get_rhs(pgmg_hier *h, int level, double *b);
+int pgmg_set_x(pgmg_hier *h, int level, const double *x);
+int pgmg_get_x(pgmg_hier *h, int level, double *x);
+/* reg[level]->A expanded to 27-entry rows. */
+int pgmg_get_operator27(pgmg_hier *h, int level, double *A27);
+/* region::allclear_x_field, region.cc:1672-1675. */
+int pgmg_clear_x(pgmg_hier *h, int level);
+
+/* multigrid::gauss_seidel(mode, level), multigrid.cc:180-182 ->
+ * region::gauss_seidel, region.cc:355-542, repeated nsweeps times. */
+int pgmg_gauss_seidel(pgmg_hier *h, int mode, int level, int nsweeps);
+/* multigrid::restriction(level), multigrid.cc:211-224: r at level from the
+ * residual at level-1, then x at level cleared. */
+int pgmg_restriction(pgmg_hier *h, int level);
+/* multigrid::interpolation(level), multigrid.cc:196-206: x at level-1 +=
+ * P x at level. */
+int pgmg_interpolation(pgmg_hier *h, int level);
+/* multigrid::solve_exact(level, smooth), multigrid.hh:265-267. */
+int pgmg_solve_exact(pgmg_hier *h, int level, int smooth);
+/* multigrid::v_cycle(mode), multigrid.cc:88-106. */
+int pgmg_v_cycle(pgmg_hier *h, int mode);
+/* multigrid::l2_error_all(level), multigrid.cc:246-248 (sum of squared
+ * residuals, no square root) and rhs_l2norm(level), multigrid.cc:259-261. */
+int pgmg_l2_error_all(pgmg_hier *h, int level, double *out);
+int pgmg_rhs_l2norm(pgmg_hier *h, int level, double *out);
+/* Solve to a relative residual: V-cycles (FP64) or FP64 defect correction
+ * around FP32 V-cycles (MIXED) from the current x until
+ * sqrt(l2_error_all/rhs_l2norm) < tol.  hist (max_cycles+1 doubles, may be
+ * NULL) receives l2_error_all before and after each cycle.  Returns
+ * PGMG_ERR_NOT_CONVERGED if max_cycles are used up, PGMG_ERR_NAN on NaN. */
+int pgmg_solve(pgmg_hier *h, double tol, int max_cycles, int *cycles, double *hist);
+/* Wait for all work on the hierarchy's stream. */
+int pgmg_synchronize(pgmg_hier *h);
+/* region::gs_num, region.hh:17: Gauss-Seidel sweeps performed so far. */
+int pgmg_gs_count(const pgmg_hier *h, long long *count);
+
+/* Kernel timing for roofline measurement (no counterpart in the reference):
+ * while enabled, every launch of the named kernel class on level `level` is
+ * bracketed by HIP events on the hierarchy's stream; pgmg_prof_read returns
+ * the launch count, the summed event time in milliseconds and the number of
+ * grid points those launches updated.  Class 0 disables timing.  Classes: */
+enum { PGMG_KC_GS = 1, PGMG_KC_RESIDUAL = 2, PGMG_KC_RESTRICT = 3, PGMG_KC_INTERP = 4 };
+int pgmg_prof_enable(pgmg_hier *h, int kernel_class, int level);
+int pgmg_prof_read(pgmg_hier *h, long long *launches, double *total_ms, long long *points);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,744 @@
+// pgmg_hier.hip - host side of libpgmg_amd: the level hierarchy resident in
+// HBM, the V-cycle schedule, the solve drivers and the C ABI of
+// include/pgmg_capi.h.  No kernels here; see pgmg_kernels.hip.
+#include "pgmg_internal.hpp"
+#include "../../include/pgmg_capi.h"
+#include "../../include/pgmg_problems.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace pgmg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+	char buf[512];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof buf, fmt, ap);
+	va_end(ap);
+	g_err = buf;
+	return code;
+}
+
+#define HIP_TRY(call)                                                                        \
+	do {                                                                                 \
+		hipError_t e_ = (call);                                                      \
+		if (e_ != hipSuccess) return fail(PGMG_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+	} while (0)
+
+const int kMaxLevels = 16;        // mg_max_levels, common.hh:15
+const int kGsExact = 24;          // pgmg_gs_exact, common.hh:62
+const int kMaxExact = 2048;       // pgmg_max_exact, common.hh:65
+// Levels with at most this many points are smoothed by one workgroup per
+// call (all sweeps in one launch); larger levels launch per colour/plane.
+const long long kSmallColour = 20000;
+const long long kSmallExact = 40000;
+
+struct Level {
+	LevelView v;
+	double *A = nullptr, *x = nullptr, *r = nullptr;
+	float *A32 = nullptr, *x32 = nullptr, *r32 = nullptr;
+	long long points() const { return (long long)v.m * v.n * v.o; }
+};
+
+}  // namespace
+
+struct pgmg_hier {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	pgmg_opts opts;
+	int nl = 0;
+	Level lev[kMaxLevels];
+	double *res64 = nullptr;   // residual scratch, top-level size
+	float *res32 = nullptr;
+	double *partial = nullptr;
+	int partial_cap = 0;
+	double *dscalar = nullptr;  // device reduction result
+	double *hscalar = nullptr;  // pinned host mirror
+	double *LU = nullptr;
+	int *ipiv = nullptr, *lu_info = nullptr;
+	int lu_N = 0, lu_active = 0;
+	bool ops_ready = false, f32_ready = false;
+	long long gs_count = 0;
+	// graphs: [mode][precision]
+	hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+	// kernel timing
+	int prof_class = KC_NONE, prof_level = 0;
+	std::vector<hipEvent_t> prof_ev;
+	size_t prof_used = 0;
+	long long prof_launches = 0, prof_points = 0;
+	double prof_ms = 0;
+	bool capturing = false;
+};
+
+namespace {
+
+long long level_bytes(const LevelView &v, bool mixed) {
+	long long b = (27 + 2) * v.npad * 8;
+	if (mixed) b += (27 + 2) * v.npad * 4;
+	return b;
+}
+
+int shape(int m, int n, int o, int *dims) {
+	int l = 0;
+	dims[0] = m; dims[1] = n; dims[2] = o;
+	while ((long long)dims[3 * l] * dims[3 * l + 1] * dims[3 * l + 2] > 1024) {
+		if (l + 1 == kMaxLevels) return -1;
+		for (int a = 0; a < 3; a++) dims[3 * (l + 1) + a] = (dims[3 * l + a] + 2) >> 1;
+		l++;
+	}
+	return l + 1;
+}
+
+int check_dims(const pgmg_dims *d) {
+	if (!d) return fail(PGMG_ERR_ARG, "null dims");
+	if (d->x_prd || d->y_prd || d->z_prd)
+		return fail(PGMG_ERR_UNSUPPORTED, "periodic axes are not supported");
+	if (d->m < 1 || d->n < 1 || d->o < 1) return fail(PGMG_ERR_ARG, "grid sizes must be positive");
+	if ((long long)d->m * d->n * d->o > (1ll << 33)) return fail(PGMG_ERR_ARG, "grid too large");
+	return PGMG_OK;
+}
+
+int check_level(const pgmg_hier *h, int level) {
+	if (!h) return fail(PGMG_ERR_ARG, "null hierarchy");
+	if (level < 0 || level >= h->nl) return fail(PGMG_ERR_ARG, "level %d out of range [0,%d)", level, h->nl);
+	return PGMG_OK;
+}
+
+// ---- profiling: events around launches of the selected class ----
+struct ProfScope {
+	pgmg_hier *h;
+	bool on;
+	long long pts;
+	ProfScope(pgmg_hier *h_, int kc, int level, long long points) : h(h_), on(false), pts(points) {
+		if (h->prof_class != kc || h->prof_level != level || h->capturing) return;
+		if (h->prof_used + 2 > h->prof_ev.size()) {
+			for (int q = 0; q < 64; q++) {
+				hipEvent_t e;
+				if (hipEventCreate(&e) != hipSuccess) return;
+				h->prof_ev.push_back(e);
+			}
+		}
+		on = true;
+		hipEventRecord(h->prof_ev[h->prof_used], h->stream);
+	}
+	~ProfScope() {
+		if (!on) return;
+		hipEventRecord(h->prof_ev[h->prof_used + 1], h->stream);
+		h->prof_used += 2;
+		h->prof_launches++;
+		h->prof_points += pts;
+	}
+};
+
+void prof_collect(pgmg_hier *h) {
+	if (h->prof_used == 0) return;
+	hipStreamSynchronize(h->stream);
+	for (size_t q = 0; q < h->prof_used; q += 2) {
+		float ms = 0;
+		if (hipEventElapsedTime(&ms, h->prof_ev[q], h->prof_ev[q + 1]) == hipSuccess) h->prof_ms += ms;
+	}
+	h->prof_used = 0;
+}
+
+// ---- level operations, templated on the arithmetic type ----
+template <typename T> T *lA(Level &L);
+template <typename T> T *lx(Level &L);
+template <typename T> T *lr(Level &L);
+template <> double *lA<double>(Level &L) { return L.A; }
+template <> double *lx<double>(Level &L) { return L.x; }
+template <> double *lr<double>(Level &L) { return L.r; }
+template <> float *lA<float>(Level &L) { return L.A32; }
+template <> float *lx<float>(Level &L) { return L.x32; }
+template <> float *lr<float>(Level &L) { return L.r32; }
+template <typename T> T *res_scratch(pgmg_hier *h);
+template <> double *res_scratch<double>(pgmg_hier *h) { return h->res64; }
+template <> float *res_scratch<float>(pgmg_hier *h) { return h->res32; }
+
+template <typename T>
+void op_gs(pgmg_hier *h, int level, int mode, int nsweeps) {
+	Level &L = h->lev[level];
+	const bool exact = h->opts.smoother == PGMG_SMOOTH_EXACT;
+	h->gs_count += nsweeps;
+	if (nsweeps <= 0) return;
+	const long long small = exact ? kSmallExact : kSmallColour;
+	if (L.points() <= small) {
+		ProfScope ps(h, KC_GS, level, L.points() * nsweeps * (mode ? 2 : 1));
+		launch_gs_small<T>(h->stream, L.v, lA<T>(L), lx<T>(L), lr<T>(L), nsweeps, mode, exact ? 1 : 0);
+		return;
+	}
+	const int tmax = (L.v.m - 1) + 2 * (L.v.n - 1) + 4 * (L.v.o - 1);
+	for (int s = 0; s < nsweeps; s++) {
+		for (int pass = 0; pass < (mode ? 2 : 1); pass++) {
+			if (exact) {
+				ProfScope ps(h, KC_GS, level, L.points());
+				for (int tt = 0; tt <= tmax; tt++)
+					launch_gs_plane<T>(h->stream, L.v, lA<T>(L), lx<T>(L), lr<T>(L), pass ? tmax - tt : tt);
+			} else if (pass == 0) {
+				// forward: colour pairs (0,1), (2,3), (4,5), (6,7)
+				for (int q = 0; q < 4; q++) {
+					const int cj = q & 1, ck = q >> 1;
+					ProfScope ps(h, KC_GS, level, (long long)L.v.m * ((L.v.n - cj + 1) / 2) * ((L.v.o - ck + 1) / 2));
+					launch_gs_pair<T>(h->stream, L.v, lA<T>(L), lx<T>(L), lr<T>(L), q);
+				}
+			} else {
+				// backward: colours 7..0
+				for (int c = 7; c >= 0; c--) {
+					const int ci = c & 1, cj = (c >> 1) & 1, ck = c >> 2;
+					ProfScope ps(h, KC_GS, level, (long long)((L.v.m - ci + 1) / 2) * ((L.v.n - cj + 1) / 2) * ((L.v.o - ck + 1) / 2));
+					launch_gs_colour<T>(h->stream, L.v, lA<T>(L), lx<T>(L), lr<T>(L), c);
+				}
+			}
+		}
+	}
+}
+
+template <typename T>
+void op_restriction(pgmg_hier *h, int level) {
+	Level &F = h->lev[level - 1], &C = h->lev[level];
+	{
+		ProfScope ps(h, KC_RESIDUAL, level - 1, F.points());
+		launch_residual<T, T>(h->stream, F.v, lA<T>(F), lx<T>(F), lr<T>(F), res_scratch<T>(h), (double *)nullptr);
+	}
+	ProfScope ps(h, KC_RESTRICT, level, C.points());
+	launch_restrict<T>(h->stream, F.v, C.v, res_scratch<T>(h), lr<T>(C), lx<T>(C));
+}
+
+template <typename T>
+void op_interpolation(pgmg_hier *h, int level) {
+	Level &C = h->lev[level], &F = h->lev[level - 1];
+	ProfScope ps(h, KC_INTERP, level, F.points());
+	launch_interp<T>(h->stream, C.v, F.v, lx<T>(C), lx<T>(F));
+}
+
+template <typename T>
+void op_solve_exact(pgmg_hier *h, int level, bool smooth) {
+	Level &L = h->lev[level];
+	if (level == h->nl - 1 && h->lu_active && !smooth) {
+		launch_lu_solve<T>(h->stream, L.v, h->lu_N, h->LU, h->ipiv, lr<T>(L), lx<T>(L));
+	} else {
+		op_gs<T>(h, level, 0, kGsExact);
+	}
+}
+
+// multigrid::v_cycle, multigrid.cc:88-106.
+template <typename T>
+void op_v_cycle(pgmg_hier *h, int mode) {
+	const int L = h->nl;
+	for (int i = 0; i < L - 1; i++) {
+		op_gs<T>(h, i, mode, h->opts.v_down + i);
+		op_restriction<T>(h, i + 1);
+	}
+	op_solve_exact<T>(h, L - 1, false);
+	for (int i = L - 1; i > 0; i--) {
+		op_interpolation<T>(h, i);
+		op_gs<T>(h, i - 1, mode, h->opts.v_up + i - 1);
+	}
+}
+
+int ensure_partial(pgmg_hier *h, int n) {
+	if (n <= h->partial_cap) return PGMG_OK;
+	if (h->partial) hipFree(h->partial);
+	h->partial = nullptr;
+	HIP_TRY(hipMalloc(&h->partial, sizeof(double) * (size_t)n));
+	h->partial_cap = n;
+	return PGMG_OK;
+}
+
+int read_scalar(pgmg_hier *h, double *out) {
+	HIP_TRY(hipMemcpyAsync(h->hscalar, h->dscalar, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+	HIP_TRY(hipStreamSynchronize(h->stream));
+	*out = *h->hscalar;
+	return PGMG_OK;
+}
+
+// Fine residual norm (FP64); optionally write the FP32 residual for the
+// mixed-precision inner cycle.
+int residual_norm(pgmg_hier *h, int level, float *res32_out, double *out) {
+	Level &L = h->lev[level];
+	int n = residual_partial_count(L.v);
+	int rc = ensure_partial(h, n);
+	if (rc) return rc;
+	{
+		ProfScope ps(h, KC_RESIDUAL, level, L.points());
+		if (res32_out) launch_residual<double, float>(h->stream, L.v, L.A, L.x, L.r, res32_out, h->partial);
+		else launch_residual<double, double>(h->stream, L.v, L.A, L.x, L.r, (double *)nullptr, h->partial);
+	}
+	launch_reduce(h->stream, h->partial, n, h->dscalar);
+	return read_scalar(h, out);
+}
+
+int run_cycle(pgmg_hier *h, int mode, bool fp32) {
+	const int pi = fp32 ? 1 : 0, mi = mode ? 1 : 0;
+	if (h->opts.use_graph && h->prof_class == KC_NONE) {
+		if (!h->graph[mi][pi]) {
+			hipGraph_t g;
+			HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+			h->capturing = true;
+			long long gc = h->gs_count;
+			if (fp32) op_v_cycle<float>(h, mode); else op_v_cycle<double>(h, mode);
+			h->gs_count = gc;
+			h->capturing = false;
+			HIP_TRY(hipStreamEndCapture(h->stream, &g));
+			hipError_t e = hipGraphInstantiate(&h->graph[mi][pi], g, nullptr, nullptr, 0);
+			hipGraphDestroy(g);
+			if (e != hipSuccess) { h->graph[mi][pi] = nullptr; return fail(PGMG_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e)); }
+		}
+		// Count sweeps as an eager cycle would.
+		long long per = 0;
+		for (int i = 0; i < h->nl - 1; i++) per += (h->opts.v_down + i) + (h->opts.v_up + i);
+		if (!(h->lu_active)) per += kGsExact;
+		h->gs_count += per;
+		HIP_TRY(hipGraphLaunch(h->graph[mi][pi], h->stream));
+	} else {
+		if (fp32) op_v_cycle<float>(h, mode); else op_v_cycle<double>(h, mode);
+	}
+	return PGMG_OK;
+}
+
+void free_graphs(pgmg_hier *h) {
+	for (int a = 0; a < 2; a++) for (int b = 0; b < 2; b++)
+		if (h->graph[a][b]) { hipGraphExecDestroy(h->graph[a][b]); h->graph[a][b] = nullptr; }
+}
+
+int ensure_f32(pgmg_hier *h) {
+	if (h->opts.precision != PGMG_PREC_MIXED) return fail(PGMG_ERR_ARG, "hierarchy was not created with MIXED precision");
+	if (!h->ops_ready) return fail(PGMG_ERR_SETUP, "operators not set up (compute_rats)");
+	return PGMG_OK;
+}
+
+}  // namespace
+
+// ======================================================================
+// C ABI
+// ======================================================================
+extern "C" {
+
+const char *pgmg_last_error(void) { return g_err.c_str(); }
+int pgmg_version(void) { return PGMG_CAPI_VERSION; }
+
+void pgmg_default_opts(pgmg_opts *o) {
+	o->precision = PGMG_PREC_FP64;
+	o->smoother = PGMG_SMOOTH_COLOUR8;
+	o->bottom = PGMG_BOTTOM_GS24;
+	o->v_down = 2; o->v_up = 2; o->v_bottom = 20;
+	o->use_graph = 1;
+}
+
+int pgmg_hierarchy_shape(const pgmg_dims *d, int *nlevels, int *dims_out) {
+	int rc = check_dims(d);
+	if (rc) return rc;
+	int tmp[3 * kMaxLevels];
+	int nl = shape(d->m, d->n, d->o, tmp);
+	if (nl < 0) return fail(PGMG_ERR_MULTIGRID, "Maximum number of multigrid levels reached");
+	if (nlevels) *nlevels = nl;
+	if (dims_out) memcpy(dims_out, tmp, sizeof(int) * 3 * nl);
+	return PGMG_OK;
+}
+
+int pgmg_hier_footprint(const pgmg_dims *d, const pgmg_opts *opts, uint64_t *bytes) {
+	int rc = check_dims(d);
+	if (rc) return rc;
+	int dims[3 * kMaxLevels];
+	int nl = shape(d->m, d->n, d->o, dims);
+	if (nl < 0) return fail(PGMG_ERR_MULTIGRID, "Maximum number of multigrid levels reached");
+	const bool mixed = opts && opts->precision == PGMG_PREC_MIXED;
+	uint64_t b = 0;
+	for (int l = 0; l < nl; l++) b += (uint64_t)level_bytes(make_level_view(dims[3 * l], dims[3 * l + 1], dims[3 * l + 2]), mixed);
+	LevelView top = make_level_view(d->m, d->n, d->o);
+	b += (uint64_t)top.npad * (8 + (mixed ? 4 : 0));
+	b += (uint64_t)residual_partial_count(top) * 8;
+	if (opts && opts->bottom == PGMG_BOTTOM_LU) {
+		LevelView bot = make_level_view(dims[3 * (nl - 1)], dims[3 * (nl - 1) + 1], dims[3 * (nl - 1) + 2]);
+		long long N = (long long)bot.m * bot.n * bot.o;
+		b += (uint64_t)(N * N * 8 + N * 4);
+	}
+	*bytes = b;
+	return PGMG_OK;
+}
+
+int pgmg_hier_create(const pgmg_dims *d, const pgmg_opts *opts, pgmg_hier **out) {
+	if (!out) return fail(PGMG_ERR_ARG, "null output");
+	*out = nullptr;
+	int rc = check_dims(d);
+	if (rc) return rc;
+	pgmg_opts o;
+	if (opts) o = *opts; else pgmg_default_opts(&o);
+	if (o.precision != PGMG_PREC_FP64 && o.precision != PGMG_PREC_MIXED) return fail(PGMG_ERR_ARG, "bad precision");
+	if (o.smoother != PGMG_SMOOTH_EXACT && o.smoother != PGMG_SMOOTH_COLOUR8) return fail(PGMG_ERR_ARG, "bad smoother");
+	if (o.bottom != PGMG_BOTTOM_GS24 && o.bottom != PGMG_BOTTOM_LU) return fail(PGMG_ERR_ARG, "bad bottom");
+	if (o.v_down < 0 || o.v_up < 0) return fail(PGMG_ERR_ARG, "negative sweep counts");
+	int dims[3 * kMaxLevels];
+	int nl = shape(d->m, d->n, d->o, dims);
+	if (nl < 0) return fail(PGMG_ERR_MULTIGRID, "Maximum number of multigrid levels reached");
+	int ndev = 0;
+	HIP_TRY(hipGetDeviceCount(&ndev));
+	if (ndev < 1) return fail(PGMG_ERR_HIP, "no HIP device");
+	pgmg_hier *h = new pgmg_hier();
+	h->opts = o;
+	h->nl = nl;
+	hipGetDevice(&h->device);
+	const bool mixed = o.precision == PGMG_PREC_MIXED;
+#define ALLOC(ptr, bytes)                                                                     \
+	do {                                                                                  \
+		hipError_t e_ = hipMalloc((void **)&(ptr), (bytes));                          \
+		if (e_ != hipSuccess) { pgmg_hier_destroy(h); return fail(PGMG_ERR_MEMORY, "device allocation of %zu bytes failed: %s", (size_t)(bytes), hipGetErrorString(e_)); } \
+	} while (0)
+	if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+		delete h;
+		return fail(PGMG_ERR_HIP, "stream creation failed");
+	}
+	for (int l = 0; l < nl; l++) {
+		Level &L = h->lev[l];
+		L.v = make_level_view(dims[3 * l], dims[3 * l + 1], dims[3 * l + 2]);
+		const size_t np = (size_t)L.v.npad;
+		ALLOC(L.A, 27 * np * 8);
+		ALLOC(L.x, np * 8);
+		ALLOC(L.r, np * 8);
+		launch_fill<double>(h->stream, L.A, 27 * (long long)np, 0.0);
+		launch_fill<double>(h->stream, L.x, (long long)np, 0.0);
+		launch_fill<double>(h->stream, L.r, (long long)np, 0.0);
+		if (mixed) {
+			ALLOC(L.A32, 27 * np * 4);
+			ALLOC(L.x32, np * 4);
+			ALLOC(L.r32, np * 4);
+			launch_fill<float>(h->stream, L.A32, 27 * (long long)np, 0.0f);
+			launch_fill<float>(h->stream, L.x32, (long long)np, 0.0f);
+			launch_fill<float>(h->stream, L.r32, (long long)np, 0.0f);
+		}
+	}
+	const size_t np0 = (size_t)h->lev[0].v.npad;
+	ALLOC(h->res64, np0 * 8);
+	launch_fill<double>(h->stream, h->res64, (long long)np0, 0.0);
+	if (mixed) {
+		ALLOC(h->res32, np0 * 4);
+		launch_fill<float>(h->stream, h->res32, (long long)np0, 0.0f);
+	}
+	ALLOC(h->dscalar, 16);
+	if (hipHostMalloc((void **)&h->hscalar, 16) != hipSuccess) { pgmg_hier_destroy(h); return fail(PGMG_ERR_MEMORY, "pinned alloc failed"); }
+	rc = ensure_partial(h, residual_partial_count(h->lev[0].v));
+	if (rc) { pgmg_hier_destroy(h); return rc; }
+#undef ALLOC
+	if (hipStreamSynchronize(h->stream) != hipSuccess) { pgmg_hier_destroy(h); return fail(PGMG_ERR_HIP, "initialisation failed"); }
+	*out = h;
+	return PGMG_OK;
+}
+
+int pgmg_hier_destroy(pgmg_hier *h) {
+	if (!h) return PGMG_OK;
+	if (h->stream) hipStreamSynchronize(h->stream);
+	free_graphs(h);
+	for (int l = 0; l < kMaxLevels; l++) {
+		Level &L = h->lev[l];
+		hipFree(L.A); hipFree(L.x); hipFree(L.r);
+		hipFree(L.A32); hipFree(L.x32); hipFree(L.r32);
+	}
+	hipFree(h->res64); hipFree(h->res32); hipFree(h->partial); hipFree(h->dscalar);
+	if (h->hscalar) hipHostFree(h->hscalar);
+	hipFree(h->LU); hipFree(h->ipiv); hipFree(h->lu_info);
+	for (auto e : h->prof_ev) hipEventDestroy(e);
+	if (h->stream) hipStreamDestroy(h->stream);
+	delete h;
+	return PGMG_OK;
+}
+
+int pgmg_num_levels(const pgmg_hier *h, int *nl) {
+	if (!h || !nl) return fail(PGMG_ERR_ARG, "null argument");
+	*nl = h->nl;
+	return PGMG_OK;
+}
+
+int pgmg_level_dims(const pgmg_hier *h, int level, int dims[3]) {
+	int rc = check_level(h, level);
+	if (rc) return rc;
+	dims[0] = h->lev[level].v.m; dims[1] = h->lev[level].v.n; dims[2] = h->lev[level].v.o;
+	return PGMG_OK;
+}
+
+static int upload_op(pgmg_hier *h, int level, const double *A27) {
+	Level &L = h->lev[level];
+	const long long N = L.points();
+	const long long chunk = 1 << 20;
+	double *stage = nullptr;
+	HIP_TRY(hipMalloc(&stage, sizeof(double) * 27 * (size_t)std::min(N, chunk)));
+	for (long long p0 = 0; p0 < N; p0 += chunk) {
+		const long long cnt = std::min(chunk, N - p0);
+		hipError_t e = hipMemcpyAsync(stage, A27 + 27 * p0, sizeof(double) * 27 * cnt, hipMemcpyHostToDevice, h->stream);
+		if (e != hipSuccess) { hipFree(stage); return fail(PGMG_ERR_HIP, "upload: %s", hipGetErrorString(e)); }
+		launch_pack_op(h->stream, L.v, stage, L.A, p0, cnt);
+		hipStreamSynchronize(h->stream);
+	}
+	hipFree(stage);
+	return PGMG_OK;
+}
+
+int pgmg_set_operator27(pgmg_hier *h, const double *A27) {
+	if (!h || !A27) return fail(PGMG_ERR_ARG, "null argument");
+	int rc = upload_op(h, 0, A27);
+	if (rc) return rc;
+	h->ops_ready = false;
+	free_graphs(h);
+	return PGMG_OK;
+}
+
+int pgmg_generate_problem(pgmg_hier *h, int family, uint64_t seed) {
+	if (!h) return fail(PGMG_ERR_ARG, "null hierarchy");
+	if (family != PGMG_FAM_FACE7_UNIFORM && family != PGMG_FAM_FACE7_LOGUNI && family != PGMG_FAM_Q1_ANISO)
+		return fail(PGMG_ERR_ARG, "unknown problem family %d", family);
+	launch_generate(h->stream, h->lev[0].v, family, seed, h->lev[0].A, h->lev[0].r);
+	HIP_TRY(hipStreamSynchronize(h->stream));
+	h->ops_ready = false;
+	free_graphs(h);
+	return PGMG_OK;
+}
+
+int pgmg_compute_rats(pgmg_hier *h) {
+	if (!h) return fail(PGMG_ERR_ARG, "null hierarchy");
+	free_graphs(h);
+	for (int l = 1; l < h->nl; l++) launch_rat(h->stream, h->lev[l - 1].v, h->lev[l].v, h->lev[l - 1].A, h->lev[l].A);
+	h->lu_active = 0;
+	if (h->opts.bottom == PGMG_BOTTOM_LU) {
+		Level &B = h->lev[h->nl - 1];
+		const long long N = B.points();
+		if (N < kMaxExact) {  // enable_exact, region_tr.cc:472-478
+			if (!h->LU) {
+				HIP_TRY(hipMalloc(&h->LU, sizeof(double) * (size_t)(N * N)));
+				HIP_TRY(hipMalloc(&h->ipiv, sizeof(int) * (size_t)N));
+				HIP_TRY(hipMalloc(&h->lu_info, sizeof(int) * 2));
+			}
+			h->lu_N = (int)N;
+			launch_fill<double>(h->stream, h->LU, N * N, 0.0);
+			launch_lu_assemble(h->stream, B.v, B.A, h->LU);
+			launch_lu_factor(h->stream, (int)N, h->LU, h->ipiv, h->lu_info);
+			int info[2] = {0, 1};
+			HIP_TRY(hipMemcpyAsync(info, h->lu_info, sizeof(info), hipMemcpyDeviceToHost, h->stream));
+			HIP_TRY(hipStreamSynchronize(h->stream));
+			h->lu_active = info[1] == 0;  // else fall back on 24 GS sweeps, region_tr.cc:527-533
+		}
+	}
+	if (h->opts.precision == PGMG_PREC_MIXED) {
+		for (int l = 0; l < h->nl; l++) launch_round_f32(h->stream, h->lev[l].A, h->lev[l].A32, 27 * h->lev[l].v.npad);
+	}
+	HIP_TRY(hipStreamSynchronize(h->stream));
+	h->ops_ready = true;
+	return PGMG_OK;
+}
+
+int pgmg_lu_active(const pgmg_hier *h, int *active) {
+	if (!h || !active) return fail(PGMG_ERR_ARG, "null argument");
+	*active = h->lu_active;
+	return PGMG_OK;
+}
+
+static int vec_io(pgmg_hier *h, int level, double *dptr, const double *hin, double *hout) {
+	Level &L = h->lev[level];
+	const long long N = L.points();
+	double *stage = nullptr;
+	HIP_TRY(hipMalloc(&stage, sizeof(double) * (size_t)N));
+	hipError_t e = hipSuccess;
+	if (hin) {
+		e = hipMemcpyAsync(stage, hin, sizeof(double) * N, hipMemcpyHostToDevice, h->stream);
+		if (e == hipSuccess) launch_pack_vec(h->stream, L.v, stage, dptr, 0, N);
+	} else {
+		launch_unpack_vec(h->stream, L.v, dptr, stage, 0, N);
+		e = hipMemcpyAsync(hout, stage, sizeof(double) * N, hipMemcpyDeviceToHost, h->stream);
+	}
+	if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+	hipFree(stage);
+	if (e != hipSuccess) return fail(PGMG_ERR_HIP, "vector transfer: %s", hipGetErrorString(e));
+	return PGMG_OK;
+}
+
+int pgmg_set_rhs(pgmg_hier *h, int level, const double *b) {
+	int rc = check_level(h, level);
+	if (rc) return rc;
+	if (!b) return fail(PGMG_ERR_ARG, "null array");
+	return vec_io(h, level, h->lev[level].r, b, nullptr);
+}
+int pgmg_get_rhs(pgmg_hier *h, int level, double *b) {
+	int rc = check_level(h, level);
+	if (rc) return rc;
+	if (!b) return fail(PGMG_ERR_ARG, "null array");
+	return vec_io(h, level, h->lev[level].r, nullptr, b);
+}
+int pgmg_set_x(pgmg_hier *h, int level, const double *x) {
+	int rc = check_level(h, level);
+	if (rc) return rc;
+	if (!x) return fail(PGMG_ERR_ARG, "null array");
+	return vec_io(h, level, h->lev[level].x, x, nullptr);
+}
+int pgmg_get_x(pgmg_hier *h, int level, double *x) {
+	int rc = check_level(h, level);
+	if (rc) return rc;
+	if (!x) return fail(PGMG_ERR_ARG, "null array");
+	return vec_io(h, level, h->lev[level].x, nullptr, x);
+}
+
+int pgmg_get_operator27(pgmg_hier *h, int level, double *A27) {
+	int rc = check_level(h, level);
+	if (rc) return rc;
+	if (!A27) return fail(PGMG_ERR_ARG, "null array");
+	Level &L = h->lev[level];
+	const long long N = L.points(), chunk = 1 << 20;
+	double *stage = nullptr;
+	HIP_TRY(hipMalloc(&stage, sizeof(double) * 27 * (size_t)std::min(N, chunk)));
+	for (long long p0 = 0; p0 < N; p0 += chunk) {
+		const long long cnt = std::min(chunk, N - p0);
+		launch_unpack_op(h->stream, L.v, L.A, stage, p0, cnt);
+		hipError_t e = hipMemcpyAsync(A27 + 27 * p0, stage, sizeof(double) * 27 * cnt, hipMemcpyDeviceToHost, h->stream);
+		if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+		if (e != hipSuccess) { hipFree(stage); return fail(PGMG_ERR_HIP, "download: %s", hipGetErrorString(e)); }
+	}
+	hipFree(stage);
+	return PGMG_OK;
+}
+
+int pgmg_clear_x(pgmg_hier *h, int level) {
+	int rc = check_level(h, level);
+	if (rc) return rc;
+	launch_fill<double>(h->stream, h->lev[level].x, h->lev[level].v.npad, 0.0);
+	HIP_TRY(hipStreamSynchronize(h->stream));
+	return PGMG_OK;
+}
+
+int pgmg_gauss_seidel(pgmg_hier *h, int mode, int level, int nsweeps) {
+	int rc = check_level(h, level);
+	if (rc) return rc;
+	op_gs<double>(h, level, mode, nsweeps);
+	HIP_TRY(hipGetLastError());
+	return PGMG_OK;
+}
+
+int pgmg_restriction(pgmg_hier *h, int level) {
+	int rc = check_level(h, level);
+	if (rc) return rc;
+	if (level < 1) return fail(PGMG_ERR_ARG, "restriction needs level >= 1");
+	op_restriction<double>(h, level);
+	HIP_TRY(hipGetLastError());
+	return PGMG_OK;
+}
+
+int pgmg_interpolation(pgmg_hier *h, int level) {
+	int rc = check_level(h, level);
+	if (rc) return rc;
+	if (level < 1) return fail(PGMG_ERR_ARG, "interpolation needs level >= 1");
+	op_interpolation<double>(h, level);
+	HIP_TRY(hipGetLastError());
+	return PGMG_OK;
+}
+
+int pgmg_solve_exact(pgmg_hier *h, int level, int smooth) {
+	int rc = check_level(h, level);
+	if (rc) return rc;
+	op_solve_exact<double>(h, level, smooth != 0);
+	HIP_TRY(hipGetLastError());
+	return PGMG_OK;
+}
+
+int pgmg_v_cycle(pgmg_hier *h, int mode) {
+	if (!h) return fail(PGMG_ERR_ARG, "null hierarchy");
+	if (!h->ops_ready) return fail(PGMG_ERR_SETUP, "operators not set up (compute_rats)");
+	int rc = run_cycle(h, mode, false);
+	if (rc) return rc;
+	HIP_TRY(hipGetLastError());
+	return PGMG_OK;
+}
+
+int pgmg_l2_error_all(pgmg_hier *h, int level, double *out) {
+	int rc = check_level(h, level);
+	if (rc) return rc;
+	if (!out) return fail(PGMG_ERR_ARG, "null output");
+	return residual_norm(h, level, nullptr, out);
+}
+
+int pgmg_rhs_l2norm(pgmg_hier *h, int level, double *out) {
+	int rc = check_level(h, level);
+	if (rc) return rc;
+	if (!out) return fail(PGMG_ERR_ARG, "null output");
+	Level &L = h->lev[level];
+	const int n = ((L.v.m + 63) / 64) * ((L.v.n + 3) / 4) * L.v.o;
+	rc = ensure_partial(h, n);
+	if (rc) return rc;
+	launch_sumsq<double>(h->stream, L.v, L.r, h->partial);
+	launch_reduce(h->stream, h->partial, n, h->dscalar);
+	return read_scalar(h, out);
+}
+
+int pgmg_solve(pgmg_hier *h, double tol, int max_cycles, int *cycles, double *hist) {
+	if (!h) return fail(PGMG_ERR_ARG, "null hierarchy");
+	if (!h->ops_ready) return fail(PGMG_ERR_SETUP, "operators not set up (compute_rats)");
+	double bn = 0;
+	int rc = pgmg_rhs_l2norm(h, 0, &bn);
+	if (rc) return rc;
+	const bool mixed = h->opts.precision == PGMG_PREC_MIXED;
+	Level &T0 = h->lev[0];
+	int c = 0;
+	double e = 0;
+	for (;;) {
+		rc = residual_norm(h, 0, mixed ? T0.r32 : nullptr, &e);
+		if (rc) return rc;
+		if (hist) hist[c] = e;
+		if (std::isnan(e)) { if (cycles) *cycles = c; return fail(PGMG_ERR_NAN, "NaN residual after %d cycles", c); }
+		if (sqrt(e / bn) < tol) break;
+		if (c >= max_cycles) { if (cycles) *cycles = c; return fail(PGMG_ERR_NOT_CONVERGED, "not converged in %d cycles", c); }
+		if (mixed) {
+			// defect correction: r32 holds (float)(b - A x); e32 = 0;
+			// one FP32 V-cycle; x += e32.
+			launch_fill<float>(h->stream, T0.x32, T0.v.npad, 0.0f);
+			rc = run_cycle(h, 0, true);
+			if (rc) return rc;
+			launch_add_f32_to_f64(h->stream, T0.v, T0.x32, T0.x);
+		} else {
+			rc = run_cycle(h, 0, false);
+			if (rc) return rc;
+		}
+		c++;
+	}
+	if (cycles) *cycles = c;
+	HIP_TRY(hipGetLastError());
+	return PGMG_OK;
+}
+
+int pgmg_synchronize(pgmg_hier *h) {
+	if (!h) return fail(PGMG_ERR_ARG, "null hierarchy");
+	HIP_TRY(hipStreamSynchronize(h->stream));
+	return PGMG_OK;
+}
+
+int pgmg_gs_count(const pgmg_hier *h, long long *count) {
+	if (!h || !count) return fail(PGMG_ERR_ARG, "null argument");
+	*count = h->gs_count;
+	return PGMG_OK;
+}
+
+int pgmg_prof_enable(pgmg_hier *h, int kernel_class, int level) {
+	if (!h) return fail(PGMG_ERR_ARG, "null hierarchy");
+	if (kernel_class < KC_NONE || kernel_class > KC_INTERP) return fail(PGMG_ERR_ARG, "bad kernel class");
+	prof_collect(h);
+	h->prof_class = kernel_class;
+	h->prof_level = level;
+	h->prof_launches = 0;
+	h->prof_points = 0;
+	h->prof_ms = 0;
+	return PGMG_OK;
+}
+
+int pgmg_prof_read(pgmg_hier *h, long long *launches, double *total_ms, long long *points) {
+	if (!h) return fail(PGMG_ERR_ARG, "null hierarchy");
+	prof_collect(h);
+	if (points) *points = h->prof_points;
+	if (launches) *launches = h->prof_launches;
+	if (total_ms) *total_ms = h->prof_ms;
+	return PGMG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+// pgmg_internal.hpp - device data layout and kernel launchers shared by the
+// host driver (pgmg_hier.hip) and the kernels (pgmg_kernels.hip).
+//
+// Level layout in HBM.  Every per-point array of a level (the 27 coefficient
+// planes, x, r, residual scratch) uses one padded index space: the grid plus
+// a zero halo, rows padded to a multiple of 16 elements.  Point (i,j,k),
+// -1 <= i <= m etc., lives at base + i + px*j + pxy*k with base = 2 + px + pxy,
+// so every even i sits at an even element offset and a pair (i, i+1), i even,
+// can be moved with one 16-byte (FP64) or 8-byte (FP32) vector access.
+// Halo entries are zero and stay zero, so a stencil at the grid edge reads
+// zeros instead of branching.  Coefficients are stored as 27 separate planes
+// (structure of arrays), plane e at A + e*npad, so that a wavefront reading
+// one coefficient for consecutive points reads contiguous memory.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+namespace pgmg {
+
+struct LevelView {
+	int m, n, o;       // grid points per axis
+	int px;            // row pitch (elements)
+	long long pxy;     // plane pitch (elements)
+	long long base;    // index of point (0,0,0)
+	long long npad;    // elements per array, halo included
+};
+
+inline LevelView make_level_view(int m, int n, int o) {
+	LevelView v;
+	v.m = m; v.n = n; v.o = o;
+	v.px = ((m + 4 + 15) / 16) * 16;
+	v.pxy = (long long)v.px * (n + 2);
+	v.base = 2 + v.px + v.pxy;
+	v.npad = v.pxy * (o + 2);
+	return v;
+}
+
+// Kernel classes for timing.
+enum { KC_NONE = 0, KC_GS = 1, KC_RESIDUAL = 2, KC_RESTRICT = 3, KC_INTERP = 4 };
+
+// ---- launchers (pgmg_kernels.hip).  All are asynchronous on `s`. ----
+// Gauss-Seidel, one colour class (ci,cj,ck) of one level.
+template <typename T> void launch_gs_colour(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int colour);
+// Forward Gauss-Seidel of the two colour classes (0,cj,ck) then (1,cj,ck)
+// in one launch, one wavefront per grid row.
+template <typename T> void launch_gs_pair(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int pair);
+// Gauss-Seidel on the hyperplane i+2j+4k = t.
+template <typename T> void launch_gs_plane(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int t);
+// nsweeps Gauss-Seidel sweeps of a small level by one workgroup.
+template <typename T> void launch_gs_small(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r,
+					   int nsweeps, int mode, int exact);
+// res = r - A x on interior points (res may be null), per-block sums of
+// res^2 into partial (may be null); returns the number of partial sums.
+template <typename T, typename TR> int launch_residual(hipStream_t s, const LevelView &L, const T *A, const T *x,
+						       const T *r, TR *res, double *partial);
+int residual_partial_count(const LevelView &L);
+// out[0] = sum of partial[0..n) in a fixed order.
+void launch_reduce(hipStream_t s, const double *partial, int n, double *out);
+// partial sums of r^2 (rhs_l2norm); returns count.
+template <typename T> int launch_sumsq(hipStream_t s, const LevelView &L, const T *r, double *partial);
+// rc = R res (gather form of the reference's scatter), xc = 0.
+template <typename T> void launch_restrict(hipStream_t s, const LevelView &F, const LevelView &C, const T *res, T *rc, T *xc);
+// xf += P xc.
+template <typename T> void launch_interp(hipStream_t s, const LevelView &C, const LevelView &F, const T *xc, T *xf);
+// Ac = P^T Af P / 8.
+void launch_rat(hipStream_t s, const LevelView &F, const LevelView &C, const double *Af, double *Ac);
+// Dense bottom LU.
+void launch_lu_assemble(hipStream_t s, const LevelView &L, const double *A, double *LU);
+void launch_lu_factor(hipStream_t s, int N, double *LU, int *ipiv, int *info);
+template <typename T> void launch_lu_solve(hipStream_t s, const LevelView &L, int N, const double *LU, const int *ipiv,
+					   const T *r, T *x);
+// Problem generation on the device.
+void launch_generate(hipStream_t s, const LevelView &L, int family, uint64_t seed, double *A, double *r);
+// Layout conversion between host point-major arrays (staged on the device)
+// and the padded level layout, for points [p0, p0+count).
+void launch_pack_vec(hipStream_t s, const LevelView &L, const double *compact, double *padded, long long p0, long long count);
+void launch_unpack_vec(hipStream_t s, const LevelView &L, const double *padded, double *compact, long long p0, long long count);
+void launch_pack_op(hipStream_t s, const LevelView &L, const double *compact27, double *A, long long p0, long long count);
+void launch_unpack_op(hipStream_t s, const LevelView &L, const double *A, double *compact27, long long p0, long long count);
+// Elementwise helpers.
+template <typename T> void launch_fill(hipStream_t s, T *p, long long n, T v);
+void launch_round_f32(hipStream_t s, const double *src, float *dst, long long n);
+void launch_add_f32_to_f64(hipStream_t s, const LevelView &L, const float *e, double *x);
+
+}  // namespace pgmg

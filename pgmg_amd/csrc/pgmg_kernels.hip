@@ -1,0 +1,907 @@
+// pgmg_kernels.hip - gfx950 kernels of the multigrid hot path.
+//
+// Built with -ffp-contract=off: every a*b+c rounds twice, as the reference's
+// x86-64 build does, so FP64 results reproduce the reference's bit for bit
+// where the operation order is the same.  All kernels are memory-bound
+// (0.225 flop/byte in FP64); none is a dense contraction, so no MFMA is used.
+#include "pgmg_internal.hpp"
+#include "../../include/pgmg_problems.h"
+
+namespace pgmg {
+
+#define HIPCHECK_LAUNCH() (void)hipGetLastError()
+
+// Minimum resident wavefronts per SIMD requested for the streaming stencil
+// kernels; it caps their register use so that enough wavefronts are in
+// flight to cover HBM latency.
+#ifndef PGMG_STREAM_WAVES
+#define PGMG_STREAM_WAVES 4
+#endif
+#define PGMG_STREAM_ATTR __attribute__((amdgpu_waves_per_eu(PGMG_STREAM_WAVES)))
+
+__device__ __forceinline__ long long idx3(const LevelView &L, int i, int j, int k) {
+	return L.base + i + (long long)L.px * j + L.pxy * k;
+}
+
+// The hardware deals workgroups to the 8 XCDs round-robin (workgroup b runs
+// on XCD b % 8), and each XCD has its own L2.  Kernels whose neighbouring
+// workgroups share input rows renumber them so that every XCD receives one
+// contiguous range of logical workgroups: x rows shared by adjacent grid rows
+// are then fetched into one L2 instead of eight.  A bijection of [0, G).
+__device__ __forceinline__ int xcd_block(int b, int G) {
+	const int q = G >> 3, rmd = G & 7, x = b & 7;
+	return x * q + min(x, rmd) + (b >> 3);
+}
+
+// Gauss-Seidel update of the point at padded index p (region.cc:369-438):
+// sum over the 26 off-centre entries in box order from zero, then
+// x = (r - Ax) * (1/a_centre) with the reciprocal taken first.
+template <typename T>
+__device__ __forceinline__ void gs_point(const LevelView &L, const T *__restrict__ A, T *__restrict__ x,
+					 const T *__restrict__ r, long long p) {
+	const long long px = L.px, pxy = L.pxy, np = L.npad;
+	T ax = (T)0;
+#pragma unroll
+	for (int e = 0; e < 27; e++) {
+		if (e == 13) continue;
+		const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
+		ax = ax + A[e * np + p] * x[p + di + dj * px + dk * pxy];
+	}
+	x[p] = (r[p] - ax) * ((T)1 / A[13 * np + p]);
+}
+
+// Residual r - sum over all 27 entries in box order (region.cc:1833-1842).
+template <typename T>
+__device__ __forceinline__ T residual_point(const LevelView &L, const T *__restrict__ A, const T *__restrict__ x,
+					    const T *__restrict__ r, long long p) {
+	const long long px = L.px, pxy = L.pxy, np = L.npad;
+	T ax = (T)0;
+#pragma unroll
+	for (int e = 0; e < 27; e++) {
+		const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
+		ax = ax + A[e * np + p] * x[p + di + dj * px + dk * pxy];
+	}
+	return r[p] - ax;
+}
+
+// ------------------------------------------------------------ smoothers
+
+// One colour class: points (2a+ci, 2b+cj, 2d+ck).  Points of one class never
+// share a stencil box, so all of them update in parallel.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gs_colour(LevelView L, const T *__restrict__ A, T *__restrict__ x,
+						   const T *__restrict__ r, int ci, int cj, int ck) {
+	const int i = 2 * (int)(blockIdx.x * 64 + threadIdx.x) + ci;
+	const int j = 2 * (int)(blockIdx.y * 4 + threadIdx.y) + cj;
+	const int k = 2 * (int)blockIdx.z + ck;
+	if (i >= L.m || j >= L.n || k >= L.o) return;
+	gs_point<T>(L, A, x, r, idx3(L, i, j, k));
+}
+
+template <typename T>
+void launch_gs_colour(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int colour) {
+	const int ci = colour & 1, cj = (colour >> 1) & 1, ck = (colour >> 2) & 1;
+	const int na = (L.m - ci + 1) / 2, nb = (L.n - cj + 1) / 2, nd = (L.o - ck + 1) / 2;
+	if (na <= 0 || nb <= 0 || nd <= 0) return;
+	dim3 grid((na + 63) / 64, (nb + 3) / 4, nd), block(64, 4, 1);
+	hipLaunchKernelGGL(k_gs_colour<T>, grid, block, 0, s, L, A, x, r, ci, cj, ck);
+	HIPCHECK_LAUNCH();
+}
+
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { typedef double2 type; };
+template <> struct Vec2<float> { typedef float2 type; };
+
+// Colour classes (0,cj,ck) and then (1,cj,ck) in one launch.  One wavefront
+// owns one grid row (j,k) with j = cj (mod 2), k = ck (mod 2); only that
+// row is both read and written by the launch, because every other row a
+// stencil reaches has the other parity in j or k.  The row is processed in
+// segments that advance by 126 points: lane l holds the pair (i0, i0+1),
+// i0 = s0 + 2l, read with one vector load per coefficient plane, so the
+// whole row is streamed once with no half-used cache lines.  Lanes 0..63
+// update the even point i0 (lane 63 is the lookahead: the first even point
+// of the next segment), then lanes 0..62 update the odd point i0+1, whose
+// two even neighbours are the lane's own new value and lane l+1's (shuffle).
+// The lookahead value is written at once and handed to lane 0 of the next
+// segment, which must not recompute it: by then lane 62's odd neighbour has
+// changed.  Per-point arithmetic and the colour order are those of
+// k_gs_colour, so the result is bit-identical.
+template <typename T>
+__global__ __launch_bounds__(256) PGMG_STREAM_ATTR void k_gs_pair(LevelView L, const T *__restrict__ A, T *__restrict__ x,
+						 const T *__restrict__ r, int cj, int ck) {
+	typedef typename Vec2<T>::type V2;
+	const int lane = threadIdx.x;
+	const int nb = (L.n - cj + 1) >> 1, nd = (L.o - ck + 1) >> 1;
+	const int q = xcd_block((int)blockIdx.x, (int)gridDim.x) * 4 + (int)threadIdx.y;
+	if (q >= nb * nd) return;  // uniform per wavefront
+	const int j = 2 * (q % nb) + cj, k = 2 * (q / nb) + ck;
+	const long long row = idx3(L, 0, j, k);
+	const long long px = L.px, pxy = L.pxy, np = L.npad;
+	T carry = (T)0;
+	for (int s0 = 0; s0 < L.m; s0 += 126) {
+		const int i0 = s0 + 2 * lane;
+		const bool live = i0 < L.m;
+		const bool doE = live && !(lane == 0 && s0 > 0);
+		const bool doO = lane < 63 && i0 + 1 < L.m;
+		const long long p = row + i0;
+		// Streamed row by row (dj,dk): the even point's sum is formed as the
+		// loads arrive; the odd point's terms before its own row are summed
+		// at once, its own-row coefficients are kept, and its 12 terms after
+		// the own row are kept as products, to be added in box order once
+		// the new even values exist.  Products are the same roundings the
+		// in-order sum would form.
+		T ax = (T)0, ay = (T)0, ac_e = (T)1, ac_o = (T)1, a12 = (T)0, a14 = (T)0, rx = (T)0, ry = (T)0;
+		T prod[12];
+		T xe = (T)0;
+		if (live) {
+			const V2 rr = *(const V2 *)(r + p);
+			rx = rr.x; ry = rr.y;
+#pragma unroll
+			for (int rq = 0; rq < 9; rq++) {
+				const int dj = rq % 3 - 1, dk = rq / 3 - 1;
+				const long long qq = p + dj * px + dk * pxy;
+				const V2 am = *(const V2 *)(A + (3 * rq) * np + p);
+				const V2 a0 = *(const V2 *)(A + (3 * rq + 1) * np + p);
+				const V2 ap = *(const V2 *)(A + (3 * rq + 2) * np + p);
+				const V2 xv = *(const V2 *)(x + qq);
+				const T xm = x[qq - 1];
+				const T xp = doO ? x[qq + 2] : (T)0;
+				if (rq == 4) {
+					ax = ax + am.x * xm;
+					ax = ax + ap.x * xv.y;
+					ac_e = a0.x; ac_o = a0.y; a12 = am.y; a14 = ap.y;
+				} else {
+					ax = ax + am.x * xm;
+					ax = ax + a0.x * xv.x;
+					ax = ax + ap.x * xv.y;
+					if (rq < 4) {
+						ay = ay + am.y * xv.x;
+						ay = ay + a0.y * xv.y;
+						ay = ay + ap.y * xp;
+					} else {
+						prod[3 * (rq - 5)] = am.y * xv.x;
+						prod[3 * (rq - 5) + 1] = a0.y * xv.y;
+						prod[3 * (rq - 5) + 2] = ap.y * xp;
+					}
+				}
+			}
+		}
+		if (doE) {
+			// even point i0: every neighbour still held its old value
+			xe = (rx - ax) * ((T)1 / ac_e);
+			x[p] = xe;
+		} else if (live) {
+			xe = carry;  // lane 0: computed by lane 63 of the previous segment
+		}
+		// Shuffles run with every lane active; lanes past the row end hold 0.
+		const T xe_next = __shfl_down(xe, 1, 64);
+		if (doO) {
+			// odd point i0+1: its even neighbours in the own row are new
+			ay = ay + a12 * xe;
+			ay = ay + a14 * xe_next;
+#pragma unroll
+			for (int q2 = 0; q2 < 12; q2++) ay = ay + prod[q2];
+			x[p + 1] = (ry - ay) * ((T)1 / ac_o);
+		}
+		carry = __shfl(xe, 63, 64);
+	}
+}
+
+template <typename T>
+void launch_gs_pair(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int pair) {
+	const int cj = pair & 1, ck = (pair >> 1) & 1;
+	const int nb = (L.n - cj + 1) >> 1, nd = (L.o - ck + 1) >> 1;
+	if (nb <= 0 || nd <= 0) return;
+	const int rows = nb * nd;
+	hipLaunchKernelGGL(k_gs_pair<T>, dim3((rows + 3) / 4), dim3(64, 4, 1), 0, s, L, A, x, r, cj, ck);
+	HIPCHECK_LAUNCH();
+}
+
+// Hyperplane i + 2j + 4k = t.  Every stencil offset (di,dj,dk) != 0 has
+// di + 2dj + 4dk != 0, negative exactly for the offsets that precede the
+// centre in lexicographic order; sweeping t upwards therefore gives each
+// point the same mix of new and old neighbours as the lexicographic sweep of
+// region.cc:367-446, and t downwards the reverse sweep of region.cc:463-537.
+// Thread (jj, kk) takes row k = klo + kk, j = jlo(k) + jj, i = t - 2j - 4k.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gs_plane(LevelView L, const T *__restrict__ A, T *__restrict__ x,
+						  const T *__restrict__ r, int t, int klo) {
+	const int k = klo + (int)blockIdx.y;
+	const int sres = t - 4 * k;  // i + 2j
+	const int jlo = max(0, (sres - (L.m - 1) + 1) >> 1);
+	const int jhi = min(L.n - 1, sres >> 1);
+	const int j = jlo + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (j > jhi) return;
+	const int i = sres - 2 * j;
+	gs_point<T>(L, A, x, r, idx3(L, i, j, k));
+}
+
+template <typename T>
+void launch_gs_plane(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int t) {
+	const int maxi = L.m - 1, maxj = L.n - 1;
+	int klo = t - maxi - 2 * maxj;
+	klo = klo <= 0 ? 0 : (klo + 3) / 4;
+	const int khi = min(L.o - 1, t / 4);
+	if (khi < klo) return;
+	const int jspan = min(L.n, (L.m + 1) / 2 + 1);
+	const int bs = jspan >= 128 ? 128 : 64;
+	dim3 grid((jspan + bs - 1) / bs, khi - klo + 1, 1), block(bs, 1, 1);
+	hipLaunchKernelGGL(k_gs_plane<T>, grid, block, 0, s, L, A, x, r, t, klo);
+	HIPCHECK_LAUNCH();
+}
+
+// A small level smoothed by one workgroup: all sweeps of one call in one
+// launch, ordered by workgroup barriers.  Exact mode walks the hyperplanes;
+// colour mode the eight classes.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_gs_small(LevelView L, const T *__restrict__ A, T *__restrict__ x,
+						   const T *__restrict__ r, int nsweeps, int mode, int exact) {
+	const int nt = blockDim.x, tid = threadIdx.x;
+	const int tmax = (L.m - 1) + 2 * (L.n - 1) + 4 * (L.o - 1);
+	const int npair = L.n * L.o;
+	for (int sw = 0; sw < nsweeps; sw++) {
+		for (int pass = 0; pass < (mode ? 2 : 1); pass++) {
+			if (exact) {
+				for (int tt = 0; tt <= tmax; tt++) {
+					const int t = pass ? tmax - tt : tt;
+					for (int q = tid; q < npair; q += nt) {
+						const int j = q % L.n, k = q / L.n;
+						const int i = t - 2 * j - 4 * k;
+						if (i >= 0 && i < L.m) gs_point<T>(L, A, x, r, idx3(L, i, j, k));
+					}
+					__syncthreads();
+				}
+			} else {
+				for (int cc = 0; cc < 8; cc++) {
+					const int c = pass ? 7 - cc : cc;
+					const int ci = c & 1, cj = (c >> 1) & 1, ck = (c >> 2) & 1;
+					const int na = (L.m - ci + 1) / 2, nb = (L.n - cj + 1) / 2, nd = (L.o - ck + 1) / 2;
+					const int tot = na * nb * nd;
+					for (int q = tid; q < tot; q += nt) {
+						const int a = q % na, b = (q / na) % nb, d = q / (na * nb);
+						gs_point<T>(L, A, x, r, idx3(L, 2 * a + ci, 2 * b + cj, 2 * d + ck));
+					}
+					__syncthreads();
+				}
+			}
+		}
+	}
+}
+
+template <typename T>
+void launch_gs_small(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int nsweeps, int mode, int exact) {
+	hipLaunchKernelGGL(k_gs_small<T>, dim3(1), dim3(1024), 0, s, L, A, x, r, nsweeps, mode, exact);
+	HIPCHECK_LAUNCH();
+}
+
+// ------------------------------------------------------ residual, norms
+
+// Deterministic block sum of 256 values (4 wavefronts of 64).
+__device__ __forceinline__ double block_sum_256(double v) {
+	__shared__ double red[256];
+	const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+	red[tid] = v;
+	__syncthreads();
+	for (int w = 128; w > 0; w >>= 1) {
+		if (tid < w) red[tid] = red[tid] + red[tid + w];
+		__syncthreads();
+	}
+	return red[0];
+}
+
+// Residual r - A x and per-block sums of its square.  One wavefront per grid
+// row, each lane a pair of points read with vector loads (as in k_gs_pair);
+// blocks of 4 rows, renumbered per XCD.  res (may be null) receives the
+// residual in type TR, partial (may be null) one sum per block.
+template <typename T, typename TR>
+__global__ __launch_bounds__(256) PGMG_STREAM_ATTR void k_residual(LevelView L, const T *__restrict__ A, const T *__restrict__ x,
+						  const T *__restrict__ r, TR *__restrict__ res, double *__restrict__ partial) {
+	typedef typename Vec2<T>::type V2;
+	typedef typename Vec2<TR>::type R2;
+	const int lane = threadIdx.x;
+	const int q = xcd_block((int)blockIdx.x, (int)gridDim.x) * 4 + (int)threadIdx.y;
+	const long long px = L.px, pxy = L.pxy, np = L.npad;
+	double v2 = 0.0;
+	if (q < L.n * L.o) {
+		const int j = q % L.n, k = q / L.n;
+		const long long row = idx3(L, 0, j, k);
+		for (int s0 = 0; s0 < L.m; s0 += 128) {
+			const int i0 = s0 + 2 * lane;
+			if (i0 >= L.m) continue;
+			const bool two = i0 + 1 < L.m;
+			const long long p = row + i0;
+			const V2 rr = *(const V2 *)(r + p);
+			T ax = (T)0, ay = (T)0;
+#pragma unroll
+			for (int rq = 0; rq < 9; rq++) {
+				const int dj = rq % 3 - 1, dk = rq / 3 - 1;
+				const long long qq = p + dj * px + dk * pxy;
+				const V2 am = *(const V2 *)(A + (3 * rq) * np + p);
+				const V2 a0 = *(const V2 *)(A + (3 * rq + 1) * np + p);
+				const V2 ap = *(const V2 *)(A + (3 * rq + 2) * np + p);
+				const V2 xv = *(const V2 *)(x + qq);
+				const T xm = x[qq - 1];
+				const T xp = two ? x[qq + 2] : (T)0;
+				ax = ax + am.x * xm;
+				ax = ax + a0.x * xv.x;
+				ax = ax + ap.x * xv.y;
+				ay = ay + am.y * xv.x;
+				ay = ay + a0.y * xv.y;
+				ay = ay + ap.y * xp;
+			}
+			const T rv0 = rr.x - ax, rv1 = rr.y - ay;
+			if (res) {
+				if (two) { R2 o; o.x = (TR)rv0; o.y = (TR)rv1; *(R2 *)(res + p) = o; }
+				else res[p] = (TR)rv0;
+			}
+			v2 = v2 + (double)rv0 * (double)rv0;
+			if (two) v2 = v2 + (double)rv1 * (double)rv1;
+		}
+	}
+	if (partial) {
+		const double sblk = block_sum_256(v2);
+		if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.x] = sblk;
+	}
+}
+
+int residual_partial_count(const LevelView &L) {
+	return (L.n * L.o + 3) / 4;
+}
+
+template <typename T, typename TR>
+int launch_residual(hipStream_t s, const LevelView &L, const T *A, const T *x, const T *r, TR *res, double *partial) {
+	const int G = residual_partial_count(L);
+	hipLaunchKernelGGL((k_residual<T, TR>), dim3(G), dim3(64, 4, 1), 0, s, L, A, x, r, res, partial);
+	HIPCHECK_LAUNCH();
+	return G;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sumsq(LevelView L, const T *__restrict__ r, double *__restrict__ partial) {
+	const int i = (int)(blockIdx.x * 64 + threadIdx.x), j = (int)(blockIdx.y * 4 + threadIdx.y), k = (int)blockIdx.z;
+	double v2 = 0.0;
+	if (i < L.m && j < L.n) {
+		const double v = (double)r[idx3(L, i, j, k)];
+		v2 = v * v;
+	}
+	const double sblk = block_sum_256(v2);
+	if (threadIdx.x == 0 && threadIdx.y == 0)
+		partial[blockIdx.x + gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z)] = sblk;
+}
+
+template <typename T>
+int launch_sumsq(hipStream_t s, const LevelView &L, const T *r, double *partial) {
+	dim3 grid((L.m + 63) / 64, (L.n + 3) / 4, L.o), block(64, 4, 1);
+	hipLaunchKernelGGL(k_sumsq<T>, grid, block, 0, s, L, r, partial);
+	HIPCHECK_LAUNCH();
+	return ((L.m + 63) / 64) * ((L.n + 3) / 4) * L.o;
+}
+
+// Fixed-order reduction: thread t sums partial[t], partial[t+1024], ... in
+// order, then a fixed binary tree.
+__global__ __launch_bounds__(1024) void k_reduce(const double *__restrict__ partial, int n, double *__restrict__ out) {
+	__shared__ double red[1024];
+	double s = 0.0;
+	for (int q = threadIdx.x; q < n; q += 1024) s = s + partial[q];
+	red[threadIdx.x] = s;
+	__syncthreads();
+	for (int w = 512; w > 0; w >>= 1) {
+		if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) out[0] = red[0];
+}
+
+void launch_reduce(hipStream_t s, const double *partial, int n, double *out) {
+	hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, partial, n, out);
+	HIPCHECK_LAUNCH();
+}
+
+// ------------------------------------------------------------ transfers
+
+// Restriction, gather form.  The reference scatters: it scans the fine grid
+// in 2x2x2 cubes in (k,j,i) order and adds each fine residual, scaled by
+// 0.125*wx*wy*wz, into up to 8 coarse points (region.cc:665-746, 818-912).
+// Per axis of fine size M, cube c holds fine 2c (local 0) and 2c+1 (local 1);
+// the last cube is c = (M-1)>>1, and for even M its local 1 is injected into
+// coarse c+1 alone.  Coarse point I therefore receives, per axis, from
+// (cube I-1, local 1), (cube I, local 0) and (cube I, local 1), and this
+// kernel visits them in the reference's order: cube (k,j,i) first, then the
+// local position (k,j,i), so every coarse sum is formed in the same order
+// and rounds identically.  The reference's restrict_partial tests the x
+// injection flag for the y split of local (0,1,1) in boundary cubes
+// (region.cc:888-895); that is reproduced.  Also clears coarse x
+// (multigrid.cc:219, region.cc:1679-1682).
+template <typename T>
+__global__ __launch_bounds__(256) void k_restrict(LevelView F, LevelView C, const T *__restrict__ res,
+						  T *__restrict__ rc, T *__restrict__ xc) {
+	const int I = (int)(blockIdx.x * 64 + threadIdx.x), J = (int)(blockIdx.y * 4 + threadIdx.y), K = (int)blockIdx.z;
+	if (I >= C.m || J >= C.n) return;
+	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
+	const bool evx = !(F.m & 1), evy = !(F.n & 1), evz = !(F.o & 1);
+	T acc = (T)0;
+	for (int cz = 0; cz < 2; cz++) {
+		const int kc = K - 1 + cz;
+		if (kc < 0 || kc > lastz) continue;
+		const bool sz = evz && kc == lastz;
+		for (int cy = 0; cy < 2; cy++) {
+			const int jc = J - 1 + cy;
+			if (jc < 0 || jc > lasty) continue;
+			const bool sy = evy && jc == lasty;
+			for (int cx = 0; cx < 2; cx++) {
+				const int ic = I - 1 + cx;
+				if (ic < 0 || ic > lastx) continue;
+				const bool sx = evx && ic == lastx;
+				const bool partial = ic == lastx || jc == lasty || kc == lastz;
+				for (int lz = cz ? 0 : 1; lz < 2; lz++) {
+					const int fk = 2 * kc + lz;
+					if (fk >= F.o) continue;
+					T wz;
+					if (lz == 0) wz = (T)1;
+					else if (cz == 0) wz = sz ? (T)1 : (T)0.5;
+					else { if (sz) continue; wz = (T)0.5; }
+					for (int ly = cy ? 0 : 1; ly < 2; ly++) {
+						const int fj = 2 * jc + ly;
+						if (fj >= F.n) continue;
+						for (int lx = cx ? 0 : 1; lx < 2; lx++) {
+							const int fi = 2 * ic + lx;
+							if (fi >= F.m) continue;
+							const bool syq = (partial && lx == 0 && ly == 1 && lz == 1) ? sx : sy;
+							T wy, wx;
+							if (ly == 0) wy = (T)1;
+							else if (cy == 0) wy = syq ? (T)1 : (T)0.5;
+							else { if (syq) continue; wy = (T)0.5; }
+							if (lx == 0) wx = (T)1;
+							else if (cx == 0) wx = sx ? (T)1 : (T)0.5;
+							else { if (sx) continue; wx = (T)0.5; }
+							const T w = (T)0.125 * wx * wy * wz;
+							acc = acc + w * res[idx3(F, fi, fj, fk)];
+						}
+					}
+				}
+			}
+		}
+	}
+	const long long pc = idx3(C, I, J, K);
+	rc[pc] = acc;
+	xc[pc] = (T)0;
+}
+
+template <typename T>
+void launch_restrict(hipStream_t s, const LevelView &F, const LevelView &C, const T *res, T *rc, T *xc) {
+	dim3 grid((C.m + 63) / 64, (C.n + 3) / 4, C.o), block(64, 4, 1);
+	hipLaunchKernelGGL(k_restrict<T>, grid, block, 0, s, F, C, res, rc, xc);
+	HIPCHECK_LAUNCH();
+}
+
+// Interpolation x_f += P x_c, one thread per fine point.  The fine point's
+// cube and local position select the term; interior cubes use
+// interpolate_full's association order (region.cc:764-771), cubes at the
+// last position of any axis interpolate_partial's (region.cc:795-809).
+template <typename T>
+__global__ __launch_bounds__(256) void k_interp(LevelView C, LevelView F, const T *__restrict__ xc, T *__restrict__ xf) {
+	const int i = (int)(blockIdx.x * 64 + threadIdx.x), j = (int)(blockIdx.y * 4 + threadIdx.y), k = (int)blockIdx.z;
+	if (i >= F.m || j >= F.n) return;
+	const int ic = i >> 1, jc = j >> 1, kc = k >> 1, li = i & 1, lj = j & 1, lk = k & 1;
+	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
+	const int i1 = ic + 1 < C.m ? ic + 1 : ic, j1 = jc + 1 < C.n ? jc + 1 : jc, k1 = kc + 1 < C.o ? kc + 1 : kc;
+	const T h = (T)0.5, q = (T)0.25, e = (T)0.125;
+	const T v0 = xc[idx3(C, ic, jc, kc)], v1 = xc[idx3(C, i1, jc, kc)];
+	const T v2 = xc[idx3(C, ic, j1, kc)], v3 = xc[idx3(C, i1, j1, kc)];
+	const T v4 = xc[idx3(C, ic, jc, k1)], v5 = xc[idx3(C, i1, jc, k1)];
+	const T v6 = xc[idx3(C, ic, j1, k1)], v7 = xc[idx3(C, i1, j1, k1)];
+	const int pos = li + 2 * lj + 4 * lk;
+	T val;
+	if (ic < lastx && jc < lasty && kc < lastz) {
+		const T t1 = v0 + v1;
+		switch (pos) {
+		case 0: val = v0; break;
+		case 1: val = h * t1; break;
+		case 2: val = h * (v0 + v2); break;
+		case 3: val = q * ((t1 + v2) + v3); break;
+		case 4: val = h * (v0 + v4); break;
+		case 5: val = q * (t1 + (v4 + v5)); break;
+		case 6: val = q * (((v0 + v2) + v4) + v6); break;
+		default: val = e * (((((t1 + v2) + v3) + (v4 + v5)) + v6) + v7); break;
+		}
+	} else {
+		const bool xs = (ic == lastx) && !(F.m & 1), ys = (jc == lasty) && !(F.n & 1), zs = (kc == lastz) && !(F.o & 1);
+		switch (pos) {
+		case 0: val = v0; break;
+		case 1: val = xs ? v1 : h * (v0 + v1); break;
+		case 2: val = ys ? v2 : h * (v0 + v2); break;
+		case 4: val = zs ? v4 : h * (v0 + v4); break;
+		case 5: val = zs ? (xs ? v5 : h * (v4 + v5)) : (xs ? h * (v1 + v5) : q * (((v0 + v1) + v4) + v5)); break;
+		case 6: val = zs ? (ys ? v6 : h * (v4 + v6)) : (ys ? h * (v2 + v6) : q * (((v0 + v2) + v4) + v6)); break;
+		default: {
+			const T t2 = ys ? (xs ? v3 : h * (v2 + v3)) : (xs ? h * (v1 + v3) : q * (((v0 + v1) + v2) + v3));
+			if (pos == 3) { val = t2; break; }
+			const T t3 = ys ? (xs ? v7 : h * (v6 + v7)) : (xs ? h * (v5 + v7) : q * (((v4 + v5) + v6) + v7));
+			val = zs ? t3 : h * (t3 + t2);
+		}
+		}
+	}
+	const long long p = idx3(F, i, j, k);
+	xf[p] = xf[p] + val;
+}
+
+template <typename T>
+void launch_interp(hipStream_t s, const LevelView &C, const LevelView &F, const T *xc, T *xf) {
+	dim3 grid((F.m + 63) / 64, (F.n + 3) / 4, F.o), block(64, 4, 1);
+	hipLaunchKernelGGL(k_interp<T>, grid, block, 0, s, C, F, xc, xf);
+	HIPCHECK_LAUNCH();
+}
+
+// ------------------------------------------------------------------ RAT
+
+// Support of interpolation column J on an axis of fine size M: fine range
+// [lo,hi) and the fine index c that coarse J sits on (igrid_map,
+// region.cc:1243-1255).
+__device__ __forceinline__ void psupport(int J, int M, int &lo, int &hi, int &c) {
+	const int j2 = 2 * J;
+	if (M & 1) { lo = j2 == 0 ? 0 : j2 - 1; hi = j2 == M - 1 ? M : j2 + 2; c = j2; }
+	else if (j2 == M) { lo = M - 1; hi = M; c = M - 1; }
+	else { lo = j2 == 0 ? 0 : j2 - 1; hi = j2 == M - 2 ? M - 1 : j2 + 2; c = j2; }
+}
+
+// Galerkin coefficient Ac(I, I+d) for one coarse point I and one entry d.
+// The reference (region.cc:1008-1192) takes each fine point f in cube-scan
+// order, forms At[J] = sum_g 0.125*Af(f,g)*P(g,J) over g in f's box in
+// (k,j,i) order (A_red), scales At by f's restriction weight (A_mul/A_sca)
+// and adds it to the coarse rows of f (A_add).  Here the sum for one (I,J)
+// visits the same f in the same order, and forms each inner sum the same way.
+__global__ __launch_bounds__(256) void k_rat(LevelView F, LevelView C, const double *__restrict__ Af,
+					     double *__restrict__ Ac) {
+	const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	const long long ncp = (long long)C.m * C.n * C.o;
+	if (tid >= ncp * 27) return;
+	const int eJ = (int)(tid % 27);
+	const long long cp = tid / 27;
+	const int I = (int)(cp % C.m), Jy = (int)((cp / C.m) % C.n), K = (int)(cp / ((long long)C.m * C.n));
+	const int dI = eJ % 3 - 1, dJ = (eJ / 3) % 3 - 1, dK = eJ / 9 - 1;
+	const int JI = I + dI, JJ = Jy + dJ, JK = K + dK;
+	double acc = 0.0;
+	if (JI >= 0 && JI < C.m && JJ >= 0 && JJ < C.n && JK >= 0 && JK < C.o) {
+		int plx, phx, pcx, ply, phy, pcy, plz, phz, pcz;
+		psupport(JI, F.m, plx, phx, pcx);
+		psupport(JJ, F.n, ply, phy, pcy);
+		psupport(JK, F.o, plz, phz, pcz);
+		const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
+		const bool evx = !(F.m & 1), evy = !(F.n & 1), evz = !(F.o & 1);
+		const long long np = F.npad;
+		for (int cz = 0; cz < 2; cz++) {
+			const int kc = K - 1 + cz;
+			if (kc < 0 || kc > lastz) continue;
+			const bool sz = evz && kc == lastz;
+			for (int cy = 0; cy < 2; cy++) {
+				const int jc = Jy - 1 + cy;
+				if (jc < 0 || jc > lasty) continue;
+				const bool sy = evy && jc == lasty;
+				for (int cx = 0; cx < 2; cx++) {
+					const int ic = I - 1 + cx;
+					if (ic < 0 || ic > lastx) continue;
+					const bool sx = evx && ic == lastx;
+					for (int lz = cz ? 0 : 1; lz < 2; lz++) {
+						const int fk = 2 * kc + lz;
+						if (fk >= F.o) continue;
+						double wz;
+						if (lz == 0) wz = 1.0;
+						else if (cz == 0) wz = sz ? 1.0 : 0.5;
+						else { if (sz) continue; wz = 0.5; }
+						const int gzl = max(plz, fk - 1), gzh = min(phz, min(fk + 2, F.o));
+						if (gzl >= gzh) continue;
+						for (int ly = cy ? 0 : 1; ly < 2; ly++) {
+							const int fj = 2 * jc + ly;
+							if (fj >= F.n) continue;
+							double wy;
+							if (ly == 0) wy = 1.0;
+							else if (cy == 0) wy = sy ? 1.0 : 0.5;
+							else { if (sy) continue; wy = 0.5; }
+							const int gyl = max(ply, fj - 1), gyh = min(phy, min(fj + 2, F.n));
+							if (gyl >= gyh) continue;
+							for (int lx = cx ? 0 : 1; lx < 2; lx++) {
+								const int fi = 2 * ic + lx;
+								if (fi >= F.m) continue;
+								double wx;
+								if (lx == 0) wx = 1.0;
+								else if (cx == 0) wx = sx ? 1.0 : 0.5;
+								else { if (sx) continue; wx = 0.5; }
+								const int gxl = max(plx, fi - 1), gxh = min(phx, min(fi + 2, F.m));
+								if (gxl >= gxh) continue;
+								const long long pf = idx3(F, fi, fj, fk);
+								double inner = 0.0;
+								for (int kk = gzl; kk < gzh; kk++) {
+									const double kfac = kk == pcz ? 1.0 : 0.5;
+									for (int jj = gyl; jj < gyh; jj++) {
+										const double jkfac = jj == pcy ? kfac : 0.5 * kfac;
+										for (int ii = gxl; ii < gxh; ii++) {
+											const int ef = (kk - fk + 1) * 9 + (jj - fj + 1) * 3 + (ii - fi + 1);
+											inner = inner + (0.125 * Af[ef * np + pf]) * (ii == pcx ? jkfac : 0.5 * jkfac);
+										}
+									}
+								}
+								const double wr = wx * wy * wz;
+								acc = acc + inner * wr;
+							}
+						}
+					}
+				}
+			}
+		}
+	}
+	Ac[eJ * C.npad + idx3(C, I, Jy, K)] = acc;
+}
+
+void launch_rat(hipStream_t s, const LevelView &F, const LevelView &C, const double *Af, double *Ac) {
+	const long long total = (long long)C.m * C.n * C.o * 27;
+	hipLaunchKernelGGL(k_rat, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, F, C, Af, Ac);
+	HIPCHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------- bottom LU
+
+// Dense column-major matrix of the bottom level, row = point, column =
+// neighbour (region_tr.cc:486-504).  The caller zero-fills LU first.
+__global__ void k_lu_assemble(LevelView L, const double *__restrict__ A, double *__restrict__ LU) {
+	const int N = L.m * L.n * L.o;
+	const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (tid >= (long long)N * 27) return;
+	const int e = (int)(tid % 27), p = (int)(tid / 27);
+	const int i = p % L.m, j = (p / L.m) % L.n, k = p / (L.m * L.n);
+	const int qi = i + e % 3 - 1, qj = j + (e / 3) % 3 - 1, qk = k + e / 9 - 1;
+	if (qi < 0 || qi >= L.m || qj < 0 || qj >= L.n || qk < 0 || qk >= L.o) return;
+	const int q = qi + L.m * (qj + L.n * qk);
+	LU[p + (long long)q * N] = A[e * L.npad + idx3(L, i, j, k)];
+}
+
+void launch_lu_assemble(hipStream_t s, const LevelView &L, const double *A, double *LU) {
+	const long long total = (long long)L.m * L.n * L.o * 27;
+	hipLaunchKernelGGL(k_lu_assemble, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, L, A, LU);
+	HIPCHECK_LAUNCH();
+}
+
+// Right-looking LU with partial pivoting by one workgroup: pivot = first
+// row of largest modulus (dgetrf's rule, region_tr.cc:518), column scaled
+// by the pivot's reciprocal, rank-1 update.  info = first zero pivot (1-based)
+// and info[1] = 1 when the factor fails the test of region_tr.cc:526-533.
+__global__ __launch_bounds__(1024) void k_lu_factor(int N, double *__restrict__ LU, int *__restrict__ ipiv,
+						    int *__restrict__ info) {
+	__shared__ double bv[1024];
+	__shared__ int bi[1024];
+	__shared__ int s_piv, s_info, s_nz;
+	const int tid = threadIdx.x, nt = blockDim.x;
+	if (tid == 0) s_info = 0;
+	for (int c = 0; c < N; c++) {
+		double best = -1.0;
+		int bidx = N;
+		for (int r = c + tid; r < N; r += nt) {
+			const double v = fabs(LU[r + (long long)c * N]);
+			if (v > best) { best = v; bidx = r; }
+		}
+		bv[tid] = best; bi[tid] = bidx;
+		__syncthreads();
+		for (int w = nt / 2; w > 0; w >>= 1) {
+			if (tid < w) {
+				const double ov = bv[tid + w];
+				const int oi = bi[tid + w];
+				if (ov > bv[tid] || (ov == bv[tid] && oi < bi[tid])) { bv[tid] = ov; bi[tid] = oi; }
+			}
+			__syncthreads();
+		}
+		if (tid == 0) {
+			// The sequential rule keeps the diagonal unless another entry
+			// is strictly larger; a NaN column keeps the diagonal.
+			int piv = bi[0];
+			if (!(bv[0] > fabs(LU[c + (long long)c * N])) || piv >= N) piv = c;
+			s_piv = piv;
+			s_nz = LU[piv + (long long)c * N] != 0.0;
+			ipiv[c] = piv;
+		}
+		__syncthreads();
+		const int piv = s_piv;
+		// Read once by thread 0 before any swap, so every thread takes the
+		// same branch and reaches the same barriers.
+		const bool nz = s_nz != 0;
+		if (nz) {
+			if (piv != c) {
+				for (int jcol = tid; jcol < N; jcol += nt) {
+					const double t = LU[c + (long long)jcol * N];
+					LU[c + (long long)jcol * N] = LU[piv + (long long)jcol * N];
+					LU[piv + (long long)jcol * N] = t;
+				}
+			}
+			__syncthreads();
+			const double rp = 1.0 / LU[c + (long long)c * N];
+			for (int r = c + 1 + tid; r < N; r += nt) LU[r + (long long)c * N] *= rp;
+		} else if (tid == 0 && s_info == 0) {
+			s_info = c + 1;
+		}
+		__syncthreads();
+		const int rem = N - c - 1;
+		const long long tot = (long long)rem * rem;
+		for (long long q = tid; q < tot; q += nt) {
+			const int r = c + 1 + (int)(q % rem), jcol = c + 1 + (int)(q / rem);
+			const double u = LU[c + (long long)jcol * N];
+			if (u != 0.0) LU[r + (long long)jcol * N] -= LU[r + (long long)c * N] * u;
+		}
+		__syncthreads();
+	}
+	if (tid == 0) {
+		info[0] = s_info;
+		const double hi = fabs(LU[0]), lo = fabs(LU[(long long)N * N - 1]);
+		info[1] = (s_info != 0 || hi == 0.0 || lo / hi < 2.220446049250313e-16 * 1e4) ? 1 : 0;
+	}
+}
+
+void launch_lu_factor(hipStream_t s, int N, double *LU, int *ipiv, int *info) {
+	hipLaunchKernelGGL(k_lu_factor, dim3(1), dim3(1024), 0, s, N, LU, ipiv, info);
+	HIPCHECK_LAUNCH();
+}
+
+// dgetrs('N') on one workgroup (region_tr.cc:544-555): interchanges, then
+// column-oriented forward and back substitution in FP64, one barrier per
+// column.  r and x are in the level layout and type T.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_lu_solve(LevelView L, int N, const double *__restrict__ LU,
+						   const int *__restrict__ ipiv, const T *__restrict__ r, T *__restrict__ x) {
+	__shared__ double xs[1024];
+	const int tid = threadIdx.x, nt = blockDim.x;
+	for (int q = tid; q < N; q += nt) {
+		const int i = q % L.m, j = (q / L.m) % L.n, k = q / (L.m * L.n);
+		xs[q] = (double)r[idx3(L, i, j, k)];
+	}
+	__syncthreads();
+	if (tid == 0) {
+		for (int c = 0; c < N; c++) {
+			const int p = ipiv[c];
+			if (p != c) { const double t = xs[c]; xs[c] = xs[p]; xs[p] = t; }
+		}
+	}
+	__syncthreads();
+	for (int c = 0; c < N; c++) {
+		const double v = xs[c];
+		if (v != 0.0)
+			for (int rr = c + 1 + tid; rr < N; rr += nt) xs[rr] -= v * LU[rr + (long long)c * N];
+		__syncthreads();
+	}
+	for (int c = N - 1; c >= 0; c--) {
+		const double v = xs[c] / LU[c + (long long)c * N];
+		__syncthreads();
+		if (tid == 0) xs[c] = v;
+		if (v != 0.0)
+			for (int rr = tid; rr < c; rr += nt) xs[rr] -= v * LU[rr + (long long)c * N];
+		__syncthreads();
+	}
+	for (int q = tid; q < N; q += nt) {
+		const int i = q % L.m, j = (q / L.m) % L.n, k = q / (L.m * L.n);
+		x[idx3(L, i, j, k)] = (T)xs[q];
+	}
+}
+
+template <typename T>
+void launch_lu_solve(hipStream_t s, const LevelView &L, int N, const double *LU, const int *ipiv, const T *r, T *x) {
+	hipLaunchKernelGGL(k_lu_solve<T>, dim3(1), dim3(1024), 0, s, L, N, LU, ipiv, r, x);
+	HIPCHECK_LAUNCH();
+}
+
+// ------------------------------------------------- generation, layout
+
+__global__ __launch_bounds__(256) void k_generate(LevelView L, int family, uint64_t seed, double *__restrict__ A,
+						  double *__restrict__ r) {
+	const int i = (int)(blockIdx.x * 64 + threadIdx.x), j = (int)(blockIdx.y * 4 + threadIdx.y), k = (int)blockIdx.z;
+	if (i >= L.m || j >= L.n) return;
+	double row[27];
+	pgmg_row27(family, seed, L.m, L.n, L.o, i, j, k, row);
+	const long long p = idx3(L, i, j, k);
+#pragma unroll
+	for (int e = 0; e < 27; e++) A[e * L.npad + p] = row[e];
+	r[p] = pgmg_rhs(seed, L.m, L.n, L.o, i, j, k);
+}
+
+void launch_generate(hipStream_t s, const LevelView &L, int family, uint64_t seed, double *A, double *r) {
+	dim3 grid((L.m + 63) / 64, (L.n + 3) / 4, L.o), block(64, 4, 1);
+	hipLaunchKernelGGL(k_generate, grid, block, 0, s, L, family, seed, A, r);
+	HIPCHECK_LAUNCH();
+}
+
+__device__ __forceinline__ long long compact_to_padded(const LevelView &L, long long p) {
+	const int i = (int)(p % L.m), j = (int)((p / L.m) % L.n), k = (int)(p / ((long long)L.m * L.n));
+	return idx3(L, i, j, k);
+}
+
+__global__ void k_pack_vec(LevelView L, const double *__restrict__ c, double *__restrict__ d, long long p0, long long n) {
+	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < n) d[compact_to_padded(L, p0 + t)] = c[t];
+}
+__global__ void k_unpack_vec(LevelView L, const double *__restrict__ d, double *__restrict__ c, long long p0, long long n) {
+	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < n) c[t] = d[compact_to_padded(L, p0 + t)];
+}
+__global__ void k_pack_op(LevelView L, const double *__restrict__ c, double *__restrict__ A, long long p0, long long n) {
+	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n * 27) return;
+	const int e = (int)(t % 27);
+	const long long p = t / 27;
+	A[e * L.npad + compact_to_padded(L, p0 + p)] = c[t];
+}
+__global__ void k_unpack_op(LevelView L, const double *__restrict__ A, double *__restrict__ c, long long p0, long long n) {
+	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n * 27) return;
+	const int e = (int)(t % 27);
+	const long long p = t / 27;
+	c[t] = A[e * L.npad + compact_to_padded(L, p0 + p)];
+}
+
+static unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+void launch_pack_vec(hipStream_t s, const LevelView &L, const double *c, double *d, long long p0, long long n) {
+	if (n > 0) hipLaunchKernelGGL(k_pack_vec, dim3(nblk(n, 256)), dim3(256), 0, s, L, c, d, p0, n);
+	HIPCHECK_LAUNCH();
+}
+void launch_unpack_vec(hipStream_t s, const LevelView &L, const double *d, double *c, long long p0, long long n) {
+	if (n > 0) hipLaunchKernelGGL(k_unpack_vec, dim3(nblk(n, 256)), dim3(256), 0, s, L, d, c, p0, n);
+	HIPCHECK_LAUNCH();
+}
+void launch_pack_op(hipStream_t s, const LevelView &L, const double *c, double *A, long long p0, long long n) {
+	if (n > 0) hipLaunchKernelGGL(k_pack_op, dim3(nblk(n * 27, 256)), dim3(256), 0, s, L, c, A, p0, n);
+	HIPCHECK_LAUNCH();
+}
+void launch_unpack_op(hipStream_t s, const LevelView &L, const double *A, double *c, long long p0, long long n) {
+	if (n > 0) hipLaunchKernelGGL(k_unpack_op, dim3(nblk(n * 27, 256)), dim3(256), 0, s, L, A, c, p0, n);
+	HIPCHECK_LAUNCH();
+}
+
+template <typename T>
+__global__ void k_fill(T *__restrict__ p, long long n, T v) {
+	for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) p[t] = v;
+}
+template <typename T>
+void launch_fill(hipStream_t s, T *p, long long n, T v) {
+	if (n <= 0) return;
+	long long b = (n + 255) / 256;
+	if (b > 65536) b = 65536;
+	hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)b), dim3(256), 0, s, p, n, v);
+	HIPCHECK_LAUNCH();
+}
+
+__global__ void k_round_f32(const double *__restrict__ a, float *__restrict__ b, long long n) {
+	for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) b[t] = (float)a[t];
+}
+void launch_round_f32(hipStream_t s, const double *a, float *b, long long n) {
+	long long bl = (n + 255) / 256;
+	if (bl > 65536) bl = 65536;
+	hipLaunchKernelGGL(k_round_f32, dim3((unsigned)bl), dim3(256), 0, s, a, b, n);
+	HIPCHECK_LAUNCH();
+}
+
+__global__ __launch_bounds__(256) void k_add_f32(LevelView L, const float *__restrict__ e, double *__restrict__ x) {
+	const int i = (int)(blockIdx.x * 64 + threadIdx.x), j = (int)(blockIdx.y * 4 + threadIdx.y), k = (int)blockIdx.z;
+	if (i >= L.m || j >= L.n) return;
+	const long long p = idx3(L, i, j, k);
+	x[p] = x[p] + (double)e[p];
+}
+void launch_add_f32_to_f64(hipStream_t s, const LevelView &L, const float *e, double *x) {
+	dim3 grid((L.m + 63) / 64, (L.n + 3) / 4, L.o), block(64, 4, 1);
+	hipLaunchKernelGGL(k_add_f32, grid, block, 0, s, L, e, x);
+	HIPCHECK_LAUNCH();
+}
+
+// ------------------------------------------------------ instantiations
+#define INST(T) \
+	template void launch_gs_colour<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int); \
+	template void launch_gs_pair<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int); \
+	template void launch_gs_plane<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int); \
+	template void launch_gs_small<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int, int, int); \
+	template int launch_sumsq<T>(hipStream_t, const LevelView &, const T *, double *); \
+	template void launch_restrict<T>(hipStream_t, const LevelView &, const LevelView &, const T *, T *, T *); \
+	template void launch_interp<T>(hipStream_t, const LevelView &, const LevelView &, const T *, T *); \
+	template void launch_lu_solve<T>(hipStream_t, const LevelView &, int, const double *, const int *, const T *, T *); \
+	template void launch_fill<T>(hipStream_t, T *, long long, T);
+INST(double)
+INST(float)
+template int launch_residual<double, double>(hipStream_t, const LevelView &, const double *, const double *, const double *, double *, double *);
+template int launch_residual<double, float>(hipStream_t, const LevelView &, const double *, const double *, const double *, float *, double *);
+template int launch_residual<float, float>(hipStream_t, const LevelView &, const float *, const float *, const float *, float *, double *);
+template void launch_fill<int>(hipStream_t, int *, long long, int);
+
+}  // namespace pgmg

@@ -1,0 +1,63 @@
+"""The native library loads, exports every function include/*.h declares, and
+its host-only entry points agree with the oracle.  No GPU calls."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import pgmg_amd
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for mt in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_ ]*[\s\*]+(pgmg_[A-Za-z0-9_]+)\s*\(", text, re.M):
+            if "static" in mt.group(0) or "PGMG_HD" in mt.group(0):
+                continue  # header-only inline helpers, not library exports
+            names.add(mt.group(1))
+    return sorted(names)
+
+
+def test_library_exports_all_declared_symbols():
+    assert os.path.exists(pgmg_amd.LIB_PATH), "build the library first"
+    L = ctypes.CDLL(pgmg_amd.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_version_and_defaults():
+    assert pgmg_amd.lib().pgmg_version() == 1
+    o = pgmg_amd.default_opts()
+    assert (o.precision, o.smoother, o.bottom, o.v_down, o.v_up, o.v_bottom) == (0, 1, 0, 2, 2, 20)
+
+
+@pytest.mark.parametrize("dims", [(65, 65, 65), (257, 257, 257), (513, 513, 513), (20, 17, 30), (33, 65, 129), (9, 9, 9)])
+def test_hierarchy_shape_matches_oracle(dims):
+    assert pgmg_amd.hierarchy_shape(*dims) == O.hierarchy(*dims)
+
+
+def test_periodic_is_rejected():
+    d = pgmg_amd.Dims(9, 9, 9, 1, 0, 0)
+    nl = ctypes.c_int()
+    out = (ctypes.c_int * 48)()
+    rc = pgmg_amd.lib().pgmg_hierarchy_shape(ctypes.byref(d), ctypes.byref(nl), out)
+    assert rc == 9  # PGMG_ERR_UNSUPPORTED
+    assert b"periodic" in pgmg_amd.lib().pgmg_last_error()
+
+
+def test_footprint_scales():
+    f257 = pgmg_amd.footprint(257, 257, 257)
+    f257m = pgmg_amd.footprint(257, 257, 257, precision=pgmg_amd.PREC_MIXED)
+    assert 3.5e9 < f257 < 6e9
+    assert f257m > f257 * 1.4
+    assert pgmg_amd.footprint(513, 513, 513, precision=pgmg_amd.PREC_MIXED) < 80e9

@@ -1,0 +1,169 @@
+"""GPU parity: the HIP library against the oracle and the reference's golden
+vectors, through the C ABI.  Bit-for-bit for every deterministic operation
+(Galerkin operators, both smoothers, restriction, interpolation, V-cycle
+iterates, LU factor and solve, the FP32 inner cycle); sums of squares are
+reduced in a different order than the reference's serial loop and are
+compared to 1e-12 relative."""
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import pgmg_amd as P
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FIXTURES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
+                  if not os.path.basename(p).startswith("solve_"))
+NORM_RTOL = 1e-12
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def load(name):
+    return np.load(os.path.join(GOLDEN, name + ".npz"))
+
+
+def setup(z, **kw):
+    fam, seed, m, n, o = [int(v) for v in z["meta"]]
+    A = O.gen_operator(fam, seed, m, n, o)
+    b = O.gen_rhs(seed, m, n, o)
+    h = P.Hierarchy(m, n, o, **kw)
+    h.set_operator27(A)
+    h.compute_rats()
+    h.set_rhs(b)
+    return h, A, b, (fam, seed, m, n, o)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_galerkin_operators_bitwise(name):
+    say("case", name)
+    z = load(name)
+    h, A, b, meta = setup(z, smoother=P.SMOOTH_EXACT)
+    assert np.array_equal(h.get_operator27(0), A)
+    for l in range(1, h.nlevels):
+        assert np.array_equal(h.get_operator27(l).reshape(-1), z["A%d" % l]), "level %d" % l
+    h.close()
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_level_operations_bitwise_exact_mode(name):
+    say("case", name)
+    z = load(name)
+    h, A, b, (fam, seed, m, n, o) = setup(z, smoother=P.SMOOTH_EXACT, use_graph=False)
+    rn = h.rhs_l2norm()
+    assert abs(rn / z["rhs_l2norm"][0] - 1) < NORM_RTOL
+    h.set_x(O.test_field(m, n, o, seed + 1))
+    assert abs(h.l2_error_all() / z["l2_error_all_xt"][0] - 1) < NORM_RTOL
+    h.gauss_seidel(0, 0)
+    assert np.array_equal(h.get_x(0).reshape(-1), z["x_gs0"])
+    h.gauss_seidel(1, 0)
+    assert np.array_equal(h.get_x(0).reshape(-1), z["x_gs1"])
+    if h.nlevels > 1:
+        h.restriction(1)
+        assert np.array_equal(h.get_rhs(1).reshape(-1), z["r1_restrict"])
+        assert np.array_equal(h.get_x(1).reshape(-1), z["x1_after_restrict"])
+        mc, nc, oc = h.dims[1]
+        h.set_x(O.test_field(mc, nc, oc, seed + 2), level=1)
+        h.interpolation(1)
+        assert np.array_equal(h.get_x(0).reshape(-1), z["x_interp"])
+    h.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("name", FIXTURES)
+def test_v_cycles_bitwise_exact_mode(name, graph):
+    say("case", name, "graph", graph)
+    z = load(name)
+    h, A, b, meta = setup(z, smoother=P.SMOOTH_EXACT, use_graph=graph)
+    hist = [h.l2_error_all()]
+    for c in range(4):
+        h.v_cycle()
+        hist.append(h.l2_error_all())
+        if c == 0:
+            assert np.array_equal(h.get_x(0).reshape(-1), z["x_v1"])
+    np.testing.assert_allclose(hist, z["vcycle_res_gs24"], rtol=NORM_RTOL, atol=0)
+    assert np.array_equal(h.get_x(0).reshape(-1), z["x_v4_gs24"])
+    h.close()
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_v_cycles_colour8_bitwise_vs_oracle(name):
+    say("case", name)
+    z = load(name)
+    h, A, b, meta = setup(z, smoother=P.SMOOTH_COLOUR8)
+    mg = O.Multigrid(A, colour8=True)
+    mg.set_rhs(b)
+    for c in range(3):
+        h.v_cycle()
+        mg.v_cycle()
+    assert np.array_equal(h.get_x(0), mg.get_x(0))
+    assert abs(h.l2_error_all() / mg.l2_error_all() - 1) < NORM_RTOL
+    h.close()
+
+
+@pytest.mark.parametrize("name", [f for f in FIXTURES if "9x9x9" not in f])
+def test_lu_bottom(name):
+    """The device LU follows the oracle's factorisation exactly; against the
+    reference's MKL factor the iterates agree to 1e-13."""
+    say("case", name)
+    z = load(name)
+    h, A, b, meta = setup(z, smoother=P.SMOOTH_EXACT, bottom=P.BOTTOM_LU)
+    assert h.lu_active()
+    mg = O.Multigrid(A, lu=True)
+    mg.set_rhs(b)
+    for c in range(4):
+        h.v_cycle()
+        mg.v_cycle()
+    x = h.get_x(0)
+    assert np.array_equal(x, mg.get_x(0))
+    ref = z["x_v4_lu"]
+    assert np.abs(x.reshape(-1) - ref).max() <= 1e-13 * np.abs(ref).max()
+    h.close()
+
+
+@pytest.mark.parametrize("name", ["f7u_20x17x30", "q1_18x13x22", "f7log_33x20x11"])
+@pytest.mark.parametrize("smoother", [P.SMOOTH_EXACT, P.SMOOTH_COLOUR8])
+def test_mixed_precision_bitwise_vs_oracle(name, smoother):
+    say("case", name, smoother)
+    z = load(name)
+    h, A, b, meta = setup(z, smoother=smoother, precision=P.PREC_MIXED)
+    # The log-uniform family converges at about 0.94 per cycle even in FP64;
+    # compare a fixed number of defect-correction steps there.
+    maxc = 12 if name.startswith("f7log") else 60
+    c, hist = h.solve(1e-10, maxc, check=False)
+    mg = O.Multigrid(A, colour8=smoother == P.SMOOTH_COLOUR8)
+    mg.set_rhs(b)
+    c2, hist2 = mg.solve(1e-10, maxc, mixed=True)
+    assert c == c2
+    assert np.array_equal(h.get_x(0), mg.get_x(0))
+    np.testing.assert_allclose(hist, hist2, rtol=1e-9)
+    h.close()
+
+
+def test_solve_65_exact_mode_matches_reference():
+    z = load("solve_f7u_65_gs24")
+    h, A, b, meta = setup(z, smoother=P.SMOOTH_EXACT)
+    c, hist = h.solve(float(z["tol"]), 50)
+    assert c == len(z["res"]) - 1
+    np.testing.assert_allclose(hist, z["res"], rtol=NORM_RTOL)
+    assert np.array_equal(h.get_x(0).reshape(-1), z["x"])
+    assert h.gs_count() == c * 42
+    h.close()
+
+
+@pytest.mark.parametrize("family", [P.FAM_FACE7_UNIFORM, P.FAM_FACE7_LOGUNI, P.FAM_Q1_ANISO])
+def test_device_generator_matches_host(family):
+    m, n, o = 13, 10, 7
+    h = P.Hierarchy(m, n, o)
+    h.generate_problem(family, 4242)
+    assert np.array_equal(h.get_operator27(0), O.gen_operator(family, 4242, m, n, o))
+    assert np.array_equal(h.get_rhs(0), O.gen_rhs(4242, m, n, o))
+    h.close()
