@@ -1,0 +1,221 @@
+/** \file region.hh
+ * \brief region of the drop-in API (reference: region.hh:12-529).
+ *
+ * A region is one level of a multigrid hierarchy.  In the reference it owns
+ * the level's arrays in host memory; here the arrays live in HBM inside a
+ * pgmg_hier handle and a region is a view of one level of it.  The host
+ * arrays x0 and r0 are mirrors: call sync_from_device() before reading them
+ * and sync_to_device() after writing them.  The level layout is the
+ * reference's one-rank layout: no ghost points, mg = sm, ng = sn, x0 = x,
+ * index i + mg*(j + ng*k).
+ *
+ * region(p_class&, geometry&, comm_buffer&) builds a lone top-level region,
+ * as gs_parallel.cc:25 does; regions inside a multigrid are made by it. */
+#ifndef PGMG_AMD_REGION_HH
+#define PGMG_AMD_REGION_HH
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "buffer.hh"
+#include "common.hh"
+#include "geometry.hh"
+
+/** Static sweep counter shared by all regions (region.hh:17), in a template
+ * so that the header defines it without a separate translation unit. */
+template <int D> struct pgmg_region_counter {static int gs_num;};
+template <int D> int pgmg_region_counter<D>::gs_num = 0;
+
+namespace pgmg_detail {
+
+/** Ingests a p_class operator (region.hh:474-529, 95-101) into 27-entry rows.
+ * Each point's box [range_xd, range_xu) x ... must lie inside [-1,1]^3; the
+ * entries of internal points come from fill_entries in k,j,i order, those of
+ * the others from external_ptr, in the box's k,j,i order. */
+template <class p_class>
+void ingest_operator(p_class &pr, int m, int n, int o, std::vector<double> &A27) {
+	A27.assign((size_t)m * n * o * 27, 0.0);
+	std::vector<double> tmp(27);
+	for (int k = 0; k < o; k++) for (int j = 0; j < n; j++) for (int i = 0; i < m; i++) {
+		int xd = pr.range_xd(i, j, k), xu = pr.range_xu(i, j, k);
+		int yd = pr.range_yd(i, j, k), yu = pr.range_yu(i, j, k);
+		int zd = pr.range_zd(i, j, k), zu = pr.range_zu(i, j, k);
+		if (xd < -1 || xu > 2 || yd < -1 || yu > 2 || zd < -1 || zu > 2 || xd >= xu || yd >= yu || zd >= zu)
+			p_fatal_error("Stencil box outside [-1,1]^3 is not supported", PGMG_SETUP_ERROR);
+		if (i + xd < 0 || i + xu > m || j + yd < 0 || j + yu > n || k + zd < 0 || k + zu > o)
+			p_fatal_error("Stencil box reaches outside the grid (periodic problems are not supported)", PGMG_SETUP_ERROR);
+		const double *src;
+		if (pr.internal(i, j, k)) {
+			int sz = pr.mem_size(i, j, k);
+			if (sz != (xu - xd) * (yu - yd) * (zu - zd)) p_fatal_error("mem_size differs from the box size", PGMG_SETUP_ERROR);
+			double *en = &tmp[0];
+			pr.fill_entries(i, j, k, en);
+			src = &tmp[0];
+		} else {
+			src = pr.external_ptr(i, j, k);
+		}
+		double *row = &A27[27 * ((size_t)i + (size_t)m * (j + (size_t)n * k))];
+		for (int dk = zd; dk < zu; dk++) for (int dj = yd; dj < yu; dj++) for (int di = xd; di < xu; di++)
+			row[(dk + 1) * 9 + (dj + 1) * 3 + (di + 1)] = *(src++);
+	}
+}
+
+}  // namespace pgmg_detail
+
+class region : public pgmg_region_counter<0> {
+	public:
+		/** Parent (finer) region, or NULL at the top. */
+		region *p;
+		int rank;
+		const bool x_prd, y_prd, z_prd;
+		bool gs_enable;
+		/** Global level size. */
+		const int m, n, o;
+		int ip, jp, kp, mp, np, op;
+		/** Host mirrors of the solution and source (sync_from_device first). */
+		double *x, *x0, *r0;
+		int mg, ng, og, mng, indent;
+		int ai, aj, ak, bi, bj, bk, sm, sn, so, smn, smno;
+		/** The device hierarchy and the level this region views. */
+		pgmg_hier *hier;
+		int level;
+
+		/** A lone top-level region (gs_parallel.cc:25): one device level. */
+		template <class p_class>
+		region(p_class &pr, geometry &gm, comm_buffer &com_)
+			: p(NULL), rank(0), x_prd(gm.x_prd), y_prd(gm.y_prd), z_prd(gm.z_prd), gs_enable(true),
+			  m(gm.m), n(gm.n), o(gm.o), hier(NULL), level(0), owns(true) {
+			(void)com_;
+			if (x_prd || y_prd || z_prd) p_fatal_error("Periodic problems are not supported", PGMG_SETUP_ERROR);
+			pgmg_dims d = {m, n, o, 0, 0, 0};
+			pgmg_opts opt;
+			pgmg_default_opts(&opt);
+			opt.smoother = PGMG_SMOOTH_EXACT;
+			opt.max_levels = 1;
+			pgmg_check(pgmg_hier_create(&d, &opt, &hier), "region");
+			init_fields();
+		}
+		/** A view of level l of an existing hierarchy (used by multigrid). */
+		region(region *parent, pgmg_hier *h, int l, int m_, int n_, int o_)
+			: p(parent), rank(0), x_prd(false), y_prd(false), z_prd(false), gs_enable(true),
+			  m(m_), n(n_), o(o_), hier(h), level(l), owns(false) {init_fields();}
+		~region() {
+			delete[] x;
+			delete[] r0;
+			if (owns) pgmg_hier_destroy(hier);
+		}
+
+		/** region::setup_matrices (region.hh:95-101): the top-level operator
+		 * from the problem class.  On a lone region this also finalises the
+		 * operator (there are no coarse levels to compute). */
+		template <class p_class>
+		void setup_matrices(p_class &pr) {
+			std::vector<double> A27;
+			pgmg_detail::ingest_operator(pr, m, n, o, A27);
+			pgmg_check(pgmg_set_operator27(hier, &A27[0]), "setup_matrices");
+			if (owns) pgmg_check(pgmg_compute_rats(hier), "setup_matrices");
+		}
+		template <class p_class>
+		void setup_fields(p_class &pr) {setup_x_field(pr); setup_r_field(pr);}
+		/** region.hh:110-116. */
+		template <class p_class>
+		void setup_x_field(p_class &pr) {
+			for (int k = 0; k < so; k++) for (int j = 0; j < sn; j++) for (int i = 0; i < sm; i++)
+				x0[i + mg * (j + k * ng)] = pr.x_field(i + ai, j + aj, k + ak);
+			sync_x_to_device();
+		}
+		/** region.hh:118-126. */
+		template <class p_class>
+		void setup_r_field(p_class &pr) {
+			double *rp = r0;
+			for (int k = 0; k < so; k++) for (int j = 0; j < sn; j++) for (int i = 0; i < sm; i++)
+				*(rp++) = pr.r_field(i + ai, j + aj, k + ak);
+			sync_r_to_device();
+		}
+
+		/** Mirrors <-> device. */
+		void sync_from_device() {
+			pgmg_check(pgmg_get_x(hier, level, x0), "sync");
+			pgmg_check(pgmg_get_rhs(hier, level, r0), "sync");
+		}
+		void sync_to_device() {sync_x_to_device(); sync_r_to_device();}
+		void sync_x_to_device() {pgmg_check(pgmg_set_x(hier, level, x0), "sync");}
+		void sync_r_to_device() {pgmg_check(pgmg_set_rhs(hier, level, r0), "sync");}
+
+		/** region::gauss_seidel (region.cc:355-542). */
+		void gauss_seidel(int mode = 0) {
+			gs_num++;
+			pgmg_check(pgmg_gauss_seidel(hier, mode, level, 1), "gauss_seidel");
+		}
+		/** region::solve_exact (region_tr.cc:540-558). */
+		void solve_exact(bool smooth = 0) {
+			pgmg_check(pgmg_solve_exact(hier, level, smooth ? 1 : 0), "solve_exact");
+		}
+		/** Sums of squared residuals (region.cc:1475-1529) and of the
+		 * source (region.cc:1549-1558); with one rank l2_error = l2_error_all. */
+		double l2_error_all() {double v; pgmg_check(pgmg_l2_error_all(hier, level, &v), "l2_error_all"); return v;}
+		double l2_error() {return l2_error_all();}
+		double rhs_l2norm() {double v; pgmg_check(pgmg_rhs_l2norm(hier, level, &v), "rhs_l2norm"); return v;}
+		/** Max-norm residuals (region.cc:1492-1546). */
+		double l0_error_all() {
+			std::vector<double> res((size_t)smno);
+			pgmg_check(pgmg_residual(hier, level, &res[0]), "l0_error_all");
+			double e = 0;
+			for (size_t q = 0; q < res.size(); q++) if (fabs(res[q]) > e) e = fabs(res[q]);
+			return e;
+		}
+		double l0_error() {return l0_error_all();}
+		/** region::compute_residual (region.cc:1848-1854) into an array of
+		 * the level's size. */
+		void compute_residual(double *ext_r) {pgmg_check(pgmg_residual(hier, level, ext_r), "compute_residual");}
+		/** region.cc:1672-1687. */
+		void allclear_x_field() {pgmg_check(pgmg_clear_x(hier, level), "allclear_x_field");}
+		void clear_x_field() {allclear_x_field();}
+		void clear_r_field() {
+			memset(r0, 0, sizeof(double) * (size_t)smno);
+			sync_r_to_device();
+		}
+		/** region.cc:1690-1701. */
+		void copy_r_to_x() {sync_from_device(); memcpy(x0, r0, sizeof(double) * (size_t)smno); sync_x_to_device();}
+		void copy_x_to_r() {sync_from_device(); memcpy(r0, x0, sizeof(double) * (size_t)smno); sync_r_to_device();}
+		/** region::setup_test (region.cc:1812-1826). */
+		void setup_test(int ca, bool xfield) {
+			sync_from_device();
+			for (int k = 0; k < so; k++) for (int j = 0; j < sn; j++) for (int i = 0; i < sm; i++) {
+				double *pp = xfield ? x0 + (i + mg * (j + ng * k)) : r0 + (i + sm * (j + sn * k));
+				switch (ca) {
+					case 0: *pp = i + ai; break;
+					case 1: *pp = j + aj; break;
+					case 2: *pp = k + ak; break;
+					case 3: *pp = ak + aj + ai + k + j + i; break;
+					case 4: *pp = rank; break;
+					case 5: *pp = i == 7 && j == 7 && k == 7 ? 1 : 0; break;
+					case 6: *pp = 1;
+				}
+			}
+			sync_to_device();
+		}
+		inline int total() {return m * n * o;}
+		inline int min_size() {return sm < sn ? (sm < so ? sm : so) : (sn < so ? sn : so);}
+		/** The level operator as 27-entry rows (the reference exposes A and S
+		 * per point; this is the same data in the fixed-box format). */
+		void get_operator27(double *A27) {pgmg_check(pgmg_get_operator27(hier, level, A27), "get_operator27");}
+	private:
+		bool owns;
+		void init_fields() {
+			ip = jp = kp = 0; mp = np = op = 1;
+			ai = aj = ak = 0; bi = m; bj = n; bk = o;
+			sm = m; sn = n; so = o; smn = sm * sn; smno = smn * so;
+			mg = sm; ng = sn; og = so; mng = mg * ng; indent = 0;
+			x = new double[(size_t)smno];
+			x0 = x;
+			r0 = new double[(size_t)smno];
+			memset(x, 0, sizeof(double) * (size_t)smno);
+			memset(r0, 0, sizeof(double) * (size_t)smno);
+		}
+		region(const region &);
+		region &operator=(const region &);
+};
+
+#endif

@@ -69,6 +69,8 @@ typedef struct {
 	int v_up;        /* multigrid.hh:128, default 2 */
 	int v_bottom;    /* multigrid.hh:125, default 20 (fmg only; kept for API parity) */
 	int use_graph;   /* capture V-cycles in a hipGraph */
+	int max_levels;  /* 0 = the reference's rule; 1 = top level only (a lone
+	                    region, as region(p, gm, com) in gs_parallel.cc:25) */
 } pgmg_opts;
 
 typedef struct pgmg_hier pgmg_hier;
@@ -131,10 +133,16 @@ int pgmg_interpolation(pgmg_hier *h, int level);
 int pgmg_solve_exact(pgmg_hier *h, int level, int smooth);
 /* multigrid::v_cycle(mode), multigrid.cc:88-106. */
 int pgmg_v_cycle(pgmg_hier *h, int mode);
+/* The public fields v_down / v_up (multigrid.hh:122-128) changed after
+ * construction: later cycles use the new counts. */
+int pgmg_set_sweeps(pgmg_hier *h, int v_down, int v_up);
 /* multigrid::l2_error_all(level), multigrid.cc:246-248 (sum of squared
  * residuals, no square root) and rhs_l2norm(level), multigrid.cc:259-261. */
 int pgmg_l2_error_all(pgmg_hier *h, int level, double *out);
 int pgmg_rhs_l2norm(pgmg_hier *h, int level, double *out);
+/* region::compute_residual, region.cc:1848-1854: the residual r - A x of a
+ * level into a host array (point-major). */
+int pgmg_residual(pgmg_hier *h, int level, double *res);
 /* Solve to a relative residual: V-cycles (FP64) or FP64 defect correction
  * around FP32 V-cycles (MIXED) from the current x until
  * sqrt(l2_error_all/rhs_l2norm) < tol.  hist (max_cycles+1 doubles, may be

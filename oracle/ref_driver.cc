@@ -121,8 +121,32 @@ static void enable_lu(multigrid &mg) {
 	}
 }
 
+/* The reference's mg_guide (multigrid.hh:34-118, multigrid.cc:268-292)
+ * driven through a fixed sequence of update(extras) calls, with and without
+ * the moving average; prints the state after each call. */
+static const int guide_extras[] = {0, 0, 1, 0, 3, 0, 0, 2, 0, 0, 0, 0, 1, 6, 0, 0, 0, 0, 0, 0, 0, 0, 0, 4, 0};
+
+static void run_guide() {
+	for (int avg = 0; avg < 2; avg++) {
+		multigrid::mg_guide g;
+		g.set_tol(3);
+		if (avg) g.average_on();
+		printf("tol %.17g\n", g.tol);
+		for (size_t q = 0; q < sizeof guide_extras / sizeof guide_extras[0]; q++) {
+			g.update(guide_extras[q]);
+			printf("%d %d %d %.17g %.17g %d\n", avg, guide_extras[q], g.iter, g.iter_avg, g.iter_var, g.pred_iters());
+		}
+	}
+	fflush(stdout);
+}
+
 int main(int argc, char **argv) {
 	MPI_Init(&argc, &argv);
+	if (argc >= 2 && std::string(argv[1]) == "guide") {
+		run_guide();
+		MPI_Finalize();
+		return 0;
+	}
 	if (argc < 7) {
 		fprintf(stderr, "usage: pgmg_ref fixture|solve|time family seed m n o ...\n");
 		MPI_Finalize();

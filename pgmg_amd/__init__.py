@@ -38,7 +38,7 @@ class Dims(ctypes.Structure):
 class Opts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("smoother", ctypes.c_int), ("bottom", ctypes.c_int),
                 ("v_down", ctypes.c_int), ("v_up", ctypes.c_int), ("v_bottom", ctypes.c_int),
-                ("use_graph", ctypes.c_int)]
+                ("use_graph", ctypes.c_int), ("max_levels", ctypes.c_int)]
 
 
 _D = ctypes.POINTER(ctypes.c_double)
@@ -71,9 +71,11 @@ _SIGS = {
     "pgmg_interpolation": (_i, [_VP, _i]),
     "pgmg_solve_exact": (_i, [_VP, _i, _i]),
     "pgmg_v_cycle": (_i, [_VP, _i]),
+    "pgmg_set_sweeps": (_i, [_VP, _i, _i]),
     "pgmg_l2_error_all": (_i, [_VP, _i, _D]),
     "pgmg_rhs_l2norm": (_i, [_VP, _i, _D]),
     "pgmg_solve": (_i, [_VP, ctypes.c_double, _i, _I, _D]),
+    "pgmg_residual": (_i, [_VP, _i, _D]),
     "pgmg_synchronize": (_i, [_VP]),
     "pgmg_gs_count": (_i, [_VP, ctypes.POINTER(ctypes.c_longlong)]),
     "pgmg_prof_enable": (_i, [_VP, _i, _i]),
@@ -137,14 +139,16 @@ class Hierarchy:
     Method names follow the reference's multigrid class (multigrid.hh)."""
 
     def __init__(self, m, n, o, precision=PREC_FP64, smoother=SMOOTH_COLOUR8, bottom=BOTTOM_GS24,
-                 v_down=2, v_up=2, use_graph=True):
+                 v_down=2, v_up=2, use_graph=True, max_levels=0):
         self.opts = default_opts(precision=precision, smoother=smoother, bottom=bottom,
-                                 v_down=v_down, v_up=v_up, use_graph=use_graph)
+                                 v_down=v_down, v_up=v_up, use_graph=use_graph, max_levels=max_levels)
         d = Dims(m, n, o, 0, 0, 0)
         h = _VP()
         _check(lib().pgmg_hier_create(ctypes.byref(d), ctypes.byref(self.opts), ctypes.byref(h)))
         self.h = h
         self.dims = hierarchy_shape(m, n, o)
+        if max_levels > 0:
+            self.dims = self.dims[:max_levels]
 
     def close(self):
         if getattr(self, "h", None):
@@ -246,6 +250,11 @@ class Hierarchy:
         if not (rc == 11 and not check):
             _check(rc)
         return c.value, hist[:c.value + 1].copy()
+
+    def residual(self, level=0):
+        out = np.zeros(self.shape(level))
+        _check(lib().pgmg_residual(self.h, level, _dptr(out)))
+        return out
 
     def synchronize(self):
         _check(lib().pgmg_synchronize(self.h))

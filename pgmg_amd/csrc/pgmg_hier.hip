@@ -332,6 +332,7 @@ void pgmg_default_opts(pgmg_opts *o) {
 	o->bottom = PGMG_BOTTOM_GS24;
 	o->v_down = 2; o->v_up = 2; o->v_bottom = 20;
 	o->use_graph = 1;
+	o->max_levels = 0;
 }
 
 int pgmg_hierarchy_shape(const pgmg_dims *d, int *nlevels, int *dims_out) {
@@ -351,6 +352,7 @@ int pgmg_hier_footprint(const pgmg_dims *d, const pgmg_opts *opts, uint64_t *byt
 	int dims[3 * kMaxLevels];
 	int nl = shape(d->m, d->n, d->o, dims);
 	if (nl < 0) return fail(PGMG_ERR_MULTIGRID, "Maximum number of multigrid levels reached");
+	if (opts && opts->max_levels > 0 && opts->max_levels < nl) nl = opts->max_levels;
 	const bool mixed = opts && opts->precision == PGMG_PREC_MIXED;
 	uint64_t b = 0;
 	for (int l = 0; l < nl; l++) b += (uint64_t)level_bytes(make_level_view(dims[3 * l], dims[3 * l + 1], dims[3 * l + 2]), mixed);
@@ -377,9 +379,11 @@ int pgmg_hier_create(const pgmg_dims *d, const pgmg_opts *opts, pgmg_hier **out)
 	if (o.smoother != PGMG_SMOOTH_EXACT && o.smoother != PGMG_SMOOTH_COLOUR8) return fail(PGMG_ERR_ARG, "bad smoother");
 	if (o.bottom != PGMG_BOTTOM_GS24 && o.bottom != PGMG_BOTTOM_LU) return fail(PGMG_ERR_ARG, "bad bottom");
 	if (o.v_down < 0 || o.v_up < 0) return fail(PGMG_ERR_ARG, "negative sweep counts");
+	if (o.max_levels < 0) return fail(PGMG_ERR_ARG, "negative max_levels");
 	int dims[3 * kMaxLevels];
 	int nl = shape(d->m, d->n, d->o, dims);
 	if (nl < 0) return fail(PGMG_ERR_MULTIGRID, "Maximum number of multigrid levels reached");
+	if (o.max_levels > 0 && o.max_levels < nl) nl = o.max_levels;
 	int ndev = 0;
 	HIP_TRY(hipGetDeviceCount(&ndev));
 	if (ndev < 1) return fail(PGMG_ERR_HIP, "no HIP device");
@@ -653,6 +657,17 @@ int pgmg_v_cycle(pgmg_hier *h, int mode) {
 	return PGMG_OK;
 }
 
+int pgmg_set_sweeps(pgmg_hier *h, int v_down, int v_up) {
+	if (!h) return fail(PGMG_ERR_ARG, "null hierarchy");
+	if (v_down < 0 || v_up < 0) return fail(PGMG_ERR_ARG, "negative sweep counts");
+	if (v_down != h->opts.v_down || v_up != h->opts.v_up) {
+		free_graphs(h);
+		h->opts.v_down = v_down;
+		h->opts.v_up = v_up;
+	}
+	return PGMG_OK;
+}
+
 int pgmg_l2_error_all(pgmg_hier *h, int level, double *out) {
 	int rc = check_level(h, level);
 	if (rc) return rc;
@@ -706,6 +721,15 @@ int pgmg_solve(pgmg_hier *h, double tol, int max_cycles, int *cycles, double *hi
 	if (cycles) *cycles = c;
 	HIP_TRY(hipGetLastError());
 	return PGMG_OK;
+}
+
+int pgmg_residual(pgmg_hier *h, int level, double *res) {
+	int rc = check_level(h, level);
+	if (rc) return rc;
+	if (!res) return fail(PGMG_ERR_ARG, "null array");
+	Level &L = h->lev[level];
+	launch_residual<double, double>(h->stream, L.v, L.A, L.x, L.r, h->res64, (double *)nullptr);
+	return vec_io(h, level, h->res64, nullptr, res);
 }
 
 int pgmg_synchronize(pgmg_hier *h) {

@@ -1,0 +1,121 @@
+"""The drop-in C++ API (include/pgmg/) driven like the reference's own
+programs (gs_parallel.cc, guide_test.cc, the fixture solve), built by
+tests/cpp/Makefile against libpgmg_amd.so."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "bin")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def run(args, timeout=300):
+    p = subprocess.run([os.path.join(BIN, args[0])] + [str(a) for a in args[1:]],
+                       capture_output=True, text=True, timeout=timeout)
+    assert p.returncode == 0, p.stdout + p.stderr
+    return p.stdout
+
+
+def test_guide_matches_reference():
+    """mg_guide's update rule against the sequence the reference's own
+    mg_guide produced (oracle/ref_driver.cc `guide`)."""
+    out = run(["guide_unit"])
+    ref = open(os.path.join(GOLDEN, "guide_sequence.txt")).read()
+    assert out == ref
+
+
+def problem_simple_operator(m, n, o):
+    A = np.zeros((o, n, m, 27))
+    b = np.zeros((o, n, m))
+    for k in range(o):
+        for j in range(n):
+            for i in range(m):
+                inner = 0 < i < m - 1 and 0 < j < n - 1 and 0 < k < o - 1
+                if inner:
+                    for e in (4, 10, 12, 14, 16, 22):
+                        A[k, j, i, e] = 1.0
+                    A[k, j, i, 13] = -6.0
+                else:
+                    A[k, j, i, 13] = -0.375
+                b[k, j, i] = 1.0 if (i, j, k) == (3, 3, 3) else (-1.0 if (i, j, k) == (12, 12, 12) else 0.0)
+    return A, b
+
+
+@pytest.mark.gpu
+def test_lone_region_gauss_seidel_history(tmp_path):
+    """gs_parallel.cc with the reference's problem_simple: every sweep's sum of
+    squared residuals and the final iterate, against the oracle."""
+    m = n = o = 24
+    sweeps = 12
+    xf = tmp_path / "x.bin"
+    out = run(["gs_history", "simple", m, n, o, sweeps, xf]).splitlines()
+    hist = [float(l.split()[1]) for l in out[:sweeps + 1]]
+    A, b = problem_simple_operator(m, n, o)
+    x = np.zeros((o, n, m))
+    ref = [O.l2_error_all(A, x, b)]
+    for s in range(sweeps):
+        x = O.gs(A, x, b, 0)
+        ref.append(O.l2_error_all(A, x, b))
+    np.testing.assert_allclose(hist, ref, rtol=1e-12)
+    assert np.array_equal(np.fromfile(xf).reshape(o, n, m), x)
+    assert out[-1] == "gs_num %d" % sweeps
+
+
+@pytest.mark.gpu
+def test_multigrid_cycles_match_reference_fixture(tmp_path):
+    z = np.load(os.path.join(GOLDEN, "solve_f7u_65_gs24.npz"))
+    fam, seed, m, n, o = [int(v) for v in z["meta"]]
+    xf = tmp_path / "x.bin"
+    out = run(["mg_drivers", "cycles", fam, seed, m, n, o, xf]).splitlines()
+    assert out[0] == "levels 4 65x65x65 33x33x33 17x17x17 9x9x9"
+    res = [float(l.split()[2]) for l in out if l.startswith("res ")]
+    sweeps = [int(l.split()[4]) for l in out if l.startswith("res ") and "sweeps" in l]
+    np.testing.assert_allclose(res, z["res"], rtol=1e-12)
+    assert sweeps == [42] * (len(res) - 1)
+    assert np.array_equal(np.fromfile(xf), z["x"])
+
+
+@pytest.mark.gpu
+def test_solve_v_cycles_guide():
+    out = run(["mg_drivers", "guide", 1, 5, 33, 33, 33, "/dev/null"]).splitlines()
+    g = [l.split() for l in out if l.startswith("guide ")]
+    assert len(g) == 3
+    tol = (1.0 ** 2) * (np.finfo(float).eps ** 2) * 1e4 * 33 ** 3
+    for row in g:
+        assert row[3] == "0"
+        assert float(row[5]) <= tol
+
+
+@pytest.mark.gpu
+def test_fmg_cycle_against_oracle_composition():
+    """fmg_cycle (multigrid.cc:109-134) through the facade against the same
+    schedule composed from the oracle's level operations."""
+    m, n, o = 20, 17, 30
+    out = run(["mg_drivers", "fmg", 1, 9, m, n, o, "/dev/null"]).splitlines()
+    l2 = float([l for l in out if l.startswith("fmg ")][0].split()[1])
+    A = O.gen_operator(1, 9, m, n, o)
+    b = O.gen_rhs(9, m, n, o)
+    mg = O.Multigrid(A)
+    mg.set_rhs(b)
+    L = mg.nlevels
+    for i in range(L - 1):
+        mg.restriction(i + 1)
+        for _ in range(2):
+            mg.gauss_seidel(0, i + 1)
+    for j in range(L - 1, 0, -1):
+        for i in range(j, L - 1):
+            mg.restriction(i + 1)
+            for _ in range(2):
+                mg.gauss_seidel(0, i + 1)
+        for _ in range(20):
+            mg.gauss_seidel(0, L - 1)
+        for i in range(L - 1, j - 1, -1):
+            mg.interpolation(i)
+            for _ in range(2):
+                mg.gauss_seidel(0, i - 1)
+    assert abs(l2 / mg.l2_error_all() - 1) < 1e-12
