@@ -154,6 +154,42 @@ int pgmg_synchronize(pgmg_hier *h);
 /* region::gs_num, region.hh:17: Gauss-Seidel sweeps performed so far. */
 int pgmg_gs_count(const pgmg_hier *h, long long *count);
 
+/* ---------------------------------------------------------------------
+ * Batches and serving (no counterpart in the reference, whose ranks solve
+ * one problem each).  A batch owns one HBM arena and up to max_slots
+ * independent hierarchies of any sizes; pgmg_batch_step advances every
+ * active slot by one cycle with one launch per (operation, level) across all
+ * of them.  A slot's arithmetic is that of a lone hierarchy with the same
+ * options, whatever else shares the batch.  Smoother: COLOUR8 only.
+ * --------------------------------------------------------------------- */
+typedef struct pgmg_batch pgmg_batch;
+enum { PGMG_SLOT_FREE = 0, PGMG_SLOT_ACTIVE = 1, PGMG_SLOT_CONVERGED = 2, PGMG_SLOT_FAILED = 3 };
+/* arena_bytes of HBM are reserved at creation. */
+int pgmg_batch_create(const pgmg_opts *opts, uint64_t arena_bytes, int max_slots, pgmg_batch **out);
+int pgmg_batch_destroy(pgmg_batch *b);
+/* Admit a synthetic problem (device-generated) or a host operator + source.
+ * Returns PGMG_ERR_ADMISSION when no slot or not enough arena is free.  The
+ * solve stops when sqrt(l2_error_all/rhs_l2norm) < tol or after max_cycles. */
+int pgmg_batch_admit(pgmg_batch *b, const pgmg_dims *dims, int family, uint64_t seed, double tol, int max_cycles,
+		     int *slot);
+int pgmg_batch_admit_operator(pgmg_batch *b, const pgmg_dims *dims, const double *A27, const double *rhs, double tol,
+			      int max_cycles, int *slot);
+/* One cycle for every active slot; reports how many remain active and how
+ * many finished in this step. */
+int pgmg_batch_step(pgmg_batch *b, int *active, int *finished);
+int pgmg_batch_slot_status(pgmg_batch *b, int slot, int *state, int *cycles, double *rel_residual);
+int pgmg_batch_get_x(pgmg_batch *b, int slot, double *x);
+/* Return a finished slot's memory to the arena. */
+int pgmg_batch_release(pgmg_batch *b, int slot);
+int pgmg_batch_free_bytes(const pgmg_batch *b, uint64_t *bytes);
+/* Open-loop serving: requests arrive at the given times (seconds from the
+ * call); each is admitted when a slot and arena space are free (FIFO), runs
+ * with the others, and completes when its solution is on the host.
+ * Results give admission and completion times (seconds from the call). */
+typedef struct { double arrival_s; int m, n, o; int family; uint64_t seed; double tol; int max_cycles; } pgmg_request;
+typedef struct { double admit_s, done_s; int cycles; int status; double rel_residual; } pgmg_result;
+int pgmg_serve_run(pgmg_batch *b, const pgmg_request *reqs, int n, pgmg_result *results, double *wall_s);
+
 /* Kernel timing for roofline measurement (no counterpart in the reference):
  * while enabled, every launch of the named kernel class on level `level` is
  * bracketed by HIP events on the hierarchy's stream; pgmg_prof_read returns

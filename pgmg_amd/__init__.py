@@ -274,3 +274,133 @@ class Hierarchy:
         pts = ctypes.c_longlong()
         _check(lib().pgmg_prof_read(self.h, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(pts)))
         return n.value, ms.value, pts.value
+
+
+# ---------------------------------------------------------------- batches
+
+class Request(ctypes.Structure):
+    _fields_ = [("arrival_s", ctypes.c_double), ("m", ctypes.c_int), ("n", ctypes.c_int), ("o", ctypes.c_int),
+                ("family", ctypes.c_int), ("seed", ctypes.c_uint64), ("tol", ctypes.c_double),
+                ("max_cycles", ctypes.c_int)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("admit_s", ctypes.c_double), ("done_s", ctypes.c_double), ("cycles", ctypes.c_int),
+                ("status", ctypes.c_int), ("rel_residual", ctypes.c_double)]
+
+
+_BATCH_SIGS = {
+    "pgmg_batch_create": (_i, [ctypes.POINTER(Opts), ctypes.c_uint64, _i, ctypes.POINTER(_VP)]),
+    "pgmg_batch_destroy": (_i, [_VP]),
+    "pgmg_batch_admit": (_i, [_VP, ctypes.POINTER(Dims), _i, ctypes.c_uint64, ctypes.c_double, _i, _I]),
+    "pgmg_batch_admit_operator": (_i, [_VP, ctypes.POINTER(Dims), _D, _D, ctypes.c_double, _i, _I]),
+    "pgmg_batch_step": (_i, [_VP, _I, _I]),
+    "pgmg_batch_slot_status": (_i, [_VP, _i, _I, _I, _D]),
+    "pgmg_batch_get_x": (_i, [_VP, _i, _D]),
+    "pgmg_batch_release": (_i, [_VP, _i]),
+    "pgmg_batch_free_bytes": (_i, [_VP, ctypes.POINTER(ctypes.c_uint64)]),
+    "pgmg_serve_run": (_i, [_VP, ctypes.POINTER(Request), _i, ctypes.POINTER(Result), _D]),
+}
+_SIGS.update(_BATCH_SIGS)
+
+SLOT_FREE, SLOT_ACTIVE, SLOT_CONVERGED, SLOT_FAILED = 0, 1, 2, 3
+
+
+class Batch:
+    """Independent hierarchies of any sizes advanced together (pgmg_batch_*)."""
+
+    def __init__(self, arena_bytes, max_slots=256, precision=PREC_MIXED, bottom=BOTTOM_GS24, v_down=2, v_up=2):
+        self.opts = default_opts(precision=precision, smoother=SMOOTH_COLOUR8, bottom=bottom, v_down=v_down, v_up=v_up)
+        h = _VP()
+        _check(lib().pgmg_batch_create(ctypes.byref(self.opts), int(arena_bytes), max_slots, ctypes.byref(h)))
+        self.h = h
+        self.shapes = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pgmg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def admit(self, m, n, o, family, seed, tol, max_cycles=100):
+        """Returns the slot, or None when the batch has no room."""
+        s = ctypes.c_int(-1)
+        rc = lib().pgmg_batch_admit(self.h, ctypes.byref(Dims(m, n, o, 0, 0, 0)), family, seed, tol, max_cycles,
+                                    ctypes.byref(s))
+        if rc == 10:
+            return None
+        _check(rc)
+        self.shapes[s.value] = (o, n, m)
+        return s.value
+
+    def admit_operator(self, A, b, tol, max_cycles=100):
+        o, n, m = b.shape
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        s = ctypes.c_int(-1)
+        rc = lib().pgmg_batch_admit_operator(self.h, ctypes.byref(Dims(m, n, o, 0, 0, 0)), _dptr(A), _dptr(b), tol,
+                                             max_cycles, ctypes.byref(s))
+        if rc == 10:
+            return None
+        _check(rc)
+        self.shapes[s.value] = (o, n, m)
+        return s.value
+
+    def step(self):
+        a, f = ctypes.c_int(), ctypes.c_int()
+        _check(lib().pgmg_batch_step(self.h, ctypes.byref(a), ctypes.byref(f)))
+        return a.value, f.value
+
+    def run(self, max_steps=10000):
+        """Steps until no slot is active; returns the number of steps."""
+        steps = 0
+        while steps < max_steps:
+            a, f = self.step()
+            steps += 1
+            if a == 0 and not self._setting_up():
+                break
+        return steps
+
+    def _setting_up(self):
+        return any(self.status(s)[0] == SLOT_ACTIVE for s in self.shapes)
+
+    def status(self, slot):
+        st, c, r = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(lib().pgmg_batch_slot_status(self.h, slot, ctypes.byref(st), ctypes.byref(c), ctypes.byref(r)))
+        return st.value, c.value, r.value
+
+    def get_x(self, slot):
+        x = np.zeros(self.shapes[slot])
+        _check(lib().pgmg_batch_get_x(self.h, slot, _dptr(x)))
+        return x
+
+    def release(self, slot):
+        _check(lib().pgmg_batch_release(self.h, slot))
+        self.shapes.pop(slot, None)
+
+    def free_bytes(self):
+        v = ctypes.c_uint64()
+        _check(lib().pgmg_batch_free_bytes(self.h, ctypes.byref(v)))
+        return v.value
+
+    def serve(self, requests):
+        """requests: list of dicts (arrival_s, m, n, o, family, seed, tol,
+        max_cycles).  Returns (results as a list of dicts, wall seconds)."""
+        n = len(requests)
+        R = (Request * n)()
+        for q, r in enumerate(requests):
+            R[q] = Request(r["arrival_s"], r["m"], r["n"], r["o"], r["family"], r["seed"], r["tol"], r.get("max_cycles", 100))
+        out = (Result * n)()
+        wall = ctypes.c_double()
+        _check(lib().pgmg_serve_run(self.h, R, n, out, ctypes.byref(wall)))
+        res = [dict(admit_s=o_.admit_s, done_s=o_.done_s, cycles=o_.cycles, status=o_.status,
+                    rel_residual=o_.rel_residual) for o_ in out]
+        return res, wall.value

@@ -52,6 +52,11 @@ struct Level {
 
 }  // namespace
 
+int pgmg::set_error(int code, const char *msg) {
+	g_err = msg;
+	return code;
+}
+
 struct pgmg_hier {
 	int device = 0;
 	hipStream_t stream = nullptr;

@@ -36,6 +36,53 @@ inline LevelView make_level_view(int m, int n, int o) {
 	return v;
 }
 
+// ---- batches of independent hierarchies (one per slot) ----
+const int kBatchMaxLevels = 16;
+// Device-side description of one slot's hierarchy.  Pointers of a precision
+// the batch does not use are null.
+struct SlotDev {
+	int nl;
+	int lu_active, lu_n;
+	int pad_;
+	LevelView lv[kBatchMaxLevels];
+	double *A[kBatchMaxLevels], *x[kBatchMaxLevels], *r[kBatchMaxLevels];
+	float *A32[kBatchMaxLevels], *x32[kBatchMaxLevels], *r32[kBatchMaxLevels];
+	double *res64;     // residual scratch, top-level size
+	float *res32;
+	double *partial;   // per-block sums of the top-level residual
+	double *LU;
+	int *ipiv;
+};
+// A list of slots and the cumulative number of work units of a batched
+// kernel over them: slot[q] owns units [prefix[q], prefix[q+1]).
+struct WorkList {
+	const int *slot;
+	const int *prefix;
+	int n;
+};
+// Unit kinds: the colour-pair row groups of a level, one unit per slot,
+// residual row groups, 64x4-point blocks of a level.
+enum { BK_PAIR0 = 0, BK_PAIR1, BK_PAIR2, BK_PAIR3, BK_ONE, BK_ROWS, BK_PTS };
+// Work units of one slot for a batched kernel kind at a level.
+int batch_units(int kind, const LevelView &L);
+
+// Batched launchers: one launch covers the level `level` of every slot in
+// the list; blocks walk the units grid-stride.
+template <typename T> void launch_b_gs_pair(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int pair);
+template <typename T> void launch_b_gs_small(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int nsweeps);
+template <typename T> void launch_b_residual(hipStream_t s, const SlotDev *slots, WorkList wl, int level);
+void launch_b_residual_top(hipStream_t s, const SlotDev *slots, WorkList wl, bool to_f32);
+void launch_b_reduce(hipStream_t s, const SlotDev *slots, WorkList wl, double *norms);
+template <typename T> void launch_b_restrict(hipStream_t s, const SlotDev *slots, WorkList wl, int level);
+template <typename T> void launch_b_interp(hipStream_t s, const SlotDev *slots, WorkList wl, int level);
+template <typename T> void launch_b_lu_solve(hipStream_t s, const SlotDev *slots, WorkList wl, int level);
+void launch_b_zero_x32(hipStream_t s, const SlotDev *slots, WorkList wl);
+void launch_b_add_x(hipStream_t s, const SlotDev *slots, WorkList wl);
+
+// Records the calling thread's last error message (pgmg_last_error) and
+// returns code.
+int set_error(int code, const char *msg);
+
 // Kernel classes for timing.
 enum { KC_NONE = 0, KC_GS = 1, KC_RESIDUAL = 2, KC_RESTRICT = 3, KC_INTERP = 4 };
 

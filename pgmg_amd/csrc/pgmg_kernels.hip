@@ -107,14 +107,10 @@ template <> struct Vec2<float> { typedef float2 type; };
 // changed.  Per-point arithmetic and the colour order are those of
 // k_gs_colour, so the result is bit-identical.
 template <typename T>
-__global__ __launch_bounds__(256) PGMG_STREAM_ATTR void k_gs_pair(LevelView L, const T *__restrict__ A, T *__restrict__ x,
-						 const T *__restrict__ r, int cj, int ck) {
+__device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restrict__ A, T *__restrict__ x,
+					    const T *__restrict__ r, int j, int k) {
 	typedef typename Vec2<T>::type V2;
 	const int lane = threadIdx.x;
-	const int nb = (L.n - cj + 1) >> 1, nd = (L.o - ck + 1) >> 1;
-	const int q = xcd_block((int)blockIdx.x, (int)gridDim.x) * 4 + (int)threadIdx.y;
-	if (q >= nb * nd) return;  // uniform per wavefront
-	const int j = 2 * (q % nb) + cj, k = 2 * (q / nb) + ck;
 	const long long row = idx3(L, 0, j, k);
 	const long long px = L.px, pxy = L.pxy, np = L.npad;
 	T carry = (T)0;
@@ -188,6 +184,15 @@ __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void k_gs_pair(LevelView L, c
 }
 
 template <typename T>
+__global__ __launch_bounds__(256) PGMG_STREAM_ATTR void k_gs_pair(LevelView L, const T *__restrict__ A, T *__restrict__ x,
+						 const T *__restrict__ r, int cj, int ck) {
+	const int nb = (L.n - cj + 1) >> 1, nd = (L.o - ck + 1) >> 1;
+	const int q = xcd_block((int)blockIdx.x, (int)gridDim.x) * 4 + (int)threadIdx.y;
+	if (q >= nb * nd) return;  // uniform per wavefront
+	gs_pair_row<T>(L, A, x, r, 2 * (q % nb) + cj, 2 * (q / nb) + ck);
+}
+
+template <typename T>
 void launch_gs_pair(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int pair) {
 	const int cj = pair & 1, ck = (pair >> 1) & 1;
 	const int nb = (L.n - cj + 1) >> 1, nd = (L.o - ck + 1) >> 1;
@@ -234,8 +239,8 @@ void launch_gs_plane(hipStream_t s, const LevelView &L, const T *A, T *x, const 
 // launch, ordered by workgroup barriers.  Exact mode walks the hyperplanes;
 // colour mode the eight classes.
 template <typename T>
-__global__ __launch_bounds__(1024) void k_gs_small(LevelView L, const T *__restrict__ A, T *__restrict__ x,
-						   const T *__restrict__ r, int nsweeps, int mode, int exact) {
+__device__ __forceinline__ void gs_small_body(const LevelView &L, const T *__restrict__ A, T *__restrict__ x,
+					      const T *__restrict__ r, int nsweeps, int mode, int exact) {
 	const int nt = blockDim.x, tid = threadIdx.x;
 	const int tmax = (L.m - 1) + 2 * (L.n - 1) + 4 * (L.o - 1);
 	const int npair = L.n * L.o;
@@ -269,6 +274,12 @@ __global__ __launch_bounds__(1024) void k_gs_small(LevelView L, const T *__restr
 }
 
 template <typename T>
+__global__ __launch_bounds__(1024) void k_gs_small(LevelView L, const T *__restrict__ A, T *__restrict__ x,
+						   const T *__restrict__ r, int nsweeps, int mode, int exact) {
+	gs_small_body<T>(L, A, x, r, nsweeps, mode, exact);
+}
+
+template <typename T>
 void launch_gs_small(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int nsweeps, int mode, int exact) {
 	hipLaunchKernelGGL(k_gs_small<T>, dim3(1), dim3(1024), 0, s, L, A, x, r, nsweeps, mode, exact);
 	HIPCHECK_LAUNCH();
@@ -294,12 +305,11 @@ __device__ __forceinline__ double block_sum_256(double v) {
 // blocks of 4 rows, renumbered per XCD.  res (may be null) receives the
 // residual in type TR, partial (may be null) one sum per block.
 template <typename T, typename TR>
-__global__ __launch_bounds__(256) PGMG_STREAM_ATTR void k_residual(LevelView L, const T *__restrict__ A, const T *__restrict__ x,
-						  const T *__restrict__ r, TR *__restrict__ res, double *__restrict__ partial) {
+__device__ __forceinline__ double residual_row(const LevelView &L, const T *__restrict__ A, const T *__restrict__ x,
+					       const T *__restrict__ r, TR *__restrict__ res, int q) {
 	typedef typename Vec2<T>::type V2;
 	typedef typename Vec2<TR>::type R2;
 	const int lane = threadIdx.x;
-	const int q = xcd_block((int)blockIdx.x, (int)gridDim.x) * 4 + (int)threadIdx.y;
 	const long long px = L.px, pxy = L.pxy, np = L.npad;
 	double v2 = 0.0;
 	if (q < L.n * L.o) {
@@ -338,9 +348,18 @@ __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void k_residual(LevelView L, 
 			if (two) v2 = v2 + (double)rv1 * (double)rv1;
 		}
 	}
+	return v2;
+}
+
+template <typename T, typename TR>
+__global__ __launch_bounds__(256) PGMG_STREAM_ATTR void k_residual(LevelView L, const T *__restrict__ A, const T *__restrict__ x,
+						  const T *__restrict__ r, TR *__restrict__ res, double *__restrict__ partial) {
+	const int qb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+	const double v2 = residual_row<T, TR>(L, A, x, r, res, qb * 4 + (int)threadIdx.y);
 	if (partial) {
+		// indexed by row group, so the sum order is that of a batched launch
 		const double sblk = block_sum_256(v2);
-		if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.x] = sblk;
+		if (threadIdx.x == 0 && threadIdx.y == 0) partial[qb] = sblk;
 	}
 }
 
@@ -413,10 +432,8 @@ void launch_reduce(hipStream_t s, const double *partial, int n, double *out) {
 // (region.cc:888-895); that is reproduced.  Also clears coarse x
 // (multigrid.cc:219, region.cc:1679-1682).
 template <typename T>
-__global__ __launch_bounds__(256) void k_restrict(LevelView F, LevelView C, const T *__restrict__ res,
-						  T *__restrict__ rc, T *__restrict__ xc) {
-	const int I = (int)(blockIdx.x * 64 + threadIdx.x), J = (int)(blockIdx.y * 4 + threadIdx.y), K = (int)blockIdx.z;
-	if (I >= C.m || J >= C.n) return;
+__device__ __forceinline__ void restrict_point(const LevelView &F, const LevelView &C, const T *__restrict__ res,
+					       T *__restrict__ rc, T *__restrict__ xc, int I, int J, int K) {
 	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
 	const bool evx = !(F.m & 1), evy = !(F.n & 1), evz = !(F.o & 1);
 	T acc = (T)0;
@@ -468,6 +485,14 @@ __global__ __launch_bounds__(256) void k_restrict(LevelView F, LevelView C, cons
 }
 
 template <typename T>
+__global__ __launch_bounds__(256) void k_restrict(LevelView F, LevelView C, const T *__restrict__ res,
+						  T *__restrict__ rc, T *__restrict__ xc) {
+	const int I = (int)(blockIdx.x * 64 + threadIdx.x), J = (int)(blockIdx.y * 4 + threadIdx.y), K = (int)blockIdx.z;
+	if (I >= C.m || J >= C.n) return;
+	restrict_point<T>(F, C, res, rc, xc, I, J, K);
+}
+
+template <typename T>
 void launch_restrict(hipStream_t s, const LevelView &F, const LevelView &C, const T *res, T *rc, T *xc) {
 	dim3 grid((C.m + 63) / 64, (C.n + 3) / 4, C.o), block(64, 4, 1);
 	hipLaunchKernelGGL(k_restrict<T>, grid, block, 0, s, F, C, res, rc, xc);
@@ -479,9 +504,8 @@ void launch_restrict(hipStream_t s, const LevelView &F, const LevelView &C, cons
 // interpolate_full's association order (region.cc:764-771), cubes at the
 // last position of any axis interpolate_partial's (region.cc:795-809).
 template <typename T>
-__global__ __launch_bounds__(256) void k_interp(LevelView C, LevelView F, const T *__restrict__ xc, T *__restrict__ xf) {
-	const int i = (int)(blockIdx.x * 64 + threadIdx.x), j = (int)(blockIdx.y * 4 + threadIdx.y), k = (int)blockIdx.z;
-	if (i >= F.m || j >= F.n) return;
+__device__ __forceinline__ void interp_point(const LevelView &C, const LevelView &F, const T *__restrict__ xc,
+					     T *__restrict__ xf, int i, int j, int k) {
 	const int ic = i >> 1, jc = j >> 1, kc = k >> 1, li = i & 1, lj = j & 1, lk = k & 1;
 	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
 	const int i1 = ic + 1 < C.m ? ic + 1 : ic, j1 = jc + 1 < C.n ? jc + 1 : jc, k1 = kc + 1 < C.o ? kc + 1 : kc;
@@ -523,6 +547,13 @@ __global__ __launch_bounds__(256) void k_interp(LevelView C, LevelView F, const 
 	}
 	const long long p = idx3(F, i, j, k);
 	xf[p] = xf[p] + val;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_interp(LevelView C, LevelView F, const T *__restrict__ xc, T *__restrict__ xf) {
+	const int i = (int)(blockIdx.x * 64 + threadIdx.x), j = (int)(blockIdx.y * 4 + threadIdx.y), k = (int)blockIdx.z;
+	if (i >= F.m || j >= F.n) return;
+	interp_point<T>(C, F, xc, xf, i, j, k);
 }
 
 template <typename T>
@@ -742,8 +773,8 @@ void launch_lu_factor(hipStream_t s, int N, double *LU, int *ipiv, int *info) {
 // column-oriented forward and back substitution in FP64, one barrier per
 // column.  r and x are in the level layout and type T.
 template <typename T>
-__global__ __launch_bounds__(1024) void k_lu_solve(LevelView L, int N, const double *__restrict__ LU,
-						   const int *__restrict__ ipiv, const T *__restrict__ r, T *__restrict__ x) {
+__device__ __forceinline__ void lu_solve_body(const LevelView &L, int N, const double *__restrict__ LU,
+					      const int *__restrict__ ipiv, const T *__restrict__ r, T *__restrict__ x) {
 	__shared__ double xs[1024];
 	const int tid = threadIdx.x, nt = blockDim.x;
 	for (int q = tid; q < N; q += nt) {
@@ -776,6 +807,12 @@ __global__ __launch_bounds__(1024) void k_lu_solve(LevelView L, int N, const dou
 		const int i = q % L.m, j = (q / L.m) % L.n, k = q / (L.m * L.n);
 		x[idx3(L, i, j, k)] = (T)xs[q];
 	}
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_lu_solve(LevelView L, int N, const double *__restrict__ LU,
+						   const int *__restrict__ ipiv, const T *__restrict__ r, T *__restrict__ x) {
+	lu_solve_body<T>(L, N, LU, ipiv, r, x);
 }
 
 template <typename T>
@@ -886,6 +923,236 @@ void launch_add_f32_to_f64(hipStream_t s, const LevelView &L, const float *e, do
 	HIPCHECK_LAUNCH();
 }
 
+// ------------------------------------------------------ batched kernels
+//
+// A batch is a set of independent hierarchies (slots) of any sizes.  One
+// launch applies one operation to one level of every slot in a work list;
+// blocks walk the list's units grid-stride and find their slot by binary
+// search over the cumulative unit counts.  The per-row and per-point bodies
+// are those of the single-hierarchy kernels, so a slot's arithmetic does not
+// depend on which other slots share its launches.
+
+int batch_units(int kind, const LevelView &L) {
+	switch (kind) {
+	case BK_PAIR0: case BK_PAIR1: case BK_PAIR2: case BK_PAIR3: {
+		const int cj = kind & 1, ck = (kind >> 1) & 1;
+		const int rows = ((L.n - cj + 1) >> 1) * ((L.o - ck + 1) >> 1);
+		return (rows + 3) / 4;
+	}
+	case BK_ONE: return 1;
+	case BK_ROWS: return (L.n * L.o + 3) / 4;
+	default: return ((L.m + 63) / 64) * ((L.n + 3) / 4) * L.o;
+	}
+}
+
+__device__ __forceinline__ int wl_find(const WorkList &wl, int u) {
+	int lo = 0, hi = wl.n - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (wl.prefix[mid] <= u) lo = mid; else hi = mid - 1;
+	}
+	return lo;
+}
+
+template <typename T> __device__ __forceinline__ T *sA(const SlotDev *S, int l);
+template <typename T> __device__ __forceinline__ T *sx(const SlotDev *S, int l);
+template <typename T> __device__ __forceinline__ T *sr(const SlotDev *S, int l);
+template <typename T> __device__ __forceinline__ T *sres(const SlotDev *S);
+template <> __device__ __forceinline__ double *sA<double>(const SlotDev *S, int l) { return S->A[l]; }
+template <> __device__ __forceinline__ double *sx<double>(const SlotDev *S, int l) { return S->x[l]; }
+template <> __device__ __forceinline__ double *sr<double>(const SlotDev *S, int l) { return S->r[l]; }
+template <> __device__ __forceinline__ double *sres<double>(const SlotDev *S) { return S->res64; }
+template <> __device__ __forceinline__ float *sA<float>(const SlotDev *S, int l) { return S->A32[l]; }
+template <> __device__ __forceinline__ float *sx<float>(const SlotDev *S, int l) { return S->x32[l]; }
+template <> __device__ __forceinline__ float *sr<float>(const SlotDev *S, int l) { return S->r32[l]; }
+template <> __device__ __forceinline__ float *sres<float>(const SlotDev *S) { return S->res32; }
+
+// Decodes a 64x4-point block unit of level view L into the thread's point.
+__device__ __forceinline__ bool pts_unit(const LevelView &L, int lu, int &i, int &j, int &k) {
+	const int gx = (L.m + 63) / 64, gy = (L.n + 3) / 4;
+	i = (lu % gx) * 64 + (int)threadIdx.x;
+	j = ((lu / gx) % gy) * 4 + (int)threadIdx.y;
+	k = lu / (gx * gy);
+	return i < L.m && j < L.n;
+}
+
+#define BATCH_LOOP                                                   \
+	const int total_ = wl.prefix[wl.n];                          \
+	for (int u = (int)blockIdx.x; u < total_; u += (int)gridDim.x) { \
+		const int q_ = wl_find(wl, u);                       \
+		const SlotDev *S = slots + wl.slot[q_];              \
+		const int lu = u - wl.prefix[q_];
+
+template <typename T>
+__global__ __launch_bounds__(256) PGMG_STREAM_ATTR void kb_gs_pair(const SlotDev *__restrict__ slots, WorkList wl,
+								   int level, int cj, int ck) {
+	BATCH_LOOP
+		const LevelView L = S->lv[level];
+		const int nb = (L.n - cj + 1) >> 1, nd = (L.o - ck + 1) >> 1;
+		const int row = lu * 4 + (int)threadIdx.y;
+		if (row < nb * nd)
+			gs_pair_row<T>(L, sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), 2 * (row % nb) + cj, 2 * (row / nb) + ck);
+	}
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void kb_gs_small(const SlotDev *__restrict__ slots, WorkList wl, int level, int nsweeps) {
+	BATCH_LOOP
+		(void)lu;
+		gs_small_body<T>(S->lv[level], sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), nsweeps, 0, 0);
+		__syncthreads();
+	}
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) PGMG_STREAM_ATTR void kb_residual(const SlotDev *__restrict__ slots, WorkList wl, int level) {
+	BATCH_LOOP
+		const LevelView L = S->lv[level];
+		(void)residual_row<T, T>(L, sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), sres<T>(S), lu * 4 + (int)threadIdx.y);
+	}
+}
+
+// Top-level FP64 residual with per-slot block sums; optionally the residual
+// rounded to FP32 into the slot's level-0 r32 (the defect-correction source).
+__global__ __launch_bounds__(256) PGMG_STREAM_ATTR void kb_residual_top(const SlotDev *__restrict__ slots, WorkList wl, int to_f32) {
+	BATCH_LOOP
+		const LevelView L = S->lv[0];
+		const int q = lu * 4 + (int)threadIdx.y;
+		const double v2 = to_f32 ? residual_row<double, float>(L, S->A[0], S->x[0], S->r[0], S->r32[0], q)
+					 : residual_row<double, double>(L, S->A[0], S->x[0], S->r[0], (double *)nullptr, q);
+		const double sblk = block_sum_256(v2);
+		if (threadIdx.x == 0 && threadIdx.y == 0) S->partial[lu] = sblk;
+		__syncthreads();
+	}
+}
+
+__global__ __launch_bounds__(1024) void kb_reduce(const SlotDev *__restrict__ slots, WorkList wl, double *__restrict__ norms) {
+	__shared__ double red[1024];
+	BATCH_LOOP
+		(void)lu;
+		const LevelView L = S->lv[0];
+		const int n = (L.n * L.o + 3) / 4;
+		double s = 0.0;
+		for (int t = threadIdx.x; t < n; t += 1024) s = s + S->partial[t];
+		red[threadIdx.x] = s;
+		__syncthreads();
+		for (int w = 512; w > 0; w >>= 1) {
+			if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
+			__syncthreads();
+		}
+		if (threadIdx.x == 0) norms[wl.slot[q_]] = red[0];
+		__syncthreads();
+	}
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_restrict(const SlotDev *__restrict__ slots, WorkList wl, int level) {
+	BATCH_LOOP
+		const LevelView C = S->lv[level];
+		int I, J, K;
+		if (pts_unit(C, lu, I, J, K))
+			restrict_point<T>(S->lv[level - 1], C, sres<T>(S), sr<T>(S, level), sx<T>(S, level), I, J, K);
+	}
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kb_interp(const SlotDev *__restrict__ slots, WorkList wl, int level) {
+	BATCH_LOOP
+		const LevelView F = S->lv[level - 1];
+		int i, j, k;
+		if (pts_unit(F, lu, i, j, k)) interp_point<T>(S->lv[level], F, sx<T>(S, level), sx<T>(S, level - 1), i, j, k);
+	}
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void kb_lu_solve(const SlotDev *__restrict__ slots, WorkList wl, int level) {
+	BATCH_LOOP
+		(void)lu;
+		lu_solve_body<T>(S->lv[level], S->lu_n, S->LU, S->ipiv, sr<T>(S, level), sx<T>(S, level));
+		__syncthreads();
+	}
+}
+
+__global__ __launch_bounds__(256) void kb_zero_x32(const SlotDev *__restrict__ slots, WorkList wl) {
+	BATCH_LOOP
+		const LevelView L = S->lv[0];
+		int i, j, k;
+		if (pts_unit(L, lu, i, j, k)) S->x32[0][idx3(L, i, j, k)] = 0.0f;
+	}
+}
+
+__global__ __launch_bounds__(256) void kb_add_x(const SlotDev *__restrict__ slots, WorkList wl) {
+	BATCH_LOOP
+		const LevelView L = S->lv[0];
+		int i, j, k;
+		if (pts_unit(L, lu, i, j, k)) {
+			const long long p = idx3(L, i, j, k);
+			S->x[0][p] = S->x[0][p] + (double)S->x32[0][p];
+		}
+	}
+}
+#undef BATCH_LOOP
+
+// Persistent grid sizes: enough blocks to fill 256 CUs several times over.
+static const int kBatchBlocks256 = 4096, kBatchBlocks1024 = 1024;
+
+template <typename T>
+void launch_b_gs_pair(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int pair) {
+	if (wl.n <= 0) return;
+	hipLaunchKernelGGL(kb_gs_pair<T>, dim3(kBatchBlocks256), dim3(64, 4), 0, s, slots, wl, level, pair & 1, (pair >> 1) & 1);
+	HIPCHECK_LAUNCH();
+}
+template <typename T>
+void launch_b_gs_small(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int nsweeps) {
+	if (wl.n <= 0) return;
+	hipLaunchKernelGGL(kb_gs_small<T>, dim3(kBatchBlocks1024), dim3(1024), 0, s, slots, wl, level, nsweeps);
+	HIPCHECK_LAUNCH();
+}
+template <typename T>
+void launch_b_residual(hipStream_t s, const SlotDev *slots, WorkList wl, int level) {
+	if (wl.n <= 0) return;
+	hipLaunchKernelGGL(kb_residual<T>, dim3(kBatchBlocks256), dim3(64, 4), 0, s, slots, wl, level);
+	HIPCHECK_LAUNCH();
+}
+void launch_b_residual_top(hipStream_t s, const SlotDev *slots, WorkList wl, bool to_f32) {
+	if (wl.n <= 0) return;
+	hipLaunchKernelGGL(kb_residual_top, dim3(kBatchBlocks256), dim3(64, 4), 0, s, slots, wl, to_f32 ? 1 : 0);
+	HIPCHECK_LAUNCH();
+}
+void launch_b_reduce(hipStream_t s, const SlotDev *slots, WorkList wl, double *norms) {
+	if (wl.n <= 0) return;
+	hipLaunchKernelGGL(kb_reduce, dim3(kBatchBlocks1024), dim3(1024), 0, s, slots, wl, norms);
+	HIPCHECK_LAUNCH();
+}
+template <typename T>
+void launch_b_restrict(hipStream_t s, const SlotDev *slots, WorkList wl, int level) {
+	if (wl.n <= 0) return;
+	hipLaunchKernelGGL(kb_restrict<T>, dim3(kBatchBlocks256), dim3(64, 4), 0, s, slots, wl, level);
+	HIPCHECK_LAUNCH();
+}
+template <typename T>
+void launch_b_interp(hipStream_t s, const SlotDev *slots, WorkList wl, int level) {
+	if (wl.n <= 0) return;
+	hipLaunchKernelGGL(kb_interp<T>, dim3(kBatchBlocks256), dim3(64, 4), 0, s, slots, wl, level);
+	HIPCHECK_LAUNCH();
+}
+template <typename T>
+void launch_b_lu_solve(hipStream_t s, const SlotDev *slots, WorkList wl, int level) {
+	if (wl.n <= 0) return;
+	hipLaunchKernelGGL(kb_lu_solve<T>, dim3(kBatchBlocks1024), dim3(1024), 0, s, slots, wl, level);
+	HIPCHECK_LAUNCH();
+}
+void launch_b_zero_x32(hipStream_t s, const SlotDev *slots, WorkList wl) {
+	if (wl.n <= 0) return;
+	hipLaunchKernelGGL(kb_zero_x32, dim3(kBatchBlocks256), dim3(64, 4), 0, s, slots, wl);
+	HIPCHECK_LAUNCH();
+}
+void launch_b_add_x(hipStream_t s, const SlotDev *slots, WorkList wl) {
+	if (wl.n <= 0) return;
+	hipLaunchKernelGGL(kb_add_x, dim3(kBatchBlocks256), dim3(64, 4), 0, s, slots, wl);
+	HIPCHECK_LAUNCH();
+}
+
 // ------------------------------------------------------ instantiations
 #define INST(T) \
 	template void launch_gs_colour<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int); \
@@ -896,7 +1163,13 @@ void launch_add_f32_to_f64(hipStream_t s, const LevelView &L, const float *e, do
 	template void launch_restrict<T>(hipStream_t, const LevelView &, const LevelView &, const T *, T *, T *); \
 	template void launch_interp<T>(hipStream_t, const LevelView &, const LevelView &, const T *, T *); \
 	template void launch_lu_solve<T>(hipStream_t, const LevelView &, int, const double *, const int *, const T *, T *); \
-	template void launch_fill<T>(hipStream_t, T *, long long, T);
+	template void launch_fill<T>(hipStream_t, T *, long long, T); \
+	template void launch_b_gs_pair<T>(hipStream_t, const SlotDev *, WorkList, int, int); \
+	template void launch_b_gs_small<T>(hipStream_t, const SlotDev *, WorkList, int, int); \
+	template void launch_b_residual<T>(hipStream_t, const SlotDev *, WorkList, int); \
+	template void launch_b_restrict<T>(hipStream_t, const SlotDev *, WorkList, int); \
+	template void launch_b_interp<T>(hipStream_t, const SlotDev *, WorkList, int); \
+	template void launch_b_lu_solve<T>(hipStream_t, const SlotDev *, WorkList, int);
 INST(double)
 INST(float)
 template int launch_residual<double, double>(hipStream_t, const LevelView &, const double *, const double *, const double *, double *, double *);

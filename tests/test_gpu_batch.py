@@ -1,0 +1,126 @@
+"""Batches and serving on the GPU: every slot's result equals the lone
+hierarchy's bit for bit (tolerance 0), alone or in any mix; admission by arena
+space; the open-loop serving loop completes every request."""
+import sys
+
+import numpy as np
+import pytest
+
+import pgmg_amd as P
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+PROBLEMS = [
+    # (m, n, o, family, seed)
+    (17, 13, 22, P.FAM_Q1_ANISO, 3),
+    (33, 33, 33, P.FAM_FACE7_UNIFORM, 4),
+    (20, 17, 30, P.FAM_FACE7_UNIFORM, 5),
+    (40, 21, 9, P.FAM_Q1_ANISO, 6),
+    (9, 9, 9, P.FAM_FACE7_UNIFORM, 7),
+    (65, 33, 17, P.FAM_FACE7_LOGUNI, 8),
+]
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def alone(m, n, o, fam, seed, precision, bottom, tol, maxc):
+    h = P.Hierarchy(m, n, o, precision=precision, smoother=P.SMOOTH_COLOUR8, bottom=bottom)
+    h.generate_problem(fam, seed)
+    h.compute_rats()
+    c, hist = h.solve(tol, maxc, check=False)
+    x = h.get_x(0)
+    h.close()
+    return c, x
+
+
+@pytest.mark.parametrize("precision", [P.PREC_MIXED, P.PREC_FP64])
+@pytest.mark.parametrize("bottom", [P.BOTTOM_GS24, P.BOTTOM_LU])
+def test_batch_equals_alone(precision, bottom):
+    say("batch", precision, bottom)
+    tol, maxc = 1e-8, 40
+    ref = [alone(m, n, o, f, s, precision, bottom, tol, maxc) for (m, n, o, f, s) in PROBLEMS]
+    with P.Batch(2 << 30, max_slots=16, precision=precision, bottom=bottom) as b:
+        slots = [b.admit(m, n, o, f, s, tol, maxc) for (m, n, o, f, s) in PROBLEMS]
+        assert None not in slots
+        b.run()
+        for q, s in enumerate(slots):
+            st, c, rel = b.status(s)
+            assert c == ref[q][0], (q, c, ref[q][0])
+            assert np.array_equal(b.get_x(s), ref[q][1]), q
+            b.release(s)
+
+
+def test_alone_vs_mix_in_batch():
+    """One request solved by itself in a batch and inside a mix of others
+    admitted at different times: identical bits."""
+    tol = 1e-8
+    m, n, o, f, s = 24, 19, 31, P.FAM_Q1_ANISO, 99
+    with P.Batch(1 << 30, max_slots=8) as b:
+        sl = b.admit(m, n, o, f, s, tol)
+        b.run()
+        x_alone = b.get_x(sl)
+        b.release(sl)
+    with P.Batch(2 << 30, max_slots=8) as b:
+        others = [b.admit(*p[:3], p[3], p[4], tol) for p in PROBLEMS[:3]]
+        b.step()
+        b.step()
+        sl = b.admit(m, n, o, f, s, tol)  # joins while the others are mid-solve
+        others += [b.admit(*p[:3], p[3], p[4], tol) for p in PROBLEMS[3:]]
+        b.run()
+        assert b.status(sl)[0] == P.SLOT_CONVERGED
+        assert np.array_equal(b.get_x(sl), x_alone)
+
+
+def test_host_operator_admission_matches_oracle_solution():
+    m, n, o = 21, 18, 15
+    A = O.gen_operator(P.FAM_FACE7_UNIFORM, 31, m, n, o)
+    bvec = O.gen_rhs(31, m, n, o)
+    mg = O.Multigrid(A, colour8=True)
+    mg.set_rhs(bvec)
+    c, hist = mg.solve(1e-9, 40, mixed=True)
+    with P.Batch(1 << 30, max_slots=4) as b:
+        s = b.admit_operator(A, bvec, 1e-9)
+        b.run()
+        st, cb, rel = b.status(s)
+        assert st == P.SLOT_CONVERGED and cb == c
+        assert np.array_equal(b.get_x(s), mg.get_x(0))
+
+
+def test_admission_by_arena_space():
+    fp = P.footprint(65, 65, 65, precision=P.PREC_MIXED)
+    with P.Batch(int(fp * 2.5), max_slots=8) as b:
+        s1 = b.admit(65, 65, 65, P.FAM_FACE7_UNIFORM, 1, 1e-8)
+        s2 = b.admit(65, 65, 65, P.FAM_FACE7_UNIFORM, 2, 1e-8)
+        assert s1 is not None and s2 is not None
+        assert b.admit(65, 65, 65, P.FAM_FACE7_UNIFORM, 3, 1e-8) is None  # no room
+        b.run()
+        b.release(s1)
+        s3 = b.admit(65, 65, 65, P.FAM_FACE7_UNIFORM, 3, 1e-8)
+        assert s3 is not None
+        b.run()
+        assert b.status(s3)[0] == P.SLOT_CONVERGED
+    with P.Batch(1 << 28, max_slots=1) as b:
+        assert b.admit(9, 9, 9, 1, 1, 1e-8) is not None
+        assert b.admit(9, 9, 9, 1, 2, 1e-8) is None  # no free slot
+
+
+def test_serving_open_loop_small():
+    rng = np.random.default_rng(5)
+    reqs = []
+    t = 0.0
+    for q in range(24):
+        t += rng.exponential(0.004)
+        dims = [int(round(np.exp(rng.uniform(np.log(17), np.log(48))))) for _ in range(3)]
+        reqs.append(dict(arrival_s=t, m=dims[0], n=dims[1], o=dims[2], family=P.FAM_Q1_ANISO, seed=1000 + q,
+                         tol=1e-8, max_cycles=60))
+    with P.Batch(4 << 30, max_slots=16) as b:
+        res, wall = b.serve(reqs)
+        assert b.free_bytes() == 4 << 30
+    for r, q in zip(res, reqs):
+        assert r["status"] == 0 and r["rel_residual"] < 1e-8
+        assert r["admit_s"] >= q["arrival_s"] - 1e-9
+        assert r["done_s"] > r["admit_s"]
+    assert wall > 0
