@@ -88,6 +88,90 @@ def cpu_baseline():
             "sample": "oracle 1 V-cycle at 129^3, x8 for 257^3, x10 cycles"}
 
 
+def q1_513(P):
+    """513^3 Q1 finite elements with per-cell anisotropic tensors: mixed
+    precision to 1e-10 and its error against the FP64 solve."""
+    n = 513
+    out = {}
+    xs = {}
+    for prec, name in [(P.PREC_FP64, "fp64"), (P.PREC_MIXED, "mixed")]:
+        h = P.Hierarchy(n, n, n, precision=prec, smoother=P.SMOOTH_COLOUR8, use_graph=False)
+        h.generate_problem(P.FAM_Q1_ANISO, 777)
+        t0 = time.perf_counter()
+        h.compute_rats()
+        h.synchronize()
+        ts = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        c, hist = h.solve(TOL, 100)
+        h.synchronize()
+        t = time.perf_counter() - t0
+        bn = h.rhs_l2norm()
+        xs[name] = h.get_x(0)
+        out[name] = {"solve_s": round(t, 4), "cycles": c, "setup_s": round(ts, 3),
+                     "rel_residual": float((hist[-1] / bn) ** 0.5)}
+        h.close()
+        log("q1 513 %s: %.3f s, %d cycles" % (name, t, c))
+    out["mixed"]["err_vs_fp64"] = float(np.linalg.norm(xs["mixed"] - xs["fp64"]) / np.linalg.norm(xs["fp64"]))
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["solve_s"] / out["mixed"]["solve_s"], 3)
+    return out
+
+
+def batch_256(P, precision, tol=1e-8, count=256, max_cycles=1000):
+    """256 independent problems, axes drawn from {33, 65, 129}, per-face
+    a = 10^U[-2,2], solved together to tol."""
+    rng = np.random.default_rng(2024)
+    sizes = [tuple(int(v) for v in rng.choice([33, 65, 129], size=3)) for _ in range(count)]
+    need = sum(P.footprint(m, n, o, precision=precision) for (m, n, o) in sizes)
+    with P.Batch(int(need * 1.05) + (1 << 30), max_slots=count, precision=precision) as b:
+        # Timed from admission (generation and Galerkin set-up on the GPU
+        # included) to the last slot's convergence.
+        t0 = time.perf_counter()
+        slots = [b.admit(m, n, o, P.FAM_FACE7_LOGUNI, 5000 + q, tol, max_cycles) for q, (m, n, o) in enumerate(sizes)]
+        steps = b.run(max_steps=max_cycles + 5)
+        t = time.perf_counter() - t0
+        st = [b.status(s) for s in slots]
+    conv = sum(1 for s in st if s[0] == P.SLOT_CONVERGED)
+    pts = sum(m * n * o for (m, n, o) in sizes)
+    cyc = [s[1] for s in st]
+    return {"problems": count, "converged": conv, "solve_s": round(t, 3), "steps": steps,
+            "cycles_mean": round(float(np.mean(cyc)), 1), "cycles_max": int(max(cyc)),
+            "points_total": pts, "dof_cycles_per_s": round(pts * float(np.mean(cyc)) / t, 1),
+            "arena_gb": round(need / 1e9, 2)}
+
+
+def serving(P, n_req=160, seed=11):
+    """Open-loop Poisson arrivals of 27-point Q1 anisotropic problems with
+    axes log-uniform in [17, 257], mixed precision to 1e-8.  Saturation is
+    measured closed-loop (every request at t = 0), then the open loop runs at
+    50%, 80% and 95% of it."""
+    rng = np.random.default_rng(seed)
+    dims = [[int(round(np.exp(rng.uniform(np.log(17), np.log(257))))) for _ in range(3)] for _ in range(n_req)]
+    arena = 96 << 30
+
+    def reqs(times):
+        return [dict(arrival_s=float(t), m=d[0], n=d[1], o=d[2], family=P.FAM_Q1_ANISO, seed=9000 + q,
+                     tol=1e-8, max_cycles=60) for q, (t, d) in enumerate(zip(times, dims))]
+
+    out = {}
+    with P.Batch(arena, max_slots=64, precision=P.PREC_MIXED) as b:
+        res, wall = b.serve(reqs(np.zeros(n_req)))
+        sat = n_req / wall
+        out["saturation_solves_per_s"] = round(sat, 2)
+        for frac in (0.5, 0.8, 0.95):
+            qps = frac * sat
+            arr = np.cumsum(rng.exponential(1.0 / qps, size=n_req))
+            res, wall = b.serve(reqs(arr))
+            lat = np.array([r["done_s"] - a for r, a in zip(res, arr)])
+            ok = sum(1 for r in res if r["status"] == 0)
+            out["qps_%d%%" % int(frac * 100)] = {
+                "offered_qps": round(qps, 2), "solves_per_s": round(n_req / wall, 2), "completed": ok,
+                "p50_ms": round(1000 * float(np.percentile(lat, 50)), 2),
+                "p99_ms": round(1000 * float(np.percentile(lat, 99)), 2)}
+            log("serve %.0f%%: %s" % (100 * frac, out["qps_%d%%" % int(frac * 100)]))
+    out["requests"] = n_req
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +179,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 513^3, batch and serving legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -195,6 +280,13 @@ def main():
         "roofline_fp64_gs": {"achieved": round(240.0 * pts_gs64 / (ms_gs64 / 1000.0) / 1e9, 1),
                              "avg_launch_us": round(1000.0 * ms_gs64 / max(n_gs64, 1), 2)},
     }
+    if not args.no_extra:
+        out["q1_513"] = q1_513(P)
+        out["batch_256_mixed"] = batch_256(P, P.PREC_MIXED)
+        log("batch mixed:", out["batch_256_mixed"])
+        out["batch_256_fp64"] = batch_256(P, P.PREC_FP64)
+        log("batch fp64:", out["batch_256_fp64"])
+        out["serving"] = serving(P)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
