@@ -180,6 +180,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 513^3, batch and serving legs")
+    ap.add_argument("--probe", action="store_true",
+                    help="short run for hardware counters: a 1 GiB elementwise copy (torch.mul, "
+                         "calibration of known size) then two mixed 257^3 V-cycles")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -188,6 +191,23 @@ def main():
 
     import pgmg_amd as P
     pts = N ** 3
+
+    if args.probe:
+        import torch
+        a = torch.ones(1 << 28, device="cuda")  # 1 GiB of FP32
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 2.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
+        h = P.Hierarchy(N, N, N, precision=P.PREC_MIXED, smoother=P.SMOOTH_COLOUR8, use_graph=False)
+        h.generate_problem(P.FAM_FACE7_UNIFORM, SEED)
+        h.compute_rats()
+        h.solve(TOL, 2, check=False)  # two mixed cycles: FP32 level kernels
+        h.synchronize()
+        h.close()
+        print(json.dumps({"probe": "done", "copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30}), flush=True)
+        return
 
     # ---- mixed precision (headline) ----
     hm = P.Hierarchy(N, N, N, precision=P.PREC_MIXED, smoother=P.SMOOTH_COLOUR8, use_graph=False)

@@ -175,7 +175,8 @@ int pgmg_batch_admit(pgmg_batch *b, const pgmg_dims *dims, int family, uint64_t 
 int pgmg_batch_admit_operator(pgmg_batch *b, const pgmg_dims *dims, const double *A27, const double *rhs, double tol,
 			      int max_cycles, int *slot);
 /* One cycle for every active slot; reports how many remain active and how
- * many finished in this step. */
+ * many finished in this step.  When no slot is active but set-ups are
+ * pending, first waits for the oldest set-up to finish. */
 int pgmg_batch_step(pgmg_batch *b, int *active, int *finished);
 int pgmg_batch_slot_status(pgmg_batch *b, int slot, int *state, int *cycles, double *rel_residual);
 int pgmg_batch_get_x(pgmg_batch *b, int slot, double *x);

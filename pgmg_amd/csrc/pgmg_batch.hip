@@ -92,6 +92,7 @@ struct SlotHost {
 	int max_cycles = 0, cycles = 0;
 	double bn = 0, rel = 1;
 	bool want_lu = false;
+	uint64_t seq = 0;  // admission order
 	hipEvent_t ready = nullptr;
 	SlotDev dev;
 };
@@ -130,6 +131,7 @@ struct pgmg_batch {
 	double *d_aux = nullptr, *h_aux = nullptr;  // per slot: source norm, LU flag
 	int *d_info = nullptr;
 	int max_nl = 0;
+	uint64_t admitted = 0;
 };
 
 namespace {
@@ -340,6 +342,7 @@ int admit_common(pgmg_batch *b, const pgmg_dims *d, int family, uint64_t seed, c
 	S.max_cycles = max_cycles;
 	S.cycles = 0;
 	S.rel = 1;
+	S.seq = ++b->admitted;
 	int rc = slot_setup(b, s, d, family, seed, A27, rhs);
 	if (rc) { b->arena.release(S.off, S.bytes); S.state = ST_FREE; return rc; }
 	*slot = s;
@@ -414,6 +417,22 @@ int pgmg_batch_step(pgmg_batch *b, int *active, int *finished) {
 	if (!b) return bfail(PGMG_ERR_ARG, "null batch");
 	int rc = activate_ready(b);
 	if (rc) return rc;
+	{
+		// Nothing to cycle but set-ups in flight: wait for the oldest one
+		// (the set-up stream runs them in admission order).
+		bool any_active = false;
+		int oldest = -1;
+		for (int s = 0; s < b->max_slots; s++) {
+			const SlotHost &S = b->slots[s];
+			if (S.state == ST_ACTIVE) any_active = true;
+			if (S.state == ST_SETUP && (oldest < 0 || S.seq < b->slots[oldest].seq)) oldest = s;
+		}
+		if (!any_active && oldest >= 0) {
+			BHIP(hipEventSynchronize(b->slots[oldest].ready));
+			rc = activate_ready(b);
+			if (rc) return rc;
+		}
+	}
 	if (b->plan_dirty) { rc = build_plan(b); if (rc) return rc; }
 	int nfin = 0, nact = 0;
 	if (b->list_n[L_TOP_ROWS] > 0) {

@@ -11,13 +11,19 @@ namespace pgmg {
 
 #define HIPCHECK_LAUNCH() (void)hipGetLastError()
 
-// Minimum resident wavefronts per SIMD requested for the streaming stencil
-// kernels; it caps their register use so that enough wavefronts are in
-// flight to cover HBM latency.
+// Optional minimum of resident wavefronts per SIMD for the streaming stencil
+// kernels (0: the compiler's choice).  The row kernels issue every load of a
+// row segment at once, so their latency is covered within the wavefront;
+// forcing more wavefronts only makes the compiler spill (FP32 row kernels
+// use about 100 VGPRs, 4 wavefronts; FP64 about 190, 2 wavefronts).
 #ifndef PGMG_STREAM_WAVES
-#define PGMG_STREAM_WAVES 4
+#define PGMG_STREAM_WAVES 0
 #endif
+#if PGMG_STREAM_WAVES > 0
 #define PGMG_STREAM_ATTR __attribute__((amdgpu_waves_per_eu(PGMG_STREAM_WAVES)))
+#else
+#define PGMG_STREAM_ATTR
+#endif
 
 __device__ __forceinline__ long long idx3(const LevelView &L, int i, int j, int k) {
 	return L.base + i + (long long)L.px * j + L.pxy * k;
@@ -92,6 +98,21 @@ template <typename T> struct Vec2;
 template <> struct Vec2<double> { typedef double2 type; };
 template <> struct Vec2<float> { typedef float2 type; };
 
+// Row-streaming kernels address every array of a grid row as a
+// wavefront-uniform base (the row's start in that array, held in scalar
+// registers) plus one per-lane byte offset below the row pitch, so the
+// compiler emits the scalar-base form of global loads: one 32-bit offset
+// register serves all 27 planes and the x rows, instead of a 64-bit
+// address per plane.
+template <typename V>
+__device__ __forceinline__ V ld_row(const void *base, uint32_t off) {
+	return *(const V *)((const char *)base + off);
+}
+template <typename V>
+__device__ __forceinline__ void st_row(void *base, uint32_t off, V v) {
+	*(V *)((char *)base + off) = v;
+}
+
 // Colour classes (0,cj,ck) and then (1,cj,ck) in one launch.  One wavefront
 // owns one grid row (j,k) with j = cj (mod 2), k = ck (mod 2); only that
 // row is both read and written by the launch, because every other row a
@@ -105,7 +126,8 @@ template <> struct Vec2<float> { typedef float2 type; };
 // The lookahead value is written at once and handed to lane 0 of the next
 // segment, which must not recompute it: by then lane 62's odd neighbour has
 // changed.  Per-point arithmetic and the colour order are those of
-// k_gs_colour, so the result is bit-identical.
+// k_gs_colour, so the result is bit-identical.  j and k must be
+// wavefront-uniform.
 template <typename T>
 __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restrict__ A, T *__restrict__ x,
 					    const T *__restrict__ r, int j, int k) {
@@ -113,13 +135,16 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 	const int lane = threadIdx.x;
 	const long long row = idx3(L, 0, j, k);
 	const long long px = L.px, pxy = L.pxy, np = L.npad;
+	const T *Ar = A + row;
+	const T *rr_ = r + row;
+	T *xr = x + row;
 	T carry = (T)0;
 	for (int s0 = 0; s0 < L.m; s0 += 126) {
 		const int i0 = s0 + 2 * lane;
 		const bool live = i0 < L.m;
 		const bool doE = live && !(lane == 0 && s0 > 0);
 		const bool doO = lane < 63 && i0 + 1 < L.m;
-		const long long p = row + i0;
+		const uint32_t ob = (uint32_t)i0 * (uint32_t)sizeof(T);
 		// Streamed row by row (dj,dk): the even point's sum is formed as the
 		// loads arrive; the odd point's terms before its own row are summed
 		// at once, its own-row coefficients are kept, and its 12 terms after
@@ -130,18 +155,18 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 		T prod[12];
 		T xe = (T)0;
 		if (live) {
-			const V2 rr = *(const V2 *)(r + p);
+			const V2 rr = ld_row<V2>(rr_, ob);
 			rx = rr.x; ry = rr.y;
 #pragma unroll
 			for (int rq = 0; rq < 9; rq++) {
 				const int dj = rq % 3 - 1, dk = rq / 3 - 1;
-				const long long qq = p + dj * px + dk * pxy;
-				const V2 am = *(const V2 *)(A + (3 * rq) * np + p);
-				const V2 a0 = *(const V2 *)(A + (3 * rq + 1) * np + p);
-				const V2 ap = *(const V2 *)(A + (3 * rq + 2) * np + p);
-				const V2 xv = *(const V2 *)(x + qq);
-				const T xm = x[qq - 1];
-				const T xp = doO ? x[qq + 2] : (T)0;
+				const T *xq = xr + dj * px + dk * pxy;
+				const V2 am = ld_row<V2>(Ar + (3 * rq) * np, ob);
+				const V2 a0 = ld_row<V2>(Ar + (3 * rq + 1) * np, ob);
+				const V2 ap = ld_row<V2>(Ar + (3 * rq + 2) * np, ob);
+				const V2 xv = ld_row<V2>(xq, ob);
+				const T xm = ld_row<T>(xq - 1, ob);
+				const T xp = doO ? ld_row<T>(xq + 2, ob) : (T)0;
 				if (rq == 4) {
 					ax = ax + am.x * xm;
 					ax = ax + ap.x * xv.y;
@@ -165,7 +190,7 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 		if (doE) {
 			// even point i0: every neighbour still held its old value
 			xe = (rx - ax) * ((T)1 / ac_e);
-			x[p] = xe;
+			st_row<T>(xr, ob, xe);
 		} else if (live) {
 			xe = carry;  // lane 0: computed by lane 63 of the previous segment
 		}
@@ -177,7 +202,7 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 			ay = ay + a14 * xe_next;
 #pragma unroll
 			for (int q2 = 0; q2 < 12; q2++) ay = ay + prod[q2];
-			x[p + 1] = (ry - ay) * ((T)1 / ac_o);
+			st_row<T>(xr + 1, ob, (ry - ay) * ((T)1 / ac_o));
 		}
 		carry = __shfl(xe, 63, 64);
 	}
@@ -187,8 +212,9 @@ template <typename T>
 __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void k_gs_pair(LevelView L, const T *__restrict__ A, T *__restrict__ x,
 						 const T *__restrict__ r, int cj, int ck) {
 	const int nb = (L.n - cj + 1) >> 1, nd = (L.o - ck + 1) >> 1;
-	const int q = xcd_block((int)blockIdx.x, (int)gridDim.x) * 4 + (int)threadIdx.y;
-	if (q >= nb * nd) return;  // uniform per wavefront
+	// one row per wavefront: the row index is wavefront-uniform
+	const int q = __builtin_amdgcn_readfirstlane(xcd_block((int)blockIdx.x, (int)gridDim.x) * 4 + (int)threadIdx.y);
+	if (q >= nb * nd) return;
 	gs_pair_row<T>(L, A, x, r, 2 * (q % nb) + cj, 2 * (q / nb) + ck);
 }
 
@@ -301,9 +327,10 @@ __device__ __forceinline__ double block_sum_256(double v) {
 }
 
 // Residual r - A x and per-block sums of its square.  One wavefront per grid
-// row, each lane a pair of points read with vector loads (as in k_gs_pair);
-// blocks of 4 rows, renumbered per XCD.  res (may be null) receives the
-// residual in type TR, partial (may be null) one sum per block.
+// row, each lane a pair of points read with vector loads (as in k_gs_pair,
+// with the same uniform row bases); blocks of 4 rows, renumbered per XCD.
+// res (may be null) receives the residual in type TR, partial (may be null)
+// one sum per block.  q (the row) must be wavefront-uniform.
 template <typename T, typename TR>
 __device__ __forceinline__ double residual_row(const LevelView &L, const T *__restrict__ A, const T *__restrict__ x,
 					       const T *__restrict__ r, TR *__restrict__ res, int q) {
@@ -315,23 +342,25 @@ __device__ __forceinline__ double residual_row(const LevelView &L, const T *__re
 	if (q < L.n * L.o) {
 		const int j = q % L.n, k = q / L.n;
 		const long long row = idx3(L, 0, j, k);
+		const T *Ar = A + row, *xr = x + row, *rr_ = r + row;
+		TR *resr = res ? res + row : nullptr;
 		for (int s0 = 0; s0 < L.m; s0 += 128) {
 			const int i0 = s0 + 2 * lane;
 			if (i0 >= L.m) continue;
 			const bool two = i0 + 1 < L.m;
-			const long long p = row + i0;
-			const V2 rr = *(const V2 *)(r + p);
+			const uint32_t ob = (uint32_t)i0 * (uint32_t)sizeof(T);
+			const V2 rr = ld_row<V2>(rr_, ob);
 			T ax = (T)0, ay = (T)0;
 #pragma unroll
 			for (int rq = 0; rq < 9; rq++) {
 				const int dj = rq % 3 - 1, dk = rq / 3 - 1;
-				const long long qq = p + dj * px + dk * pxy;
-				const V2 am = *(const V2 *)(A + (3 * rq) * np + p);
-				const V2 a0 = *(const V2 *)(A + (3 * rq + 1) * np + p);
-				const V2 ap = *(const V2 *)(A + (3 * rq + 2) * np + p);
-				const V2 xv = *(const V2 *)(x + qq);
-				const T xm = x[qq - 1];
-				const T xp = two ? x[qq + 2] : (T)0;
+				const T *xq = xr + dj * px + dk * pxy;
+				const V2 am = ld_row<V2>(Ar + (3 * rq) * np, ob);
+				const V2 a0 = ld_row<V2>(Ar + (3 * rq + 1) * np, ob);
+				const V2 ap = ld_row<V2>(Ar + (3 * rq + 2) * np, ob);
+				const V2 xv = ld_row<V2>(xq, ob);
+				const T xm = ld_row<T>(xq - 1, ob);
+				const T xp = two ? ld_row<T>(xq + 2, ob) : (T)0;
 				ax = ax + am.x * xm;
 				ax = ax + a0.x * xv.x;
 				ax = ax + ap.x * xv.y;
@@ -340,9 +369,10 @@ __device__ __forceinline__ double residual_row(const LevelView &L, const T *__re
 				ay = ay + ap.y * xp;
 			}
 			const T rv0 = rr.x - ax, rv1 = rr.y - ay;
-			if (res) {
-				if (two) { R2 o; o.x = (TR)rv0; o.y = (TR)rv1; *(R2 *)(res + p) = o; }
-				else res[p] = (TR)rv0;
+			if (resr) {
+				const uint32_t obr = (uint32_t)i0 * (uint32_t)sizeof(TR);
+				if (two) { R2 o; o.x = (TR)rv0; o.y = (TR)rv1; st_row<R2>(resr, obr, o); }
+				else st_row<TR>(resr, obr, (TR)rv0);
 			}
 			v2 = v2 + (double)rv0 * (double)rv0;
 			if (two) v2 = v2 + (double)rv1 * (double)rv1;
@@ -355,7 +385,7 @@ template <typename T, typename TR>
 __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void k_residual(LevelView L, const T *__restrict__ A, const T *__restrict__ x,
 						  const T *__restrict__ r, TR *__restrict__ res, double *__restrict__ partial) {
 	const int qb = xcd_block((int)blockIdx.x, (int)gridDim.x);
-	const double v2 = residual_row<T, TR>(L, A, x, r, res, qb * 4 + (int)threadIdx.y);
+	const double v2 = residual_row<T, TR>(L, A, x, r, res, __builtin_amdgcn_readfirstlane(qb * 4 + (int)threadIdx.y));
 	if (partial) {
 		// indexed by row group, so the sum order is that of a batched launch
 		const double sblk = block_sum_256(v2);
@@ -979,7 +1009,7 @@ __device__ __forceinline__ bool pts_unit(const LevelView &L, int lu, int &i, int
 #define BATCH_LOOP                                                   \
 	const int total_ = wl.prefix[wl.n];                          \
 	for (int u = (int)blockIdx.x; u < total_; u += (int)gridDim.x) { \
-		const int q_ = wl_find(wl, u);                       \
+		const int q_ = __builtin_amdgcn_readfirstlane(wl_find(wl, u)); \
 		const SlotDev *S = slots + wl.slot[q_];              \
 		const int lu = u - wl.prefix[q_];
 
@@ -989,7 +1019,7 @@ __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void kb_gs_pair(const SlotDev
 	BATCH_LOOP
 		const LevelView L = S->lv[level];
 		const int nb = (L.n - cj + 1) >> 1, nd = (L.o - ck + 1) >> 1;
-		const int row = lu * 4 + (int)threadIdx.y;
+		const int row = __builtin_amdgcn_readfirstlane(lu * 4 + (int)threadIdx.y);
 		if (row < nb * nd)
 			gs_pair_row<T>(L, sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), 2 * (row % nb) + cj, 2 * (row / nb) + ck);
 	}
@@ -1008,7 +1038,7 @@ template <typename T>
 __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void kb_residual(const SlotDev *__restrict__ slots, WorkList wl, int level) {
 	BATCH_LOOP
 		const LevelView L = S->lv[level];
-		(void)residual_row<T, T>(L, sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), sres<T>(S), lu * 4 + (int)threadIdx.y);
+		(void)residual_row<T, T>(L, sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), sres<T>(S), __builtin_amdgcn_readfirstlane(lu * 4 + (int)threadIdx.y));
 	}
 }
 
@@ -1017,7 +1047,7 @@ __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void kb_residual(const SlotDe
 __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void kb_residual_top(const SlotDev *__restrict__ slots, WorkList wl, int to_f32) {
 	BATCH_LOOP
 		const LevelView L = S->lv[0];
-		const int q = lu * 4 + (int)threadIdx.y;
+		const int q = __builtin_amdgcn_readfirstlane(lu * 4 + (int)threadIdx.y);
 		const double v2 = to_f32 ? residual_row<double, float>(L, S->A[0], S->x[0], S->r[0], S->r32[0], q)
 					 : residual_row<double, double>(L, S->A[0], S->x[0], S->r[0], (double *)nullptr, q);
 		const double sblk = block_sum_256(v2);

@@ -32,6 +32,7 @@ def alone(m, n, o, fam, seed, precision, bottom, tol, maxc):
     h.compute_rats()
     c, hist = h.solve(tol, maxc, check=False)
     x = h.get_x(0)
+    say("alone", (m, n, o, fam, seed), "cycles", c, "lu_active", h.lu_active())
     h.close()
     return c, x
 
@@ -45,9 +46,11 @@ def test_batch_equals_alone(precision, bottom):
     with P.Batch(2 << 30, max_slots=16, precision=precision, bottom=bottom) as b:
         slots = [b.admit(m, n, o, f, s, tol, maxc) for (m, n, o, f, s) in PROBLEMS]
         assert None not in slots
-        b.run()
+        steps = b.run()
+        stats = [b.status(s) for s in slots]
+        say("steps", steps, "slots (state, cycles, rel):", stats, "alone cycles:", [r[0] for r in ref])
         for q, s in enumerate(slots):
-            st, c, rel = b.status(s)
+            st, c, rel = stats[q]
             assert c == ref[q][0], (q, c, ref[q][0])
             assert np.array_equal(b.get_x(s), ref[q][1]), q
             b.release(s)
