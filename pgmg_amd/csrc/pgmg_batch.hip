@@ -42,7 +42,6 @@ int bfail(int code, const char *fmt, ...) {
 		if (e_ != hipSuccess) return bfail(PGMG_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
 	} while (0)
 
-const long long kSmallColourB = 20000;  // as the lone hierarchy's threshold
 const int kGs24 = 24;
 const uint64_t kAlign = 256;
 
@@ -171,7 +170,10 @@ int build_plan(pgmg_batch *b) {
 		for (int l = 0; l < S.nl; l++) {
 			const LevelView &L = D.lv[l];
 			if (l < S.nl - 1) {
-				if ((long long)L.m * L.n * L.o <= kSmallColourB) plan_add(b, list_index(l, L_SMALL), s, 1);
+				// as the lone hierarchy's forward colour sweeps
+				if (small_lds_bytes(L, b->mixed ? 4 : 8) <= kSmallLdsBytes &&
+				    ((long long)L.m * L.n * L.o <= 1024 || small_pairs_per_pass(L) <= 1024))
+					plan_add(b, list_index(l, L_SMALL), s, 1);
 				else for (int q = 0; q < 4; q++) plan_add(b, list_index(l, L_PAIR0 + q), s, batch_units(BK_PAIR0 + q, L));
 				plan_add(b, list_index(l, L_RES), s, batch_units(BK_ROWS, L));
 				plan_add(b, list_index(l + 1, L_RESTRICT), s, batch_units(BK_PTS, D.lv[l + 1]));

@@ -175,8 +175,15 @@ void op_gs(pgmg_hier *h, int level, int mode, int nsweeps) {
 	const bool exact = h->opts.smoother == PGMG_SMOOTH_EXACT;
 	h->gs_count += nsweeps;
 	if (nsweeps <= 0) return;
-	const long long small = exact ? kSmallExact : kSmallColour;
-	if (L.points() <= small) {
+	// One workgroup smooths a small level when that beats per-class (or
+	// per-hyperplane) launches: with x in LDS, for exact mode, and for forward
+	// colour sweeps when each thread holds one point or one point pair.
+	const bool lds = small_lds_bytes(L.v, (int)sizeof(T)) <= kSmallLdsBytes;
+	bool use_small;
+	if (exact) use_small = lds || L.points() <= kSmallExact;
+	else if (mode == 0) use_small = lds && (L.points() <= 1024 || small_pairs_per_pass(L.v) <= 1024);
+	else use_small = L.points() <= kSmallColour;
+	if (use_small) {
 		ProfScope ps(h, KC_GS, level, L.points() * nsweeps * (mode ? 2 : 1));
 		launch_gs_small<T>(h->stream, L.v, lA<T>(L), lx<T>(L), lr<T>(L), nsweeps, mode, exact ? 1 : 0);
 		return;

@@ -3,10 +3,12 @@
 //
 // Level layout in HBM.  Every per-point array of a level (the 27 coefficient
 // planes, x, r, residual scratch) uses one padded index space: the grid plus
-// a zero halo, rows padded to a multiple of 16 elements.  Point (i,j,k),
-// -1 <= i <= m etc., lives at base + i + px*j + pxy*k with base = 2 + px + pxy,
-// so every even i sits at an even element offset and a pair (i, i+1), i even,
-// can be moved with one 16-byte (FP64) or 8-byte (FP32) vector access.
+// a zero halo, rows padded to a multiple of 16 elements with room for
+// i = -1 .. m+2.  Point (i,j,k),
+// -1 <= i <= m etc., lives at base + i + px*j + pxy*k with base = 4 + px + pxy,
+// so i = 0 of every row sits at a multiple of 4 elements: a pair (i, i+1),
+// i even, moves with one 16-byte (FP64) or 8-byte (FP32) vector access, and
+// a quad (i .. i+3), i a multiple of 4, with one 16-byte FP32 access.
 // Halo entries are zero and stay zero, so a stencil at the grid edge reads
 // zeros instead of branching.  Coefficients are stored as 27 separate planes
 // (structure of arrays), plane e at A + e*npad, so that a wavefront reading
@@ -29,9 +31,9 @@ struct LevelView {
 inline LevelView make_level_view(int m, int n, int o) {
 	LevelView v;
 	v.m = m; v.n = n; v.o = o;
-	v.px = ((m + 4 + 15) / 16) * 16;
+	v.px = ((m + 7 + 15) / 16) * 16;  // i = -1 .. m+2 fit: a quad at i0 <= m-1 reads i0+3
 	v.pxy = (long long)v.px * (n + 2);
-	v.base = 2 + v.px + v.pxy;
+	v.base = 4 + v.px + v.pxy;
 	v.npad = v.pxy * (o + 2);
 	return v;
 }
@@ -83,6 +85,18 @@ void launch_b_add_x(hipStream_t s, const SlotDev *slots, WorkList wl);
 // returns code.
 int set_error(int code, const char *msg);
 
+// LDS bytes the small-level smoother needs to stage x with its halo, and the
+// most it uses (within every HIP device's limit for dynamic LDS).
+__host__ __device__ inline long long small_lds_bytes(const LevelView &L, int elem) {
+	return (long long)(L.m + 2) * (L.n + 2) * (L.o + 2) * elem;
+}
+const int kSmallLdsBytes = 64 * 1024;
+// Point pairs (i0, i0+1), i0 even, of one colour pair (0,cj,ck), (1,cj,ck)
+// at most (cj = ck = 0).
+__host__ __device__ inline long long small_pairs_per_pass(const LevelView &L) {
+	return (long long)((L.m + 1) / 2) * ((L.n + 1) / 2) * ((L.o + 1) / 2);
+}
+
 // Kernel classes for timing.
 enum { KC_NONE = 0, KC_GS = 1, KC_RESIDUAL = 2, KC_RESTRICT = 3, KC_INTERP = 4 };
 
@@ -94,7 +108,8 @@ template <typename T> void launch_gs_colour(hipStream_t s, const LevelView &L, c
 template <typename T> void launch_gs_pair(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int pair);
 // Gauss-Seidel on the hyperplane i+2j+4k = t.
 template <typename T> void launch_gs_plane(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int t);
-// nsweeps Gauss-Seidel sweeps of a small level by one workgroup.
+// nsweeps Gauss-Seidel sweeps of a small level by one workgroup, with x
+// staged in LDS when small_lds_bytes(L, sizeof(T)) <= kSmallLdsBytes.
 template <typename T> void launch_gs_small(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r,
 					   int nsweeps, int mode, int exact);
 // res = r - A x on interior points (res may be null), per-block sums of

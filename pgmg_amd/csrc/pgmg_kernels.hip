@@ -113,6 +113,27 @@ __device__ __forceinline__ void st_row(void *base, uint32_t off, V v) {
 	*(V *)((char *)base + off) = v;
 }
 
+// Coefficient planes and r are read once per launch.  In FP64 their loads
+// carry the non-temporal hint, so the streamed planes do not evict from L2
+// the x rows that neighbouring grid rows read again (measured: fewer L2
+// misses in FP64, more in FP32, which keeps plain loads).
+#ifndef PGMG_NT_COEF
+#define PGMG_NT_COEF 1
+#endif
+typedef double d2_nt __attribute__((ext_vector_type(2)));
+template <typename V> __device__ __forceinline__ V ld_stream(const void *base, uint32_t off);
+template <> __device__ __forceinline__ float2 ld_stream<float2>(const void *base, uint32_t off) {
+	return *(const float2 *)((const char *)base + off);
+}
+template <> __device__ __forceinline__ double2 ld_stream<double2>(const void *base, uint32_t off) {
+#if PGMG_NT_COEF
+	const d2_nt v = __builtin_nontemporal_load((const d2_nt *)((const char *)base + off));
+	return make_double2(v.x, v.y);
+#else
+	return *(const double2 *)((const char *)base + off);
+#endif
+}
+
 // Colour classes (0,cj,ck) and then (1,cj,ck) in one launch.  One wavefront
 // owns one grid row (j,k) with j = cj (mod 2), k = ck (mod 2); only that
 // row is both read and written by the launch, because every other row a
@@ -155,15 +176,15 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 		T prod[12];
 		T xe = (T)0;
 		if (live) {
-			const V2 rr = ld_row<V2>(rr_, ob);
+			const V2 rr = ld_stream<V2>(rr_, ob);
 			rx = rr.x; ry = rr.y;
 #pragma unroll
 			for (int rq = 0; rq < 9; rq++) {
 				const int dj = rq % 3 - 1, dk = rq / 3 - 1;
 				const T *xq = xr + dj * px + dk * pxy;
-				const V2 am = ld_row<V2>(Ar + (3 * rq) * np, ob);
-				const V2 a0 = ld_row<V2>(Ar + (3 * rq + 1) * np, ob);
-				const V2 ap = ld_row<V2>(Ar + (3 * rq + 2) * np, ob);
+				const V2 am = ld_stream<V2>(Ar + (3 * rq) * np, ob);
+				const V2 a0 = ld_stream<V2>(Ar + (3 * rq + 1) * np, ob);
+				const V2 ap = ld_stream<V2>(Ar + (3 * rq + 2) * np, ob);
 				const V2 xv = ld_row<V2>(xq, ob);
 				const T xm = ld_row<T>(xq - 1, ob);
 				const T xp = doO ? ld_row<T>(xq + 2, ob) : (T)0;
@@ -208,6 +229,133 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 	}
 }
 
+// FP32 variant of gs_pair_row with four points per lane: lane l holds
+// i0 = s0 + 4l .. i0+3 (one 16-byte load per plane, i0 a multiple of 4 and
+// so 16-byte aligned), segments advance by 252 points, so a 257-point row
+// takes two segments and a 129-point row one.  Even points i0, i0+2 update
+// first from old neighbours; odd i0+1 then takes the lane's own new evens,
+// odd i0+3 its own i0+2 and lane l+1's i0+4 (shuffle).  Lane 63's i0+3 waits
+// for the next segment, whose lane 0 re-reads that quad and updates only
+// i0+3, with the carried new i0+2.  Per-point arithmetic and order as in
+// gs_point.  j and k must be wavefront-uniform.
+__device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float *__restrict__ A, float *__restrict__ x,
+						const float *__restrict__ r, int j, int k) {
+	const int lane = threadIdx.x;
+	const long long row = idx3(L, 0, j, k);
+	const long long px = L.px, pxy = L.pxy, np = L.npad;
+	const float *Ar = A + row;
+	const float *rr_ = r + row;
+	float *xr = x + row;
+	float carry = 0.0f;
+	for (int s0 = 0; s0 < L.m; s0 += 252) {
+		const int i0 = s0 + 4 * lane;
+		const bool live = i0 < L.m;
+		const bool first = lane == 0 && s0 > 0;  // only i0+3 is left to do
+		const bool doE0 = live && !first;
+		const bool doE2 = doE0 && i0 + 2 < L.m;
+		const bool doO1 = doE0 && i0 + 1 < L.m;
+		const bool doO3 = live && lane < 63 && i0 + 3 < L.m;
+		const uint32_t ob = (uint32_t)i0 * 4u;
+		// even sums e0 (i0), e2 (i0+2); odd sums o1 (i0+1), o3 (i0+3) up to
+		// the own row; own-row coefficients kept; the odd points' 12 terms
+		// after the own row kept as products (the in-order sums' roundings)
+		float e0 = 0.0f, e2 = 0.0f, o1 = 0.0f, o3 = 0.0f;
+		float c0 = 1.0f, c1 = 1.0f, c2 = 1.0f, c3 = 1.0f;
+		float w1m = 0.0f, w1p = 0.0f, w3m = 0.0f, w3p = 0.0f;
+		float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f;
+		float p1[12], p3[12];
+		if (live) {
+			const float4 rv = ld_row<float4>(rr_, ob);
+			r0 = rv.x; r1 = rv.y; r2 = rv.z; r3 = rv.w;
+#pragma unroll
+			for (int rq = 0; rq < 9; rq++) {
+				const int dj = rq % 3 - 1, dk = rq / 3 - 1;
+				const float *xq = xr + dj * px + dk * pxy;
+				const float4 am = ld_row<float4>(Ar + (3 * rq) * np, ob);
+				const float4 a0 = ld_row<float4>(Ar + (3 * rq + 1) * np, ob);
+				const float4 ap = ld_row<float4>(Ar + (3 * rq + 2) * np, ob);
+				const float4 xv = ld_row<float4>(xq, ob);
+				const float xm = ld_row<float>(xq - 1, ob);
+				const float xp = doO3 ? ld_row<float>(xq + 4, ob) : 0.0f;
+				if (rq == 4) {
+					e0 = e0 + am.x * xm;
+					e0 = e0 + ap.x * xv.y;
+					e2 = e2 + am.z * xv.y;
+					e2 = e2 + ap.z * xv.w;
+					c0 = a0.x; c1 = a0.y; c2 = a0.z; c3 = a0.w;
+					w1m = am.y; w1p = ap.y; w3m = am.w; w3p = ap.w;
+				} else {
+					e0 = e0 + am.x * xm;
+					e0 = e0 + a0.x * xv.x;
+					e0 = e0 + ap.x * xv.y;
+					e2 = e2 + am.z * xv.y;
+					e2 = e2 + a0.z * xv.z;
+					e2 = e2 + ap.z * xv.w;
+					if (rq < 4) {
+						o1 = o1 + am.y * xv.x;
+						o1 = o1 + a0.y * xv.y;
+						o1 = o1 + ap.y * xv.z;
+						o3 = o3 + am.w * xv.z;
+						o3 = o3 + a0.w * xv.w;
+						o3 = o3 + ap.w * xp;
+					} else {
+						p1[3 * (rq - 5)] = am.y * xv.x;
+						p1[3 * (rq - 5) + 1] = a0.y * xv.y;
+						p1[3 * (rq - 5) + 2] = ap.y * xv.z;
+						p3[3 * (rq - 5)] = am.w * xv.z;
+						p3[3 * (rq - 5) + 1] = a0.w * xv.w;
+						p3[3 * (rq - 5) + 2] = ap.w * xp;
+					}
+				}
+			}
+		}
+		float x0 = 0.0f, x2 = 0.0f;
+		if (doE0) {
+			x0 = (r0 - e0) * (1.0f / c0);
+			st_row<float>(xr, ob, x0);
+			if (doE2) {
+				x2 = (r2 - e2) * (1.0f / c2);
+				st_row<float>(xr, ob + 8u, x2);
+			}
+		} else if (first) {
+			x2 = carry;  // i0+2, updated by lane 63 of the previous segment
+		}
+		// shuffles with every lane active; lanes past the row end hold 0
+		const float x4 = __shfl_down(x0, 1, 64);
+		if (doO1) {
+			o1 = o1 + w1m * x0;
+			o1 = o1 + w1p * x2;
+#pragma unroll
+			for (int q2 = 0; q2 < 12; q2++) o1 = o1 + p1[q2];
+			st_row<float>(xr, ob + 4u, (r1 - o1) * (1.0f / c1));
+		}
+		if (doO3) {
+			o3 = o3 + w3m * x2;
+			o3 = o3 + w3p * x4;
+#pragma unroll
+			for (int q2 = 0; q2 < 12; q2++) o3 = o3 + p3[q2];
+			st_row<float>(xr, ob + 12u, (r3 - o3) * (1.0f / c3));
+		}
+		carry = __shfl(x2, 63, 64);
+	}
+}
+
+// Forward colour-pair update of one row: four points per lane in FP32,
+// two in FP64 (whose 16-byte pair loads already fill the registers).
+template <typename T>
+__device__ __forceinline__ void gs_row(const LevelView &L, const T *__restrict__ A, T *__restrict__ x,
+				       const T *__restrict__ r, int j, int k);
+template <>
+__device__ __forceinline__ void gs_row<float>(const LevelView &L, const float *__restrict__ A, float *__restrict__ x,
+					      const float *__restrict__ r, int j, int k) {
+	gs_quad_row_f32(L, A, x, r, j, k);
+}
+template <>
+__device__ __forceinline__ void gs_row<double>(const LevelView &L, const double *__restrict__ A, double *__restrict__ x,
+					       const double *__restrict__ r, int j, int k) {
+	gs_pair_row<double>(L, A, x, r, j, k);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void k_gs_pair(LevelView L, const T *__restrict__ A, T *__restrict__ x,
 						 const T *__restrict__ r, int cj, int ck) {
@@ -215,7 +363,7 @@ __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void k_gs_pair(LevelView L, c
 	// one row per wavefront: the row index is wavefront-uniform
 	const int q = __builtin_amdgcn_readfirstlane(xcd_block((int)blockIdx.x, (int)gridDim.x) * 4 + (int)threadIdx.y);
 	if (q >= nb * nd) return;
-	gs_pair_row<T>(L, A, x, r, 2 * (q % nb) + cj, 2 * (q / nb) + ck);
+	gs_row<T>(L, A, x, r, 2 * (q % nb) + cj, 2 * (q / nb) + ck);
 }
 
 template <typename T>
@@ -299,6 +447,198 @@ __device__ __forceinline__ void gs_small_body(const LevelView &L, const T *__res
 	}
 }
 
+// The same sweeps with x staged in LDS (small_lds_fits): x, halo included,
+// is copied into LDS in a compact layout (pitches m+2 and (m+2)(n+2)), every
+// sweep of the call reads and writes it there, and the interior is written
+// back at the end, so a colour class (or hyperplane) costs one coefficient
+// fetch from L2 and a barrier, not a round trip to memory for x.  A level of
+// at most blockDim points keeps each thread's own point's 27 coefficients
+// and r in registers for the whole call (points numbered colour class by
+// colour class in colour mode, lexicographically in exact mode).  Per-point
+// arithmetic is gs_point's, so the result is bit-identical.
+template <typename T>
+__device__ __forceinline__ void gs_lds_update(const T *__restrict__ c, T rv, T *__restrict__ xs, int lq, int hx, int hxy) {
+	T ax = (T)0;
+#pragma unroll
+	for (int e = 0; e < 27; e++) {
+		if (e == 13) continue;
+		const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
+		ax = ax + c[e] * xs[lq + di + dj * hx + dk * hxy];
+	}
+	xs[lq] = (rv - ax) * ((T)1 / c[13]);
+}
+
+template <typename T>
+__device__ __forceinline__ void gs_lds_global(const LevelView &L, const T *__restrict__ A, const T *__restrict__ r,
+					      T *__restrict__ xs, int i, int j, int k, int hx, int hxy) {
+	const long long p = idx3(L, i, j, k), np = L.npad;
+	T c[27];
+#pragma unroll
+	for (int e = 0; e < 27; e++) c[e] = A[e * np + p];
+	gs_lds_update<T>(c, r[p], xs, (i + 1) + hx * (j + 1) + hxy * (k + 1), hx, hxy);
+}
+
+// One colour pair (0,cj,ck), (1,cj,ck) of a small level with x in LDS, one
+// point pair (i0, i0+1), i0 even, per thread (at most blockDim pairs).
+// Forward (rev = 0) updates the even point first, reverse the odd one; the
+// second point's coefficients, loaded with the first's, are kept in
+// registers over the barrier between the two classes.  The caller's barrier
+// follows.
+// One colour pair (0,cj,ck), (1,cj,ck) of a small level with x in LDS, one
+// point pair (i0, i0+1), i0 even, per thread (at most blockDim pairs): the
+// even point is updated from vector loads of both points' coefficients, the
+// odd point's are kept in registers over the barrier between the two
+// classes, then it is updated.  The caller's barrier follows.
+template <typename T>
+__device__ __forceinline__ void gs_lds_pair_pass(const LevelView &L, const T *__restrict__ A, const T *__restrict__ r,
+						 T *__restrict__ xs, int pq, int hx, int hxy) {
+	typedef typename Vec2<T>::type V2;
+	const int cj = pq & 1, ck = pq >> 1;
+	const int na = (L.m + 1) / 2, nb = (L.n - cj + 1) / 2, nd = (L.o - ck + 1) / 2;
+	const int q = threadIdx.x;
+	const bool live = q < na * nb * nd;
+	T c2[27], r2 = (T)0;
+	int lq = 0;
+	bool two = false;
+	if (live) {
+		const int i0 = 2 * (q % na), j = 2 * ((q / na) % nb) + cj, k = 2 * (q / (na * nb)) + ck;
+		two = i0 + 1 < L.m;
+		const long long p = idx3(L, i0, j, k), np = L.npad;
+		lq = (i0 + 1) + hx * (j + 1) + hxy * (k + 1);
+		const V2 rv = *(const V2 *)(r + p);
+		r2 = rv.y;
+		// the even point's sum, in box order, as the planes arrive
+		T ax = (T)0, a13 = (T)1;
+#pragma unroll
+		for (int e = 0; e < 27; e++) {
+			const V2 a = *(const V2 *)(A + e * np + p);
+			c2[e] = a.y;
+			if (e == 13) { a13 = a.x; continue; }
+			const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
+			ax = ax + a.x * xs[lq + di + dj * hx + dk * hxy];
+		}
+		xs[lq] = (rv.x - ax) * ((T)1 / a13);
+	}
+	__syncthreads();
+	if (live && two) gs_lds_update<T>(c2, r2, xs, lq + 1, hx, hxy);
+}
+
+template <typename T, bool resident>
+__device__ __forceinline__ void gs_lds_run(const LevelView &L, const T *__restrict__ A, T *__restrict__ x,
+					   const T *__restrict__ r, int nsweeps, int mode, int exact, T *__restrict__ xs) {
+	const int nt = blockDim.x, tid = threadIdx.x;
+	const int hx = L.m + 2, hxy = hx * (L.n + 2), nh = hxy * (L.o + 2);
+	const int npts = L.m * L.n * L.o;
+	for (int q = tid; q < nh; q += nt) {
+		const int i = q % hx - 1, j = (q / hx) % (L.n + 2) - 1, k = q / hxy - 1;
+		xs[q] = x[idx3(L, i, j, k)];
+	}
+	// resident: this thread's point, its colour or hyperplane, coefficients, r
+	bool own = false;
+	int oc = -1, ot = -1, olq = 0;
+	T ca[27], cr = (T)0;
+	if (resident && tid < npts) {
+		int i, j, k;
+		if (exact) {
+			i = tid % L.m; j = (tid / L.m) % L.n; k = tid / (L.m * L.n);
+		} else {
+			int q = tid, c = 0;
+			for (; c < 8; c++) {
+				const int na = (L.m - (c & 1) + 1) / 2, nb = (L.n - ((c >> 1) & 1) + 1) / 2, nd = (L.o - (c >> 2) + 1) / 2;
+				const int cnt = na * nb * nd;
+				if (q < cnt) {
+					i = 2 * (q % na) + (c & 1); j = 2 * ((q / na) % nb) + ((c >> 1) & 1); k = 2 * (q / (na * nb)) + (c >> 2);
+					break;
+				}
+				q -= cnt;
+			}
+			oc = c;
+		}
+		own = true;
+		ot = i + 2 * j + 4 * k;
+		olq = (i + 1) + hx * (j + 1) + hxy * (k + 1);
+		const long long p = idx3(L, i, j, k), np = L.npad;
+#pragma unroll
+		for (int e = 0; e < 27; e++) ca[e] = A[e * np + p];
+		cr = r[p];
+	}
+	__syncthreads();
+	const int tmax = (L.m - 1) + 2 * (L.n - 1) + 4 * (L.o - 1);
+	const int npair = L.n * L.o;
+	// forward colour sweeps: classes (0,cj,ck) and (1,cj,ck) as one pass
+	const bool staged = !resident && !exact && !mode && small_pairs_per_pass(L) <= nt;
+	if (staged) {
+#pragma unroll 1
+		for (int sw = 0; sw < nsweeps; sw++) {
+#pragma unroll 1
+			for (int pp = 0; pp < 4; pp++) {
+				gs_lds_pair_pass<T>(L, A, r, xs, pp, hx, hxy);
+				__syncthreads();
+			}
+		}
+	} else {
+#pragma unroll 1
+	for (int sw = 0; sw < nsweeps; sw++) {
+#pragma unroll 1
+		for (int pass = 0; pass < (mode ? 2 : 1); pass++) {
+			if (exact) {
+#pragma unroll 1
+				for (int tt = 0; tt <= tmax; tt++) {
+					const int t = pass ? tmax - tt : tt;
+					if (resident) {
+						if (own && ot == t) gs_lds_update<T>(ca, cr, xs, olq, hx, hxy);
+					} else {
+						for (int q = tid; q < npair; q += nt) {
+							const int j = q % L.n, k = q / L.n;
+							const int i = t - 2 * j - 4 * k;
+							if (i >= 0 && i < L.m) gs_lds_global<T>(L, A, r, xs, i, j, k, hx, hxy);
+						}
+					}
+					__syncthreads();
+				}
+			} else {
+#pragma unroll 1
+				for (int cc = 0; cc < 8; cc++) {
+					const int c = pass ? 7 - cc : cc;
+					if (resident) {
+						if (own && oc == c) gs_lds_update<T>(ca, cr, xs, olq, hx, hxy);
+					} else {
+						const int ci = c & 1, cj = (c >> 1) & 1, ck = (c >> 2) & 1;
+						const int na = (L.m - ci + 1) / 2, nb = (L.n - cj + 1) / 2, nd = (L.o - ck + 1) / 2;
+						const int tot = na * nb * nd;
+						for (int q = tid; q < tot; q += nt) {
+							const int a = q % na, b = (q / na) % nb, d = q / (na * nb);
+							gs_lds_global<T>(L, A, r, xs, 2 * a + ci, 2 * b + cj, 2 * d + ck, hx, hxy);
+						}
+					}
+					__syncthreads();
+				}
+			}
+		}
+	}
+	}
+	for (int q = tid; q < npts; q += nt) {
+		const int i = q % L.m, j = (q / L.m) % L.n, k = q / (L.m * L.n);
+		x[idx3(L, i, j, k)] = xs[(i + 1) + hx * (j + 1) + hxy * (k + 1)];
+	}
+}
+
+template <typename T>
+__device__ __forceinline__ void gs_lds_body(const LevelView &L, const T *__restrict__ A, T *__restrict__ x,
+					    const T *__restrict__ r, int nsweeps, int mode, int exact, T *__restrict__ xs) {
+	if ((long long)L.m * L.n * L.o <= (long long)blockDim.x)
+		gs_lds_run<T, true>(L, A, x, r, nsweeps, mode, exact, xs);
+	else
+		gs_lds_run<T, false>(L, A, x, r, nsweeps, mode, exact, xs);
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_gs_small_lds(LevelView L, const T *__restrict__ A, T *__restrict__ x,
+						       const T *__restrict__ r, int nsweeps, int mode, int exact) {
+	extern __shared__ char smem_[];
+	gs_lds_body<T>(L, A, x, r, nsweeps, mode, exact, (T *)smem_);
+}
+
 template <typename T>
 __global__ __launch_bounds__(1024) void k_gs_small(LevelView L, const T *__restrict__ A, T *__restrict__ x,
 						   const T *__restrict__ r, int nsweeps, int mode, int exact) {
@@ -307,7 +647,11 @@ __global__ __launch_bounds__(1024) void k_gs_small(LevelView L, const T *__restr
 
 template <typename T>
 void launch_gs_small(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int nsweeps, int mode, int exact) {
-	hipLaunchKernelGGL(k_gs_small<T>, dim3(1), dim3(1024), 0, s, L, A, x, r, nsweeps, mode, exact);
+	const long long lds = small_lds_bytes(L, (int)sizeof(T));
+	if (lds <= kSmallLdsBytes)
+		hipLaunchKernelGGL(k_gs_small_lds<T>, dim3(1), dim3(1024), (unsigned)lds, s, L, A, x, r, nsweeps, mode, exact);
+	else
+		hipLaunchKernelGGL(k_gs_small<T>, dim3(1), dim3(1024), 0, s, L, A, x, r, nsweeps, mode, exact);
 	HIPCHECK_LAUNCH();
 }
 
@@ -349,15 +693,15 @@ __device__ __forceinline__ double residual_row(const LevelView &L, const T *__re
 			if (i0 >= L.m) continue;
 			const bool two = i0 + 1 < L.m;
 			const uint32_t ob = (uint32_t)i0 * (uint32_t)sizeof(T);
-			const V2 rr = ld_row<V2>(rr_, ob);
+			const V2 rr = ld_stream<V2>(rr_, ob);
 			T ax = (T)0, ay = (T)0;
 #pragma unroll
 			for (int rq = 0; rq < 9; rq++) {
 				const int dj = rq % 3 - 1, dk = rq / 3 - 1;
 				const T *xq = xr + dj * px + dk * pxy;
-				const V2 am = ld_row<V2>(Ar + (3 * rq) * np, ob);
-				const V2 a0 = ld_row<V2>(Ar + (3 * rq + 1) * np, ob);
-				const V2 ap = ld_row<V2>(Ar + (3 * rq + 2) * np, ob);
+				const V2 am = ld_stream<V2>(Ar + (3 * rq) * np, ob);
+				const V2 a0 = ld_stream<V2>(Ar + (3 * rq + 1) * np, ob);
+				const V2 ap = ld_stream<V2>(Ar + (3 * rq + 2) * np, ob);
 				const V2 xv = ld_row<V2>(xq, ob);
 				const T xm = ld_row<T>(xq - 1, ob);
 				const T xp = two ? ld_row<T>(xq + 2, ob) : (T)0;
@@ -1021,15 +1365,20 @@ __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void kb_gs_pair(const SlotDev
 		const int nb = (L.n - cj + 1) >> 1, nd = (L.o - ck + 1) >> 1;
 		const int row = __builtin_amdgcn_readfirstlane(lu * 4 + (int)threadIdx.y);
 		if (row < nb * nd)
-			gs_pair_row<T>(L, sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), 2 * (row % nb) + cj, 2 * (row / nb) + ck);
+			gs_row<T>(L, sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), 2 * (row % nb) + cj, 2 * (row / nb) + ck);
 	}
 }
 
 template <typename T>
 __global__ __launch_bounds__(1024) void kb_gs_small(const SlotDev *__restrict__ slots, WorkList wl, int level, int nsweeps) {
+	extern __shared__ char smem_[];
 	BATCH_LOOP
 		(void)lu;
-		gs_small_body<T>(S->lv[level], sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), nsweeps, 0, 0);
+		const LevelView L = S->lv[level];
+		if (small_lds_bytes(L, (int)sizeof(T)) <= kSmallLdsBytes)
+			gs_lds_body<T>(L, sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), nsweeps, 0, 0, (T *)smem_);
+		else
+			gs_small_body<T>(L, sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), nsweeps, 0, 0);
 		__syncthreads();
 	}
 }
@@ -1135,7 +1484,7 @@ void launch_b_gs_pair(hipStream_t s, const SlotDev *slots, WorkList wl, int leve
 template <typename T>
 void launch_b_gs_small(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int nsweeps) {
 	if (wl.n <= 0) return;
-	hipLaunchKernelGGL(kb_gs_small<T>, dim3(kBatchBlocks1024), dim3(1024), 0, s, slots, wl, level, nsweeps);
+	hipLaunchKernelGGL(kb_gs_small<T>, dim3(kBatchBlocks1024), dim3(1024), kSmallLdsBytes, s, slots, wl, level, nsweeps);
 	HIPCHECK_LAUNCH();
 }
 template <typename T>
