@@ -231,13 +231,16 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 
 // FP32 variant of gs_pair_row with four points per lane: lane l holds
 // i0 = s0 + 4l .. i0+3 (one 16-byte load per plane, i0 a multiple of 4 and
-// so 16-byte aligned), segments advance by 252 points, so a 257-point row
-// takes two segments and a 129-point row one.  Even points i0, i0+2 update
-// first from old neighbours; odd i0+1 then takes the lane's own new evens,
-// odd i0+3 its own i0+2 and lane l+1's i0+4 (shuffle).  Lane 63's i0+3 waits
-// for the next segment, whose lane 0 re-reads that quad and updates only
-// i0+3, with the carried new i0+2.  Per-point arithmetic and order as in
-// gs_point.  j and k must be wavefront-uniform.
+// so 16-byte aligned).  Even points i0, i0+2 update first from old
+// neighbours; odd i0+1 then takes the lane's own new evens, odd i0+3 its own
+// i0+2 and lane l+1's i0+4 (shuffle).  A segment that does not reach the row
+// end advances by 252 points: lane 63's i0+3 waits for the next segment,
+// whose lane 0 re-reads that quad and updates only i0+3, with the carried
+// new i0+2.  The last segment covers up to 257 points: when one point is
+// left past lane 63's quad (a row of 2^k + 1 points), lane 63 also updates
+// that even point, from scalar loads, with the evens.  So a 257-point row
+// takes one segment.  Per-point arithmetic and order as in gs_point.  j and
+// k must be wavefront-uniform.
 __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float *__restrict__ A, float *__restrict__ x,
 						const float *__restrict__ r, int j, int k) {
 	const int lane = threadIdx.x;
@@ -247,26 +250,30 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 	const float *rr_ = r + row;
 	float *xr = x + row;
 	float carry = 0.0f;
-	for (int s0 = 0; s0 < L.m; s0 += 252) {
+	for (int s0 = 0;; s0 += 252) {
 		const int i0 = s0 + 4 * lane;
+		const bool last = L.m - s0 <= 257;
 		const bool live = i0 < L.m;
 		const bool first = lane == 0 && s0 > 0;  // only i0+3 is left to do
 		const bool doE0 = live && !first;
 		const bool doE2 = doE0 && i0 + 2 < L.m;
 		const bool doO1 = doE0 && i0 + 1 < L.m;
-		const bool doO3 = live && lane < 63 && i0 + 3 < L.m;
+		const bool doO3 = live && i0 + 3 < L.m && (lane < 63 || last);
+		const bool ext = last && lane == 63 && L.m - s0 == 257;  // even point i0+4 = m-1
 		const uint32_t ob = (uint32_t)i0 * 4u;
-		// even sums e0 (i0), e2 (i0+2); odd sums o1 (i0+1), o3 (i0+3) up to
-		// the own row; own-row coefficients kept; the odd points' 12 terms
-		// after the own row kept as products (the in-order sums' roundings)
-		float e0 = 0.0f, e2 = 0.0f, o1 = 0.0f, o3 = 0.0f;
-		float c0 = 1.0f, c1 = 1.0f, c2 = 1.0f, c3 = 1.0f;
+		// even sums e0 (i0), e2 (i0+2), ex (i0+4, ext); odd sums o1 (i0+1),
+		// o3 (i0+3) up to the own row; own-row coefficients kept; the odd
+		// points' 12 terms after the own row kept as products (the in-order
+		// sums' roundings)
+		float e0 = 0.0f, e2 = 0.0f, ex = 0.0f, o1 = 0.0f, o3 = 0.0f;
+		float c0 = 1.0f, c1 = 1.0f, c2 = 1.0f, c3 = 1.0f, cx = 1.0f;
 		float w1m = 0.0f, w1p = 0.0f, w3m = 0.0f, w3p = 0.0f;
-		float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f;
+		float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f, rx = 0.0f;
 		float p1[12], p3[12];
 		if (live) {
 			const float4 rv = ld_row<float4>(rr_, ob);
 			r0 = rv.x; r1 = rv.y; r2 = rv.z; r3 = rv.w;
+			if (ext) rx = ld_row<float>(rr_, ob + 16u);
 #pragma unroll
 			for (int rq = 0; rq < 9; rq++) {
 				const int dj = rq % 3 - 1, dk = rq / 3 - 1;
@@ -277,12 +284,20 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 				const float4 xv = ld_row<float4>(xq, ob);
 				const float xm = ld_row<float>(xq - 1, ob);
 				const float xp = doO3 ? ld_row<float>(xq + 4, ob) : 0.0f;
+				float bm = 0.0f, b0 = 0.0f, bp = 0.0f;
+				if (ext) {
+					bm = ld_row<float>(Ar + (3 * rq) * np, ob + 16u);
+					b0 = ld_row<float>(Ar + (3 * rq + 1) * np, ob + 16u);
+					bp = ld_row<float>(Ar + (3 * rq + 2) * np, ob + 16u);
+				}
 				if (rq == 4) {
 					e0 = e0 + am.x * xm;
 					e0 = e0 + ap.x * xv.y;
 					e2 = e2 + am.z * xv.y;
 					e2 = e2 + ap.z * xv.w;
-					c0 = a0.x; c1 = a0.y; c2 = a0.z; c3 = a0.w;
+					ex = ex + bm * xv.w;
+					ex = ex + bp * 0.0f;  // the halo point i = m
+					c0 = a0.x; c1 = a0.y; c2 = a0.z; c3 = a0.w; cx = b0;
 					w1m = am.y; w1p = ap.y; w3m = am.w; w3p = ap.w;
 				} else {
 					e0 = e0 + am.x * xm;
@@ -291,6 +306,9 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 					e2 = e2 + am.z * xv.y;
 					e2 = e2 + a0.z * xv.z;
 					e2 = e2 + ap.z * xv.w;
+					ex = ex + bm * xv.w;
+					ex = ex + b0 * xp;
+					ex = ex + bp * 0.0f;
 					if (rq < 4) {
 						o1 = o1 + am.y * xv.x;
 						o1 = o1 + a0.y * xv.y;
@@ -309,7 +327,7 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 				}
 			}
 		}
-		float x0 = 0.0f, x2 = 0.0f;
+		float x0 = 0.0f, x2 = 0.0f, xx = 0.0f;
 		if (doE0) {
 			x0 = (r0 - e0) * (1.0f / c0);
 			st_row<float>(xr, ob, x0);
@@ -317,11 +335,16 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 				x2 = (r2 - e2) * (1.0f / c2);
 				st_row<float>(xr, ob + 8u, x2);
 			}
+			if (ext) {
+				xx = (rx - ex) * (1.0f / cx);
+				st_row<float>(xr, ob + 16u, xx);
+			}
 		} else if (first) {
 			x2 = carry;  // i0+2, updated by lane 63 of the previous segment
 		}
 		// shuffles with every lane active; lanes past the row end hold 0
-		const float x4 = __shfl_down(x0, 1, 64);
+		const float xn = __shfl_down(x0, 1, 64);
+		const float x4 = lane == 63 ? xx : xn;  // lane 63: the extra point or the halo
 		if (doO1) {
 			o1 = o1 + w1m * x0;
 			o1 = o1 + w1p * x2;
@@ -336,6 +359,7 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 			for (int q2 = 0; q2 < 12; q2++) o3 = o3 + p3[q2];
 			st_row<float>(xr, ob + 12u, (r3 - o3) * (1.0f / c3));
 		}
+		if (last) break;
 		carry = __shfl(x2, 63, 64);
 	}
 }
@@ -671,13 +695,14 @@ __device__ __forceinline__ double block_sum_256(double v) {
 }
 
 // Residual r - A x and per-block sums of its square.  One wavefront per grid
-// row, each lane a pair of points read with vector loads (as in k_gs_pair,
-// with the same uniform row bases); blocks of 4 rows, renumbered per XCD.
+// row, each lane a pair of points (FP64) or four (FP32) read with vector
+// loads (as in k_gs_pair, with the same uniform row bases); blocks of 4
+// rows, renumbered per XCD.
 // res (may be null) receives the residual in type TR, partial (may be null)
 // one sum per block.  q (the row) must be wavefront-uniform.
 template <typename T, typename TR>
-__device__ __forceinline__ double residual_row(const LevelView &L, const T *__restrict__ A, const T *__restrict__ x,
-					       const T *__restrict__ r, TR *__restrict__ res, int q) {
+__device__ __forceinline__ double residual_row_pair(const LevelView &L, const T *__restrict__ A, const T *__restrict__ x,
+						    const T *__restrict__ r, TR *__restrict__ res, int q) {
 	typedef typename Vec2<T>::type V2;
 	typedef typename Vec2<TR>::type R2;
 	const int lane = threadIdx.x;
@@ -688,41 +713,143 @@ __device__ __forceinline__ double residual_row(const LevelView &L, const T *__re
 		const long long row = idx3(L, 0, j, k);
 		const T *Ar = A + row, *xr = x + row, *rr_ = r + row;
 		TR *resr = res ? res + row : nullptr;
-		for (int s0 = 0; s0 < L.m; s0 += 128) {
+		// segments of 128 points; the last one takes up to 129, lane 63
+		// doing the odd point out from scalar loads
+		for (int s0 = 0;; s0 += 128) {
+			const bool last = L.m - s0 <= 129;
 			const int i0 = s0 + 2 * lane;
-			if (i0 >= L.m) continue;
-			const bool two = i0 + 1 < L.m;
-			const uint32_t ob = (uint32_t)i0 * (uint32_t)sizeof(T);
-			const V2 rr = ld_stream<V2>(rr_, ob);
-			T ax = (T)0, ay = (T)0;
+			const bool ext = last && lane == 63 && L.m - s0 == 129;  // point i0+2 = m-1
+			if (i0 < L.m) {
+				const bool two = i0 + 1 < L.m;
+				const uint32_t ob = (uint32_t)i0 * (uint32_t)sizeof(T);
+				const V2 rr = ld_stream<V2>(rr_, ob);
+				const T rz = ext ? ld_row<T>(rr_, ob + 2u * (uint32_t)sizeof(T)) : (T)0;
+				T ax = (T)0, ay = (T)0, az = (T)0;
 #pragma unroll
-			for (int rq = 0; rq < 9; rq++) {
-				const int dj = rq % 3 - 1, dk = rq / 3 - 1;
-				const T *xq = xr + dj * px + dk * pxy;
-				const V2 am = ld_stream<V2>(Ar + (3 * rq) * np, ob);
-				const V2 a0 = ld_stream<V2>(Ar + (3 * rq + 1) * np, ob);
-				const V2 ap = ld_stream<V2>(Ar + (3 * rq + 2) * np, ob);
-				const V2 xv = ld_row<V2>(xq, ob);
-				const T xm = ld_row<T>(xq - 1, ob);
-				const T xp = two ? ld_row<T>(xq + 2, ob) : (T)0;
-				ax = ax + am.x * xm;
-				ax = ax + a0.x * xv.x;
-				ax = ax + ap.x * xv.y;
-				ay = ay + am.y * xv.x;
-				ay = ay + a0.y * xv.y;
-				ay = ay + ap.y * xp;
+				for (int rq = 0; rq < 9; rq++) {
+					const int dj = rq % 3 - 1, dk = rq / 3 - 1;
+					const T *xq = xr + dj * px + dk * pxy;
+					const V2 am = ld_stream<V2>(Ar + (3 * rq) * np, ob);
+					const V2 a0 = ld_stream<V2>(Ar + (3 * rq + 1) * np, ob);
+					const V2 ap = ld_stream<V2>(Ar + (3 * rq + 2) * np, ob);
+					const V2 xv = ld_row<V2>(xq, ob);
+					const T xm = ld_row<T>(xq - 1, ob);
+					const T xp = two ? ld_row<T>(xq + 2, ob) : (T)0;
+					ax = ax + am.x * xm;
+					ax = ax + a0.x * xv.x;
+					ax = ax + ap.x * xv.y;
+					ay = ay + am.y * xv.x;
+					ay = ay + a0.y * xv.y;
+					ay = ay + ap.y * xp;
+					if (ext) {
+						const uint32_t oz = ob + 2u * (uint32_t)sizeof(T);
+						az = az + ld_row<T>(Ar + (3 * rq) * np, oz) * xv.y;
+						az = az + ld_row<T>(Ar + (3 * rq + 1) * np, oz) * xp;
+						az = az + ld_row<T>(Ar + (3 * rq + 2) * np, oz) * (T)0;  // the halo point i = m
+					}
+				}
+				const T rv0 = rr.x - ax, rv1 = rr.y - ay, rv2 = rz - az;
+				if (resr) {
+					const uint32_t obr = (uint32_t)i0 * (uint32_t)sizeof(TR);
+					if (two) { R2 o; o.x = (TR)rv0; o.y = (TR)rv1; st_row<R2>(resr, obr, o); }
+					else st_row<TR>(resr, obr, (TR)rv0);
+					if (ext) st_row<TR>(resr, obr + 2u * (uint32_t)sizeof(TR), (TR)rv2);
+				}
+				v2 = v2 + (double)rv0 * (double)rv0;
+				if (two) v2 = v2 + (double)rv1 * (double)rv1;
+				if (ext) v2 = v2 + (double)rv2 * (double)rv2;
 			}
-			const T rv0 = rr.x - ax, rv1 = rr.y - ay;
-			if (resr) {
-				const uint32_t obr = (uint32_t)i0 * (uint32_t)sizeof(TR);
-				if (two) { R2 o; o.x = (TR)rv0; o.y = (TR)rv1; st_row<R2>(resr, obr, o); }
-				else st_row<TR>(resr, obr, (TR)rv0);
-			}
-			v2 = v2 + (double)rv0 * (double)rv0;
-			if (two) v2 = v2 + (double)rv1 * (double)rv1;
+			if (last) break;
 		}
 	}
 	return v2;
+}
+
+// FP32 residual with four points per lane (16-byte loads): segments of 256
+// points, the last one up to 257 with lane 63 doing the point out.
+__device__ __forceinline__ double residual_row_quad_f32(const LevelView &L, const float *__restrict__ A,
+							const float *__restrict__ x, const float *__restrict__ r,
+							float *__restrict__ res, int q) {
+	const int lane = threadIdx.x;
+	const long long px = L.px, pxy = L.pxy, np = L.npad;
+	double v2 = 0.0;
+	if (q < L.n * L.o) {
+		const int j = q % L.n, k = q / L.n;
+		const long long row = idx3(L, 0, j, k);
+		const float *Ar = A + row, *xr = x + row, *rr_ = r + row;
+		float *resr = res ? res + row : nullptr;
+		for (int s0 = 0;; s0 += 256) {
+			const bool last = L.m - s0 <= 257;
+			const int i0 = s0 + 4 * lane;
+			const bool ext = last && lane == 63 && L.m - s0 == 257;  // point i0+4 = m-1
+			if (i0 < L.m) {
+				const int nv = min(4, L.m - i0);
+				const uint32_t ob = (uint32_t)i0 * 4u;
+				const float4 rr = ld_row<float4>(rr_, ob);
+				const float rz = ext ? ld_row<float>(rr_, ob + 16u) : 0.0f;
+				float a0s = 0.0f, a1s = 0.0f, a2s = 0.0f, a3s = 0.0f, azs = 0.0f;
+#pragma unroll
+				for (int rq = 0; rq < 9; rq++) {
+					const int dj = rq % 3 - 1, dk = rq / 3 - 1;
+					const float *xq = xr + dj * px + dk * pxy;
+					const float4 am = ld_row<float4>(Ar + (3 * rq) * np, ob);
+					const float4 ac = ld_row<float4>(Ar + (3 * rq + 1) * np, ob);
+					const float4 ap = ld_row<float4>(Ar + (3 * rq + 2) * np, ob);
+					const float4 xv = ld_row<float4>(xq, ob);
+					const float xm = ld_row<float>(xq - 1, ob);
+					const float xp = nv == 4 ? ld_row<float>(xq + 4, ob) : 0.0f;
+					a0s = a0s + am.x * xm;
+					a0s = a0s + ac.x * xv.x;
+					a0s = a0s + ap.x * xv.y;
+					a1s = a1s + am.y * xv.x;
+					a1s = a1s + ac.y * xv.y;
+					a1s = a1s + ap.y * xv.z;
+					a2s = a2s + am.z * xv.y;
+					a2s = a2s + ac.z * xv.z;
+					a2s = a2s + ap.z * xv.w;
+					a3s = a3s + am.w * xv.z;
+					a3s = a3s + ac.w * xv.w;
+					a3s = a3s + ap.w * xp;
+					if (ext) {
+						azs = azs + ld_row<float>(Ar + (3 * rq) * np, ob + 16u) * xv.w;
+						azs = azs + ld_row<float>(Ar + (3 * rq + 1) * np, ob + 16u) * xp;
+						azs = azs + ld_row<float>(Ar + (3 * rq + 2) * np, ob + 16u) * 0.0f;  // halo
+					}
+				}
+				const float v0 = rr.x - a0s, v1 = rr.y - a1s, v2f = rr.z - a2s, v3 = rr.w - a3s, vz = rz - azs;
+				if (resr) {
+					if (nv == 4) st_row<float4>(resr, ob, make_float4(v0, v1, v2f, v3));
+					else {
+						st_row<float>(resr, ob, v0);
+						if (nv > 1) st_row<float>(resr, ob + 4u, v1);
+						if (nv > 2) st_row<float>(resr, ob + 8u, v2f);
+					}
+					if (ext) st_row<float>(resr, ob + 16u, vz);
+				}
+				v2 = v2 + (double)v0 * (double)v0;
+				if (nv > 1) v2 = v2 + (double)v1 * (double)v1;
+				if (nv > 2) v2 = v2 + (double)v2f * (double)v2f;
+				if (nv > 3) v2 = v2 + (double)v3 * (double)v3;
+				if (ext) v2 = v2 + (double)vz * (double)vz;
+			}
+			if (last) break;
+		}
+	}
+	return v2;
+}
+
+// Residual rows, dispatched on the arithmetic type: four points per lane in
+// FP32 (residual into FP32), two in FP64.
+template <typename T, typename TR>
+__device__ __forceinline__ double residual_row(const LevelView &L, const T *__restrict__ A, const T *__restrict__ x,
+					       const T *__restrict__ r, TR *__restrict__ res, int q) {
+	return residual_row_pair<T, TR>(L, A, x, r, res, q);
+}
+template <>
+__device__ __forceinline__ double residual_row<float, float>(const LevelView &L, const float *__restrict__ A,
+							     const float *__restrict__ x, const float *__restrict__ r,
+							     float *__restrict__ res, int q) {
+	return residual_row_quad_f32(L, A, x, r, res, q);
 }
 
 template <typename T, typename TR>
