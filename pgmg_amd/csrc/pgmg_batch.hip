@@ -177,7 +177,7 @@ int build_plan(pgmg_batch *b) {
 				else for (int q = 0; q < 4; q++) plan_add(b, list_index(l, L_PAIR0 + q), s, batch_units(BK_PAIR0 + q, L));
 				plan_add(b, list_index(l, L_RES), s, batch_units(BK_ROWS, L));
 				plan_add(b, list_index(l + 1, L_RESTRICT), s, batch_units(BK_PTS, D.lv[l + 1]));
-				plan_add(b, list_index(l + 1, L_INTERP), s, batch_units(BK_PTS, L));
+				plan_add(b, list_index(l + 1, L_INTERP), s, batch_units(BK_CUBES, L));
 			} else if (D.lu_active) {
 				plan_add(b, list_index(l, L_LU), s, 1);
 			} else {
@@ -455,7 +455,7 @@ int pgmg_batch_step(pgmg_batch *b, int *active, int *finished) {
 		if (nfin) { rc = build_plan(b); if (rc) return rc; }
 		if (b->list_n[L_TOP_ROWS] > 0) {
 			if (b->mixed) {
-				launch_b_zero_x32(b->stream, b->d_slots, wlist(b, L_TOP_PTS));
+				// x32 was cleared by the residual launch above
 				batched_cycle<float>(b);
 				launch_b_add_x(b->stream, b->d_slots, wlist(b, L_TOP_PTS));
 			} else {

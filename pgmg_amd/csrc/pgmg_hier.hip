@@ -275,14 +275,14 @@ int read_scalar(pgmg_hier *h, double *out) {
 
 // Fine residual norm (FP64); optionally write the FP32 residual for the
 // mixed-precision inner cycle.
-int residual_norm(pgmg_hier *h, int level, float *res32_out, double *out) {
+int residual_norm(pgmg_hier *h, int level, float *res32_out, double *out, float *zero32 = nullptr) {
 	Level &L = h->lev[level];
 	int n = residual_partial_count(L.v);
 	int rc = ensure_partial(h, n);
 	if (rc) return rc;
 	{
 		ProfScope ps(h, KC_RESIDUAL, level, L.points());
-		if (res32_out) launch_residual<double, float>(h->stream, L.v, L.A, L.x, L.r, res32_out, h->partial);
+		if (res32_out) launch_residual<double, float>(h->stream, L.v, L.A, L.x, L.r, res32_out, h->partial, zero32);
 		else launch_residual<double, double>(h->stream, L.v, L.A, L.x, L.r, (double *)nullptr, h->partial);
 	}
 	launch_reduce(h->stream, h->partial, n, h->dscalar);
@@ -320,12 +320,6 @@ int run_cycle(pgmg_hier *h, int mode, bool fp32) {
 void free_graphs(pgmg_hier *h) {
 	for (int a = 0; a < 2; a++) for (int b = 0; b < 2; b++)
 		if (h->graph[a][b]) { hipGraphExecDestroy(h->graph[a][b]); h->graph[a][b] = nullptr; }
-}
-
-int ensure_f32(pgmg_hier *h) {
-	if (h->opts.precision != PGMG_PREC_MIXED) return fail(PGMG_ERR_ARG, "hierarchy was not created with MIXED precision");
-	if (!h->ops_ready) return fail(PGMG_ERR_SETUP, "operators not set up (compute_rats)");
-	return PGMG_OK;
 }
 
 }  // namespace
@@ -711,16 +705,17 @@ int pgmg_solve(pgmg_hier *h, double tol, int max_cycles, int *cycles, double *hi
 	int c = 0;
 	double e = 0;
 	for (;;) {
-		rc = residual_norm(h, 0, mixed ? T0.r32 : nullptr, &e);
+		// mixed: the FP64 defect goes to r32 and the FP32 correction x32 is
+		// cleared in the same pass
+		rc = residual_norm(h, 0, mixed ? T0.r32 : nullptr, &e, mixed ? T0.x32 : nullptr);
 		if (rc) return rc;
 		if (hist) hist[c] = e;
 		if (std::isnan(e)) { if (cycles) *cycles = c; return fail(PGMG_ERR_NAN, "NaN residual after %d cycles", c); }
 		if (sqrt(e / bn) < tol) break;
 		if (c >= max_cycles) { if (cycles) *cycles = c; return fail(PGMG_ERR_NOT_CONVERGED, "not converged in %d cycles", c); }
 		if (mixed) {
-			// defect correction: r32 holds (float)(b - A x); e32 = 0;
-			// one FP32 V-cycle; x += e32.
-			launch_fill<float>(h->stream, T0.x32, T0.v.npad, 0.0f);
+			// defect correction: r32 holds (float)(b - A x), x32 = 0; one
+			// FP32 V-cycle; x += x32.
 			rc = run_cycle(h, 0, true);
 			if (rc) return rc;
 			launch_add_f32_to_f64(h->stream, T0.v, T0.x32, T0.x);

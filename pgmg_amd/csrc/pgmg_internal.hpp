@@ -63,8 +63,9 @@ struct WorkList {
 	int n;
 };
 // Unit kinds: the colour-pair row groups of a level, one unit per slot,
-// residual row groups, 64x4-point blocks of a level.
-enum { BK_PAIR0 = 0, BK_PAIR1, BK_PAIR2, BK_PAIR3, BK_ONE, BK_ROWS, BK_PTS };
+// residual row groups, 64x4-point blocks of a level, 64x4 blocks of its
+// 2x2x2 cubes.
+enum { BK_PAIR0 = 0, BK_PAIR1, BK_PAIR2, BK_PAIR3, BK_ONE, BK_ROWS, BK_PTS, BK_CUBES };
 // Work units of one slot for a batched kernel kind at a level.
 int batch_units(int kind, const LevelView &L);
 
@@ -78,7 +79,6 @@ void launch_b_reduce(hipStream_t s, const SlotDev *slots, WorkList wl, double *n
 template <typename T> void launch_b_restrict(hipStream_t s, const SlotDev *slots, WorkList wl, int level);
 template <typename T> void launch_b_interp(hipStream_t s, const SlotDev *slots, WorkList wl, int level);
 template <typename T> void launch_b_lu_solve(hipStream_t s, const SlotDev *slots, WorkList wl, int level);
-void launch_b_zero_x32(hipStream_t s, const SlotDev *slots, WorkList wl);
 void launch_b_add_x(hipStream_t s, const SlotDev *slots, WorkList wl);
 
 // Records the calling thread's last error message (pgmg_last_error) and
@@ -113,9 +113,11 @@ template <typename T> void launch_gs_plane(hipStream_t s, const LevelView &L, co
 template <typename T> void launch_gs_small(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r,
 					   int nsweeps, int mode, int exact);
 // res = r - A x on interior points (res may be null), per-block sums of
-// res^2 into partial (may be null); returns the number of partial sums.
+// res^2 into partial (may be null); zero_out (may be null, FP64 arithmetic
+// only) is set to 0 at interior points in the same pass; returns the number
+// of partial sums.
 template <typename T, typename TR> int launch_residual(hipStream_t s, const LevelView &L, const T *A, const T *x,
-						       const T *r, TR *res, double *partial);
+						       const T *r, TR *res, double *partial, TR *zero_out = nullptr);
 int residual_partial_count(const LevelView &L);
 // out[0] = sum of partial[0..n) in a fixed order.
 void launch_reduce(hipStream_t s, const double *partial, int n, double *out);

@@ -702,7 +702,8 @@ __device__ __forceinline__ double block_sum_256(double v) {
 // one sum per block.  q (the row) must be wavefront-uniform.
 template <typename T, typename TR>
 __device__ __forceinline__ double residual_row_pair(const LevelView &L, const T *__restrict__ A, const T *__restrict__ x,
-						    const T *__restrict__ r, TR *__restrict__ res, int q) {
+						    const T *__restrict__ r, TR *__restrict__ res, int q,
+						    TR *__restrict__ zx = nullptr) {
 	typedef typename Vec2<T>::type V2;
 	typedef typename Vec2<TR>::type R2;
 	const int lane = threadIdx.x;
@@ -713,6 +714,7 @@ __device__ __forceinline__ double residual_row_pair(const LevelView &L, const T 
 		const long long row = idx3(L, 0, j, k);
 		const T *Ar = A + row, *xr = x + row, *rr_ = r + row;
 		TR *resr = res ? res + row : nullptr;
+		TR *zxr = zx ? zx + row : nullptr;
 		// segments of 128 points; the last one takes up to 129, lane 63
 		// doing the odd point out from scalar loads
 		for (int s0 = 0;; s0 += 128) {
@@ -754,6 +756,12 @@ __device__ __forceinline__ double residual_row_pair(const LevelView &L, const T 
 					if (two) { R2 o; o.x = (TR)rv0; o.y = (TR)rv1; st_row<R2>(resr, obr, o); }
 					else st_row<TR>(resr, obr, (TR)rv0);
 					if (ext) st_row<TR>(resr, obr + 2u * (uint32_t)sizeof(TR), (TR)rv2);
+				}
+				if (zxr) {
+					const uint32_t obr = (uint32_t)i0 * (uint32_t)sizeof(TR);
+					if (two) { R2 o; o.x = (TR)0; o.y = (TR)0; st_row<R2>(zxr, obr, o); }
+					else st_row<TR>(zxr, obr, (TR)0);
+					if (ext) st_row<TR>(zxr, obr + 2u * (uint32_t)sizeof(TR), (TR)0);
 				}
 				v2 = v2 + (double)rv0 * (double)rv0;
 				if (two) v2 = v2 + (double)rv1 * (double)rv1;
@@ -842,21 +850,22 @@ __device__ __forceinline__ double residual_row_quad_f32(const LevelView &L, cons
 // FP32 (residual into FP32), two in FP64.
 template <typename T, typename TR>
 __device__ __forceinline__ double residual_row(const LevelView &L, const T *__restrict__ A, const T *__restrict__ x,
-					       const T *__restrict__ r, TR *__restrict__ res, int q) {
-	return residual_row_pair<T, TR>(L, A, x, r, res, q);
+					       const T *__restrict__ r, TR *__restrict__ res, int q, TR *__restrict__ zx = nullptr) {
+	return residual_row_pair<T, TR>(L, A, x, r, res, q, zx);
 }
 template <>
 __device__ __forceinline__ double residual_row<float, float>(const LevelView &L, const float *__restrict__ A,
 							     const float *__restrict__ x, const float *__restrict__ r,
-							     float *__restrict__ res, int q) {
-	return residual_row_quad_f32(L, A, x, r, res, q);
+							     float *__restrict__ res, int q, float *__restrict__ zx) {
+	return residual_row_quad_f32(L, A, x, r, res, q);  // zx: not used in FP32
 }
 
 template <typename T, typename TR>
 __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void k_residual(LevelView L, const T *__restrict__ A, const T *__restrict__ x,
-						  const T *__restrict__ r, TR *__restrict__ res, double *__restrict__ partial) {
+						  const T *__restrict__ r, TR *__restrict__ res, double *__restrict__ partial,
+						  TR *__restrict__ zx) {
 	const int qb = xcd_block((int)blockIdx.x, (int)gridDim.x);
-	const double v2 = residual_row<T, TR>(L, A, x, r, res, __builtin_amdgcn_readfirstlane(qb * 4 + (int)threadIdx.y));
+	const double v2 = residual_row<T, TR>(L, A, x, r, res, __builtin_amdgcn_readfirstlane(qb * 4 + (int)threadIdx.y), zx);
 	if (partial) {
 		// indexed by row group, so the sum order is that of a batched launch
 		const double sblk = block_sum_256(v2);
@@ -869,9 +878,10 @@ int residual_partial_count(const LevelView &L) {
 }
 
 template <typename T, typename TR>
-int launch_residual(hipStream_t s, const LevelView &L, const T *A, const T *x, const T *r, TR *res, double *partial) {
+int launch_residual(hipStream_t s, const LevelView &L, const T *A, const T *x, const T *r, TR *res, double *partial,
+		    TR *zero_out) {
 	const int G = residual_partial_count(L);
-	hipLaunchKernelGGL((k_residual<T, TR>), dim3(G), dim3(64, 4, 1), 0, s, L, A, x, r, res, partial);
+	hipLaunchKernelGGL((k_residual<T, TR>), dim3(G), dim3(64, 4, 1), 0, s, L, A, x, r, res, partial, zero_out);
 	HIPCHECK_LAUNCH();
 	return G;
 }
@@ -985,12 +995,66 @@ __device__ __forceinline__ void restrict_point(const LevelView &F, const LevelVi
 	xc[pc] = (T)0;
 }
 
+// Restriction at a coarse point whose eight surrounding cubes are all
+// interior (1 <= I < lastx, likewise J, K): the same 27 terms, weights and
+// order as restrict_point, with the fine values of each of the 9 fine rows
+// read as one pair (2I, 2I+1) and one scalar (2I-1).
+template <typename T>
+__device__ __forceinline__ void restrict_interior(const LevelView &F, const LevelView &C, const T *__restrict__ res,
+						  T *__restrict__ rc, T *__restrict__ xc, int I, int J, int K) {
+	typedef typename Vec2<T>::type V2;
+	T v[3][3][3];  // [dk+1][dj+1][di+1], fine point (2I+di, 2J+dj, 2K+dk)
+#pragma unroll
+	for (int dk = -1; dk <= 1; dk++) {
+#pragma unroll
+		for (int dj = -1; dj <= 1; dj++) {
+			const long long p = idx3(F, 2 * I, 2 * J + dj, 2 * K + dk);
+			const V2 pr = *(const V2 *)(res + p);
+			v[dk + 1][dj + 1][0] = res[p - 1];
+			v[dk + 1][dj + 1][1] = pr.x;
+			v[dk + 1][dj + 1][2] = pr.y;
+		}
+	}
+	T acc = (T)0;
+#pragma unroll
+	for (int cz = 0; cz < 2; cz++)
+#pragma unroll
+		for (int cy = 0; cy < 2; cy++)
+#pragma unroll
+			for (int cx = 0; cx < 2; cx++)
+#pragma unroll
+				for (int lz = cz ? 0 : 1; lz < 2; lz++)
+#pragma unroll
+					for (int ly = cy ? 0 : 1; ly < 2; ly++)
+#pragma unroll
+						for (int lx = cx ? 0 : 1; lx < 2; lx++) {
+							const T wz = lz == 0 ? (T)1 : (T)0.5;
+							const T wy = ly == 0 ? (T)1 : (T)0.5;
+							const T wx = lx == 0 ? (T)1 : (T)0.5;
+							const T w = (T)0.125 * wx * wy * wz;
+							acc = acc + w * v[2 * cz + lz - 1][2 * cy + ly - 1][2 * cx + lx - 1];
+						}
+	const long long pc = idx3(C, I, J, K);
+	rc[pc] = acc;
+	xc[pc] = (T)0;
+}
+
+template <typename T>
+__device__ __forceinline__ void restrict_any(const LevelView &F, const LevelView &C, const T *__restrict__ res,
+					     T *__restrict__ rc, T *__restrict__ xc, int I, int J, int K) {
+	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
+	if (I >= 1 && I < lastx && J >= 1 && J < lasty && K >= 1 && K < lastz)
+		restrict_interior<T>(F, C, res, rc, xc, I, J, K);
+	else
+		restrict_point<T>(F, C, res, rc, xc, I, J, K);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_restrict(LevelView F, LevelView C, const T *__restrict__ res,
 						  T *__restrict__ rc, T *__restrict__ xc) {
 	const int I = (int)(blockIdx.x * 64 + threadIdx.x), J = (int)(blockIdx.y * 4 + threadIdx.y), K = (int)blockIdx.z;
 	if (I >= C.m || J >= C.n) return;
-	restrict_point<T>(F, C, res, rc, xc, I, J, K);
+	restrict_any<T>(F, C, res, rc, xc, I, J, K);
 }
 
 template <typename T>
@@ -1000,66 +1064,91 @@ void launch_restrict(hipStream_t s, const LevelView &F, const LevelView &C, cons
 	HIPCHECK_LAUNCH();
 }
 
-// Interpolation x_f += P x_c, one thread per fine point.  The fine point's
-// cube and local position select the term; interior cubes use
-// interpolate_full's association order (region.cc:764-771), cubes at the
+// Interpolation x_f += P x_c.  Fine point (2ic+li, 2jc+lj, 2kc+lk) takes its
+// value from the eight coarse points around cube (ic,jc,kc); interior cubes
+// use interpolate_full's association order (region.cc:764-771), cubes at the
 // last position of any axis interpolate_partial's (region.cc:795-809).
 template <typename T>
-__device__ __forceinline__ void interp_point(const LevelView &C, const LevelView &F, const T *__restrict__ xc,
-					     T *__restrict__ xf, int i, int j, int k) {
-	const int ic = i >> 1, jc = j >> 1, kc = k >> 1, li = i & 1, lj = j & 1, lk = k & 1;
+__device__ __forceinline__ T interp_value(int pos, bool full, bool xs, bool ys, bool zs, T v0, T v1, T v2, T v3, T v4,
+					  T v5, T v6, T v7) {
+	const T h = (T)0.5, q = (T)0.25, e = (T)0.125;
+	if (full) {
+		const T t1 = v0 + v1;
+		switch (pos) {
+		case 0: return v0;
+		case 1: return h * t1;
+		case 2: return h * (v0 + v2);
+		case 3: return q * ((t1 + v2) + v3);
+		case 4: return h * (v0 + v4);
+		case 5: return q * (t1 + (v4 + v5));
+		case 6: return q * (((v0 + v2) + v4) + v6);
+		default: return e * (((((t1 + v2) + v3) + (v4 + v5)) + v6) + v7);
+		}
+	}
+	switch (pos) {
+	case 0: return v0;
+	case 1: return xs ? v1 : h * (v0 + v1);
+	case 2: return ys ? v2 : h * (v0 + v2);
+	case 4: return zs ? v4 : h * (v0 + v4);
+	case 5: return zs ? (xs ? v5 : h * (v4 + v5)) : (xs ? h * (v1 + v5) : q * (((v0 + v1) + v4) + v5));
+	case 6: return zs ? (ys ? v6 : h * (v4 + v6)) : (ys ? h * (v2 + v6) : q * (((v0 + v2) + v4) + v6));
+	default: {
+		const T t2 = ys ? (xs ? v3 : h * (v2 + v3)) : (xs ? h * (v1 + v3) : q * (((v0 + v1) + v2) + v3));
+		if (pos == 3) return t2;
+		const T t3 = ys ? (xs ? v7 : h * (v6 + v7)) : (xs ? h * (v5 + v7) : q * (((v4 + v5) + v6) + v7));
+		return zs ? t3 : h * (t3 + t2);
+	}
+	}
+}
+
+// One fine cube per thread: its eight coarse values are read once and the
+// (up to) eight fine points updated as four x pairs (2ic, 2ic+1).
+template <typename T>
+__device__ __forceinline__ void interp_cube(const LevelView &C, const LevelView &F, const T *__restrict__ xc,
+					    T *__restrict__ xf, int ic, int jc, int kc) {
+	typedef typename Vec2<T>::type V2;
 	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
 	const int i1 = ic + 1 < C.m ? ic + 1 : ic, j1 = jc + 1 < C.n ? jc + 1 : jc, k1 = kc + 1 < C.o ? kc + 1 : kc;
-	const T h = (T)0.5, q = (T)0.25, e = (T)0.125;
 	const T v0 = xc[idx3(C, ic, jc, kc)], v1 = xc[idx3(C, i1, jc, kc)];
 	const T v2 = xc[idx3(C, ic, j1, kc)], v3 = xc[idx3(C, i1, j1, kc)];
 	const T v4 = xc[idx3(C, ic, jc, k1)], v5 = xc[idx3(C, i1, jc, k1)];
 	const T v6 = xc[idx3(C, ic, j1, k1)], v7 = xc[idx3(C, i1, j1, k1)];
-	const int pos = li + 2 * lj + 4 * lk;
-	T val;
-	if (ic < lastx && jc < lasty && kc < lastz) {
-		const T t1 = v0 + v1;
-		switch (pos) {
-		case 0: val = v0; break;
-		case 1: val = h * t1; break;
-		case 2: val = h * (v0 + v2); break;
-		case 3: val = q * ((t1 + v2) + v3); break;
-		case 4: val = h * (v0 + v4); break;
-		case 5: val = q * (t1 + (v4 + v5)); break;
-		case 6: val = q * (((v0 + v2) + v4) + v6); break;
-		default: val = e * (((((t1 + v2) + v3) + (v4 + v5)) + v6) + v7); break;
-		}
-	} else {
-		const bool xs = (ic == lastx) && !(F.m & 1), ys = (jc == lasty) && !(F.n & 1), zs = (kc == lastz) && !(F.o & 1);
-		switch (pos) {
-		case 0: val = v0; break;
-		case 1: val = xs ? v1 : h * (v0 + v1); break;
-		case 2: val = ys ? v2 : h * (v0 + v2); break;
-		case 4: val = zs ? v4 : h * (v0 + v4); break;
-		case 5: val = zs ? (xs ? v5 : h * (v4 + v5)) : (xs ? h * (v1 + v5) : q * (((v0 + v1) + v4) + v5)); break;
-		case 6: val = zs ? (ys ? v6 : h * (v4 + v6)) : (ys ? h * (v2 + v6) : q * (((v0 + v2) + v4) + v6)); break;
-		default: {
-			const T t2 = ys ? (xs ? v3 : h * (v2 + v3)) : (xs ? h * (v1 + v3) : q * (((v0 + v1) + v2) + v3));
-			if (pos == 3) { val = t2; break; }
-			const T t3 = ys ? (xs ? v7 : h * (v6 + v7)) : (xs ? h * (v5 + v7) : q * (((v4 + v5) + v6) + v7));
-			val = zs ? t3 : h * (t3 + t2);
-		}
+	const bool full = ic < lastx && jc < lasty && kc < lastz;
+	const bool xs = (ic == lastx) && !(F.m & 1), ys = (jc == lasty) && !(F.n & 1), zs = (kc == lastz) && !(F.o & 1);
+	const bool two = 2 * ic + 1 < F.m;
+#pragma unroll
+	for (int lk = 0; lk < 2; lk++) {
+		if (2 * kc + lk >= F.o) break;
+#pragma unroll
+		for (int lj = 0; lj < 2; lj++) {
+			if (2 * jc + lj >= F.n) break;
+			const int pos = 2 * lj + 4 * lk;
+			const long long p = idx3(F, 2 * ic, 2 * jc + lj, 2 * kc + lk);
+			const T a = interp_value<T>(pos, full, xs, ys, zs, v0, v1, v2, v3, v4, v5, v6, v7);
+			if (two) {
+				const T b = interp_value<T>(pos + 1, full, xs, ys, zs, v0, v1, v2, v3, v4, v5, v6, v7);
+				V2 f = *(const V2 *)(xf + p);
+				f.x = f.x + a;
+				f.y = f.y + b;
+				*(V2 *)(xf + p) = f;
+			} else {
+				xf[p] = xf[p] + a;
+			}
 		}
 	}
-	const long long p = idx3(F, i, j, k);
-	xf[p] = xf[p] + val;
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_interp(LevelView C, LevelView F, const T *__restrict__ xc, T *__restrict__ xf) {
-	const int i = (int)(blockIdx.x * 64 + threadIdx.x), j = (int)(blockIdx.y * 4 + threadIdx.y), k = (int)blockIdx.z;
-	if (i >= F.m || j >= F.n) return;
-	interp_point<T>(C, F, xc, xf, i, j, k);
+	const int ic = (int)(blockIdx.x * 64 + threadIdx.x), jc = (int)(blockIdx.y * 4 + threadIdx.y), kc = (int)blockIdx.z;
+	if (ic >= (F.m + 1) / 2 || jc >= (F.n + 1) / 2) return;
+	interp_cube<T>(C, F, xc, xf, ic, jc, kc);
 }
 
 template <typename T>
 void launch_interp(hipStream_t s, const LevelView &C, const LevelView &F, const T *xc, T *xf) {
-	dim3 grid((F.m + 63) / 64, (F.n + 3) / 4, F.o), block(64, 4, 1);
+	const int ncx = (F.m + 1) / 2, ncy = (F.n + 1) / 2, ncz = (F.o + 1) / 2;
+	dim3 grid((ncx + 63) / 64, (ncy + 3) / 4, ncz), block(64, 4, 1);
 	hipLaunchKernelGGL(k_interp<T>, grid, block, 0, s, C, F, xc, xf);
 	HIPCHECK_LAUNCH();
 }
@@ -1442,6 +1531,7 @@ int batch_units(int kind, const LevelView &L) {
 	}
 	case BK_ONE: return 1;
 	case BK_ROWS: return (L.n * L.o + 3) / 4;
+	case BK_CUBES: return (((L.m + 1) / 2 + 63) / 64) * (((L.n + 1) / 2 + 3) / 4) * ((L.o + 1) / 2);
 	default: return ((L.m + 63) / 64) * ((L.n + 3) / 4) * L.o;
 	}
 }
@@ -1475,6 +1565,16 @@ __device__ __forceinline__ bool pts_unit(const LevelView &L, int lu, int &i, int
 	j = ((lu / gx) % gy) * 4 + (int)threadIdx.y;
 	k = lu / (gx * gy);
 	return i < L.m && j < L.n;
+}
+
+// Unit lu of kind BK_CUBES: the fine cubes (2ic.., 2jc.., 2kc..) of level L.
+__device__ __forceinline__ bool cubes_unit(const LevelView &L, int lu, int &ic, int &jc, int &kc) {
+	const int ncx = (L.m + 1) / 2, ncy = (L.n + 1) / 2;
+	const int gx = (ncx + 63) / 64, gy = (ncy + 3) / 4;
+	ic = (lu % gx) * 64 + (int)threadIdx.x;
+	jc = ((lu / gx) % gy) * 4 + (int)threadIdx.y;
+	kc = lu / (gx * gy);
+	return ic < ncx && jc < ncy;
 }
 
 #define BATCH_LOOP                                                   \
@@ -1524,7 +1624,7 @@ __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void kb_residual_top(const Sl
 	BATCH_LOOP
 		const LevelView L = S->lv[0];
 		const int q = __builtin_amdgcn_readfirstlane(lu * 4 + (int)threadIdx.y);
-		const double v2 = to_f32 ? residual_row<double, float>(L, S->A[0], S->x[0], S->r[0], S->r32[0], q)
+		const double v2 = to_f32 ? residual_row<double, float>(L, S->A[0], S->x[0], S->r[0], S->r32[0], q, S->x32[0])
 					 : residual_row<double, double>(L, S->A[0], S->x[0], S->r[0], (double *)nullptr, q);
 		const double sblk = block_sum_256(v2);
 		if (threadIdx.x == 0 && threadIdx.y == 0) S->partial[lu] = sblk;
@@ -1557,7 +1657,7 @@ __global__ __launch_bounds__(256) void kb_restrict(const SlotDev *__restrict__ s
 		const LevelView C = S->lv[level];
 		int I, J, K;
 		if (pts_unit(C, lu, I, J, K))
-			restrict_point<T>(S->lv[level - 1], C, sres<T>(S), sr<T>(S, level), sx<T>(S, level), I, J, K);
+			restrict_any<T>(S->lv[level - 1], C, sres<T>(S), sr<T>(S, level), sx<T>(S, level), I, J, K);
 	}
 }
 
@@ -1565,8 +1665,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void kb_interp(const SlotDev *__restrict__ slots, WorkList wl, int level) {
 	BATCH_LOOP
 		const LevelView F = S->lv[level - 1];
-		int i, j, k;
-		if (pts_unit(F, lu, i, j, k)) interp_point<T>(S->lv[level], F, sx<T>(S, level), sx<T>(S, level - 1), i, j, k);
+		int ic, jc, kc;
+		if (cubes_unit(F, lu, ic, jc, kc)) interp_cube<T>(S->lv[level], F, sx<T>(S, level), sx<T>(S, level - 1), ic, jc, kc);
 	}
 }
 
@@ -1576,14 +1676,6 @@ __global__ __launch_bounds__(1024) void kb_lu_solve(const SlotDev *__restrict__ 
 		(void)lu;
 		lu_solve_body<T>(S->lv[level], S->lu_n, S->LU, S->ipiv, sr<T>(S, level), sx<T>(S, level));
 		__syncthreads();
-	}
-}
-
-__global__ __launch_bounds__(256) void kb_zero_x32(const SlotDev *__restrict__ slots, WorkList wl) {
-	BATCH_LOOP
-		const LevelView L = S->lv[0];
-		int i, j, k;
-		if (pts_unit(L, lu, i, j, k)) S->x32[0][idx3(L, i, j, k)] = 0.0f;
 	}
 }
 
@@ -1648,11 +1740,6 @@ void launch_b_lu_solve(hipStream_t s, const SlotDev *slots, WorkList wl, int lev
 	hipLaunchKernelGGL(kb_lu_solve<T>, dim3(kBatchBlocks1024), dim3(1024), 0, s, slots, wl, level);
 	HIPCHECK_LAUNCH();
 }
-void launch_b_zero_x32(hipStream_t s, const SlotDev *slots, WorkList wl) {
-	if (wl.n <= 0) return;
-	hipLaunchKernelGGL(kb_zero_x32, dim3(kBatchBlocks256), dim3(64, 4), 0, s, slots, wl);
-	HIPCHECK_LAUNCH();
-}
 void launch_b_add_x(hipStream_t s, const SlotDev *slots, WorkList wl) {
 	if (wl.n <= 0) return;
 	hipLaunchKernelGGL(kb_add_x, dim3(kBatchBlocks256), dim3(64, 4), 0, s, slots, wl);
@@ -1678,9 +1765,9 @@ void launch_b_add_x(hipStream_t s, const SlotDev *slots, WorkList wl) {
 	template void launch_b_lu_solve<T>(hipStream_t, const SlotDev *, WorkList, int);
 INST(double)
 INST(float)
-template int launch_residual<double, double>(hipStream_t, const LevelView &, const double *, const double *, const double *, double *, double *);
-template int launch_residual<double, float>(hipStream_t, const LevelView &, const double *, const double *, const double *, float *, double *);
-template int launch_residual<float, float>(hipStream_t, const LevelView &, const float *, const float *, const float *, float *, double *);
+template int launch_residual<double, double>(hipStream_t, const LevelView &, const double *, const double *, const double *, double *, double *, double *);
+template int launch_residual<double, float>(hipStream_t, const LevelView &, const double *, const double *, const double *, float *, double *, float *);
+template int launch_residual<float, float>(hipStream_t, const LevelView &, const float *, const float *, const float *, float *, double *, float *);
 template void launch_fill<int>(hipStream_t, int *, long long, int);
 
 }  // namespace pgmg
