@@ -10,9 +10,7 @@
 // i even, moves with one 16-byte (FP64) or 8-byte (FP32) vector access, and
 // a quad (i .. i+3), i a multiple of 4, with one 16-byte FP32 access.
 // Halo entries are zero and stay zero, so a stencil at the grid edge reads
-// zeros instead of branching.  Coefficients are stored as 27 separate planes
-// (structure of arrays), plane e at A + e*npad, so that a wavefront reading
-// one coefficient for consecutive points reads contiguous memory.
+// zeros instead of branching.  Coefficients: see arow0.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -36,6 +34,23 @@ inline LevelView make_level_view(int m, int n, int o) {
 	v.base = 4 + v.px + v.pxy;
 	v.npad = v.pxy * (o + 2);
 	return v;
+}
+
+// Point (i,j,k) of a per-point array.
+__host__ __device__ inline long long idx3(const LevelView &L, int i, int j, int k) {
+	return L.base + i + (long long)L.px * j + L.pxy * k;
+}
+// Coefficients are interleaved by grid row: the 27 planes' rows of grid row
+// (j,k) are one contiguous block of 27*px elements, plane e's row at offset
+// e*px in it, entry i at +4+i as in the per-point arrays.  A wavefront that
+// streams one grid row therefore reads one contiguous 27*px block (one or
+// two memory pages), not 27 rows a plane apart.  The array keeps the size of
+// 27 per-point arrays; the halo rows' blocks are unused.
+__host__ __device__ inline long long arow0(const LevelView &L, int j, int k) {  // plane 0, i = 0
+	return 27 * (idx3(L, 0, j, k) - 4) + 4;
+}
+__host__ __device__ inline long long aidx3(const LevelView &L, int e, int i, int j, int k) {
+	return arow0(L, j, k) + (long long)e * L.px + i;
 }
 
 // ---- batches of independent hierarchies (one per slot) ----

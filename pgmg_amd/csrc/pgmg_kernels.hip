@@ -25,9 +25,6 @@ namespace pgmg {
 #define PGMG_STREAM_ATTR
 #endif
 
-__device__ __forceinline__ long long idx3(const LevelView &L, int i, int j, int k) {
-	return L.base + i + (long long)L.px * j + L.pxy * k;
-}
 
 // The hardware deals workgroups to the 8 XCDs round-robin (workgroup b runs
 // on XCD b % 8), and each XCD has its own L2.  Kernels whose neighbouring
@@ -39,35 +36,22 @@ __device__ __forceinline__ int xcd_block(int b, int G) {
 	return x * q + min(x, rmd) + (b >> 3);
 }
 
-// Gauss-Seidel update of the point at padded index p (region.cc:369-438):
-// sum over the 26 off-centre entries in box order from zero, then
+// Gauss-Seidel update of point (i,j,k) (region.cc:369-438): sum over the
+// 26 off-centre entries in box order from zero, then
 // x = (r - Ax) * (1/a_centre) with the reciprocal taken first.
 template <typename T>
 __device__ __forceinline__ void gs_point(const LevelView &L, const T *__restrict__ A, T *__restrict__ x,
-					 const T *__restrict__ r, long long p) {
-	const long long px = L.px, pxy = L.pxy, np = L.npad;
+					 const T *__restrict__ r, int i, int j, int k) {
+	const long long px = L.px, pxy = L.pxy, p = idx3(L, i, j, k);
+	const T *Ap = A + aidx3(L, 0, i, j, k);
 	T ax = (T)0;
 #pragma unroll
 	for (int e = 0; e < 27; e++) {
 		if (e == 13) continue;
 		const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
-		ax = ax + A[e * np + p] * x[p + di + dj * px + dk * pxy];
+		ax = ax + Ap[e * px] * x[p + di + dj * px + dk * pxy];
 	}
-	x[p] = (r[p] - ax) * ((T)1 / A[13 * np + p]);
-}
-
-// Residual r - sum over all 27 entries in box order (region.cc:1833-1842).
-template <typename T>
-__device__ __forceinline__ T residual_point(const LevelView &L, const T *__restrict__ A, const T *__restrict__ x,
-					    const T *__restrict__ r, long long p) {
-	const long long px = L.px, pxy = L.pxy, np = L.npad;
-	T ax = (T)0;
-#pragma unroll
-	for (int e = 0; e < 27; e++) {
-		const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
-		ax = ax + A[e * np + p] * x[p + di + dj * px + dk * pxy];
-	}
-	return r[p] - ax;
+	x[p] = (r[p] - ax) * ((T)1 / Ap[13 * px]);
 }
 
 // ------------------------------------------------------------ smoothers
@@ -81,7 +65,7 @@ __global__ __launch_bounds__(256) void k_gs_colour(LevelView L, const T *__restr
 	const int j = 2 * (int)(blockIdx.y * 4 + threadIdx.y) + cj;
 	const int k = 2 * (int)blockIdx.z + ck;
 	if (i >= L.m || j >= L.n || k >= L.o) return;
-	gs_point<T>(L, A, x, r, idx3(L, i, j, k));
+	gs_point<T>(L, A, x, r, i, j, k);
 }
 
 template <typename T>
@@ -155,8 +139,8 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 	typedef typename Vec2<T>::type V2;
 	const int lane = threadIdx.x;
 	const long long row = idx3(L, 0, j, k);
-	const long long px = L.px, pxy = L.pxy, np = L.npad;
-	const T *Ar = A + row;
+	const long long px = L.px, pxy = L.pxy;
+	const T *Ar = A + arow0(L, j, k);
 	const T *rr_ = r + row;
 	T *xr = x + row;
 	T carry = (T)0;
@@ -182,9 +166,9 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 			for (int rq = 0; rq < 9; rq++) {
 				const int dj = rq % 3 - 1, dk = rq / 3 - 1;
 				const T *xq = xr + dj * px + dk * pxy;
-				const V2 am = ld_stream<V2>(Ar + (3 * rq) * np, ob);
-				const V2 a0 = ld_stream<V2>(Ar + (3 * rq + 1) * np, ob);
-				const V2 ap = ld_stream<V2>(Ar + (3 * rq + 2) * np, ob);
+				const V2 am = ld_stream<V2>(Ar + (3 * rq) * px, ob);
+				const V2 a0 = ld_stream<V2>(Ar + (3 * rq + 1) * px, ob);
+				const V2 ap = ld_stream<V2>(Ar + (3 * rq + 2) * px, ob);
 				const V2 xv = ld_row<V2>(xq, ob);
 				const T xm = ld_row<T>(xq - 1, ob);
 				const T xp = doO ? ld_row<T>(xq + 2, ob) : (T)0;
@@ -245,8 +229,8 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 						const float *__restrict__ r, int j, int k) {
 	const int lane = threadIdx.x;
 	const long long row = idx3(L, 0, j, k);
-	const long long px = L.px, pxy = L.pxy, np = L.npad;
-	const float *Ar = A + row;
+	const long long px = L.px, pxy = L.pxy;
+	const float *Ar = A + arow0(L, j, k);
 	const float *rr_ = r + row;
 	float *xr = x + row;
 	float carry = 0.0f;
@@ -278,17 +262,17 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 			for (int rq = 0; rq < 9; rq++) {
 				const int dj = rq % 3 - 1, dk = rq / 3 - 1;
 				const float *xq = xr + dj * px + dk * pxy;
-				const float4 am = ld_row<float4>(Ar + (3 * rq) * np, ob);
-				const float4 a0 = ld_row<float4>(Ar + (3 * rq + 1) * np, ob);
-				const float4 ap = ld_row<float4>(Ar + (3 * rq + 2) * np, ob);
+				const float4 am = ld_row<float4>(Ar + (3 * rq) * px, ob);
+				const float4 a0 = ld_row<float4>(Ar + (3 * rq + 1) * px, ob);
+				const float4 ap = ld_row<float4>(Ar + (3 * rq + 2) * px, ob);
 				const float4 xv = ld_row<float4>(xq, ob);
 				const float xm = ld_row<float>(xq - 1, ob);
 				const float xp = doO3 ? ld_row<float>(xq + 4, ob) : 0.0f;
 				float bm = 0.0f, b0 = 0.0f, bp = 0.0f;
 				if (ext) {
-					bm = ld_row<float>(Ar + (3 * rq) * np, ob + 16u);
-					b0 = ld_row<float>(Ar + (3 * rq + 1) * np, ob + 16u);
-					bp = ld_row<float>(Ar + (3 * rq + 2) * np, ob + 16u);
+					bm = ld_row<float>(Ar + (3 * rq) * px, ob + 16u);
+					b0 = ld_row<float>(Ar + (3 * rq + 1) * px, ob + 16u);
+					bp = ld_row<float>(Ar + (3 * rq + 2) * px, ob + 16u);
 				}
 				if (rq == 4) {
 					e0 = e0 + am.x * xm;
@@ -416,7 +400,7 @@ __global__ __launch_bounds__(256) void k_gs_plane(LevelView L, const T *__restri
 	const int j = jlo + (int)(blockIdx.x * blockDim.x + threadIdx.x);
 	if (j > jhi) return;
 	const int i = sres - 2 * j;
-	gs_point<T>(L, A, x, r, idx3(L, i, j, k));
+	gs_point<T>(L, A, x, r, i, j, k);
 }
 
 template <typename T>
@@ -450,7 +434,7 @@ __device__ __forceinline__ void gs_small_body(const LevelView &L, const T *__res
 					for (int q = tid; q < npair; q += nt) {
 						const int j = q % L.n, k = q / L.n;
 						const int i = t - 2 * j - 4 * k;
-						if (i >= 0 && i < L.m) gs_point<T>(L, A, x, r, idx3(L, i, j, k));
+						if (i >= 0 && i < L.m) gs_point<T>(L, A, x, r, i, j, k);
 					}
 					__syncthreads();
 				}
@@ -462,7 +446,7 @@ __device__ __forceinline__ void gs_small_body(const LevelView &L, const T *__res
 					const int tot = na * nb * nd;
 					for (int q = tid; q < tot; q += nt) {
 						const int a = q % na, b = (q / na) % nb, d = q / (na * nb);
-						gs_point<T>(L, A, x, r, idx3(L, 2 * a + ci, 2 * b + cj, 2 * d + ck));
+						gs_point<T>(L, A, x, r, 2 * a + ci, 2 * b + cj, 2 * d + ck);
 					}
 					__syncthreads();
 				}
@@ -495,10 +479,11 @@ __device__ __forceinline__ void gs_lds_update(const T *__restrict__ c, T rv, T *
 template <typename T>
 __device__ __forceinline__ void gs_lds_global(const LevelView &L, const T *__restrict__ A, const T *__restrict__ r,
 					      T *__restrict__ xs, int i, int j, int k, int hx, int hxy) {
-	const long long p = idx3(L, i, j, k), np = L.npad;
+	const long long p = idx3(L, i, j, k);
+	const T *Ap = A + aidx3(L, 0, i, j, k);
 	T c[27];
 #pragma unroll
-	for (int e = 0; e < 27; e++) c[e] = A[e * np + p];
+	for (int e = 0; e < 27; e++) c[e] = Ap[e * L.px];
 	gs_lds_update<T>(c, r[p], xs, (i + 1) + hx * (j + 1) + hxy * (k + 1), hx, hxy);
 }
 
@@ -527,7 +512,8 @@ __device__ __forceinline__ void gs_lds_pair_pass(const LevelView &L, const T *__
 	if (live) {
 		const int i0 = 2 * (q % na), j = 2 * ((q / na) % nb) + cj, k = 2 * (q / (na * nb)) + ck;
 		two = i0 + 1 < L.m;
-		const long long p = idx3(L, i0, j, k), np = L.npad;
+		const long long p = idx3(L, i0, j, k);
+		const T *Ap = A + aidx3(L, 0, i0, j, k);
 		lq = (i0 + 1) + hx * (j + 1) + hxy * (k + 1);
 		const V2 rv = *(const V2 *)(r + p);
 		r2 = rv.y;
@@ -535,7 +521,7 @@ __device__ __forceinline__ void gs_lds_pair_pass(const LevelView &L, const T *__
 		T ax = (T)0, a13 = (T)1;
 #pragma unroll
 		for (int e = 0; e < 27; e++) {
-			const V2 a = *(const V2 *)(A + e * np + p);
+			const V2 a = *(const V2 *)(Ap + e * L.px);
 			c2[e] = a.y;
 			if (e == 13) { a13 = a.x; continue; }
 			const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
@@ -581,9 +567,10 @@ __device__ __forceinline__ void gs_lds_run(const LevelView &L, const T *__restri
 		own = true;
 		ot = i + 2 * j + 4 * k;
 		olq = (i + 1) + hx * (j + 1) + hxy * (k + 1);
-		const long long p = idx3(L, i, j, k), np = L.npad;
+		const long long p = idx3(L, i, j, k);
+		const T *Ap = A + aidx3(L, 0, i, j, k);
 #pragma unroll
-		for (int e = 0; e < 27; e++) ca[e] = A[e * np + p];
+		for (int e = 0; e < 27; e++) ca[e] = Ap[e * L.px];
 		cr = r[p];
 	}
 	__syncthreads();
@@ -707,12 +694,12 @@ __device__ __forceinline__ double residual_row_pair(const LevelView &L, const T 
 	typedef typename Vec2<T>::type V2;
 	typedef typename Vec2<TR>::type R2;
 	const int lane = threadIdx.x;
-	const long long px = L.px, pxy = L.pxy, np = L.npad;
+	const long long px = L.px, pxy = L.pxy;
 	double v2 = 0.0;
 	if (q < L.n * L.o) {
 		const int j = q % L.n, k = q / L.n;
 		const long long row = idx3(L, 0, j, k);
-		const T *Ar = A + row, *xr = x + row, *rr_ = r + row;
+		const T *Ar = A + arow0(L, j, k), *xr = x + row, *rr_ = r + row;
 		TR *resr = res ? res + row : nullptr;
 		TR *zxr = zx ? zx + row : nullptr;
 		// segments of 128 points; the last one takes up to 129, lane 63
@@ -731,9 +718,9 @@ __device__ __forceinline__ double residual_row_pair(const LevelView &L, const T 
 				for (int rq = 0; rq < 9; rq++) {
 					const int dj = rq % 3 - 1, dk = rq / 3 - 1;
 					const T *xq = xr + dj * px + dk * pxy;
-					const V2 am = ld_stream<V2>(Ar + (3 * rq) * np, ob);
-					const V2 a0 = ld_stream<V2>(Ar + (3 * rq + 1) * np, ob);
-					const V2 ap = ld_stream<V2>(Ar + (3 * rq + 2) * np, ob);
+					const V2 am = ld_stream<V2>(Ar + (3 * rq) * px, ob);
+					const V2 a0 = ld_stream<V2>(Ar + (3 * rq + 1) * px, ob);
+					const V2 ap = ld_stream<V2>(Ar + (3 * rq + 2) * px, ob);
 					const V2 xv = ld_row<V2>(xq, ob);
 					const T xm = ld_row<T>(xq - 1, ob);
 					const T xp = two ? ld_row<T>(xq + 2, ob) : (T)0;
@@ -745,9 +732,9 @@ __device__ __forceinline__ double residual_row_pair(const LevelView &L, const T 
 					ay = ay + ap.y * xp;
 					if (ext) {
 						const uint32_t oz = ob + 2u * (uint32_t)sizeof(T);
-						az = az + ld_row<T>(Ar + (3 * rq) * np, oz) * xv.y;
-						az = az + ld_row<T>(Ar + (3 * rq + 1) * np, oz) * xp;
-						az = az + ld_row<T>(Ar + (3 * rq + 2) * np, oz) * (T)0;  // the halo point i = m
+						az = az + ld_row<T>(Ar + (3 * rq) * px, oz) * xv.y;
+						az = az + ld_row<T>(Ar + (3 * rq + 1) * px, oz) * xp;
+						az = az + ld_row<T>(Ar + (3 * rq + 2) * px, oz) * (T)0;  // the halo point i = m
 					}
 				}
 				const T rv0 = rr.x - ax, rv1 = rr.y - ay, rv2 = rz - az;
@@ -779,12 +766,12 @@ __device__ __forceinline__ double residual_row_quad_f32(const LevelView &L, cons
 							const float *__restrict__ x, const float *__restrict__ r,
 							float *__restrict__ res, int q) {
 	const int lane = threadIdx.x;
-	const long long px = L.px, pxy = L.pxy, np = L.npad;
+	const long long px = L.px, pxy = L.pxy;
 	double v2 = 0.0;
 	if (q < L.n * L.o) {
 		const int j = q % L.n, k = q / L.n;
 		const long long row = idx3(L, 0, j, k);
-		const float *Ar = A + row, *xr = x + row, *rr_ = r + row;
+		const float *Ar = A + arow0(L, j, k), *xr = x + row, *rr_ = r + row;
 		float *resr = res ? res + row : nullptr;
 		for (int s0 = 0;; s0 += 256) {
 			const bool last = L.m - s0 <= 257;
@@ -800,9 +787,9 @@ __device__ __forceinline__ double residual_row_quad_f32(const LevelView &L, cons
 				for (int rq = 0; rq < 9; rq++) {
 					const int dj = rq % 3 - 1, dk = rq / 3 - 1;
 					const float *xq = xr + dj * px + dk * pxy;
-					const float4 am = ld_row<float4>(Ar + (3 * rq) * np, ob);
-					const float4 ac = ld_row<float4>(Ar + (3 * rq + 1) * np, ob);
-					const float4 ap = ld_row<float4>(Ar + (3 * rq + 2) * np, ob);
+					const float4 am = ld_row<float4>(Ar + (3 * rq) * px, ob);
+					const float4 ac = ld_row<float4>(Ar + (3 * rq + 1) * px, ob);
+					const float4 ap = ld_row<float4>(Ar + (3 * rq + 2) * px, ob);
 					const float4 xv = ld_row<float4>(xq, ob);
 					const float xm = ld_row<float>(xq - 1, ob);
 					const float xp = nv == 4 ? ld_row<float>(xq + 4, ob) : 0.0f;
@@ -819,9 +806,9 @@ __device__ __forceinline__ double residual_row_quad_f32(const LevelView &L, cons
 					a3s = a3s + ac.w * xv.w;
 					a3s = a3s + ap.w * xp;
 					if (ext) {
-						azs = azs + ld_row<float>(Ar + (3 * rq) * np, ob + 16u) * xv.w;
-						azs = azs + ld_row<float>(Ar + (3 * rq + 1) * np, ob + 16u) * xp;
-						azs = azs + ld_row<float>(Ar + (3 * rq + 2) * np, ob + 16u) * 0.0f;  // halo
+						azs = azs + ld_row<float>(Ar + (3 * rq) * px, ob + 16u) * xv.w;
+						azs = azs + ld_row<float>(Ar + (3 * rq + 1) * px, ob + 16u) * xp;
+						azs = azs + ld_row<float>(Ar + (3 * rq + 2) * px, ob + 16u) * 0.0f;  // halo
 					}
 				}
 				const float v0 = rr.x - a0s, v1 = rr.y - a1s, v2f = rr.z - a2s, v3 = rr.w - a3s, vz = rz - azs;
@@ -1189,7 +1176,6 @@ __global__ __launch_bounds__(256) void k_rat(LevelView F, LevelView C, const dou
 		psupport(JK, F.o, plz, phz, pcz);
 		const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
 		const bool evx = !(F.m & 1), evy = !(F.n & 1), evz = !(F.o & 1);
-		const long long np = F.npad;
 		for (int cz = 0; cz < 2; cz++) {
 			const int kc = K - 1 + cz;
 			if (kc < 0 || kc > lastz) continue;
@@ -1229,7 +1215,7 @@ __global__ __launch_bounds__(256) void k_rat(LevelView F, LevelView C, const dou
 								else { if (sx) continue; wx = 0.5; }
 								const int gxl = max(plx, fi - 1), gxh = min(phx, min(fi + 2, F.m));
 								if (gxl >= gxh) continue;
-								const long long pf = idx3(F, fi, fj, fk);
+								const double *Afr = Af + aidx3(F, 0, fi, fj, fk);
 								double inner = 0.0;
 								for (int kk = gzl; kk < gzh; kk++) {
 									const double kfac = kk == pcz ? 1.0 : 0.5;
@@ -1237,7 +1223,7 @@ __global__ __launch_bounds__(256) void k_rat(LevelView F, LevelView C, const dou
 										const double jkfac = jj == pcy ? kfac : 0.5 * kfac;
 										for (int ii = gxl; ii < gxh; ii++) {
 											const int ef = (kk - fk + 1) * 9 + (jj - fj + 1) * 3 + (ii - fi + 1);
-											inner = inner + (0.125 * Af[ef * np + pf]) * (ii == pcx ? jkfac : 0.5 * jkfac);
+											inner = inner + (0.125 * Afr[ef * F.px]) * (ii == pcx ? jkfac : 0.5 * jkfac);
 										}
 									}
 								}
@@ -1250,7 +1236,7 @@ __global__ __launch_bounds__(256) void k_rat(LevelView F, LevelView C, const dou
 			}
 		}
 	}
-	Ac[eJ * C.npad + idx3(C, I, Jy, K)] = acc;
+	Ac[aidx3(C, eJ, I, Jy, K)] = acc;
 }
 
 void launch_rat(hipStream_t s, const LevelView &F, const LevelView &C, const double *Af, double *Ac) {
@@ -1272,7 +1258,7 @@ __global__ void k_lu_assemble(LevelView L, const double *__restrict__ A, double 
 	const int qi = i + e % 3 - 1, qj = j + (e / 3) % 3 - 1, qk = k + e / 9 - 1;
 	if (qi < 0 || qi >= L.m || qj < 0 || qj >= L.n || qk < 0 || qk >= L.o) return;
 	const int q = qi + L.m * (qj + L.n * qk);
-	LU[p + (long long)q * N] = A[e * L.npad + idx3(L, i, j, k)];
+	LU[p + (long long)q * N] = A[aidx3(L, e, i, j, k)];
 }
 
 void launch_lu_assemble(hipStream_t s, const LevelView &L, const double *A, double *LU) {
@@ -1420,8 +1406,9 @@ __global__ __launch_bounds__(256) void k_generate(LevelView L, int family, uint6
 	double row[27];
 	pgmg_row27(family, seed, L.m, L.n, L.o, i, j, k, row);
 	const long long p = idx3(L, i, j, k);
+	double *Ap = A + aidx3(L, 0, i, j, k);
 #pragma unroll
-	for (int e = 0; e < 27; e++) A[e * L.npad + p] = row[e];
+	for (int e = 0; e < 27; e++) Ap[e * L.px] = row[e];
 	r[p] = pgmg_rhs(seed, L.m, L.n, L.o, i, j, k);
 }
 
@@ -1434,6 +1421,11 @@ void launch_generate(hipStream_t s, const LevelView &L, int family, uint64_t see
 __device__ __forceinline__ long long compact_to_padded(const LevelView &L, long long p) {
 	const int i = (int)(p % L.m), j = (int)((p / L.m) % L.n), k = (int)(p / ((long long)L.m * L.n));
 	return idx3(L, i, j, k);
+}
+
+__device__ __forceinline__ long long compact_to_coef(const LevelView &L, int e, long long p) {
+	const int i = (int)(p % L.m), j = (int)((p / L.m) % L.n), k = (int)(p / ((long long)L.m * L.n));
+	return aidx3(L, e, i, j, k);
 }
 
 __global__ void k_pack_vec(LevelView L, const double *__restrict__ c, double *__restrict__ d, long long p0, long long n) {
@@ -1449,14 +1441,14 @@ __global__ void k_pack_op(LevelView L, const double *__restrict__ c, double *__r
 	if (t >= n * 27) return;
 	const int e = (int)(t % 27);
 	const long long p = t / 27;
-	A[e * L.npad + compact_to_padded(L, p0 + p)] = c[t];
+	A[compact_to_coef(L, e, p0 + p)] = c[t];
 }
 __global__ void k_unpack_op(LevelView L, const double *__restrict__ A, double *__restrict__ c, long long p0, long long n) {
 	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= n * 27) return;
 	const int e = (int)(t % 27);
 	const long long p = t / 27;
-	c[t] = A[e * L.npad + compact_to_padded(L, p0 + p)];
+	c[t] = A[compact_to_coef(L, e, p0 + p)];
 }
 
 static unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
