@@ -260,7 +260,7 @@ def main():
     bytes_launch = 120.0 * pts_gs / max(n_gs, 1)
     achieved = bytes_launch / avg_launch_s / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_gs_colour_f32_257.json")
+    tf = os.path.join(ROOT, "profiles", "traffic_gs_pair_f32_257.json")
     if os.path.exists(tf):
         try:
             traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
