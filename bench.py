@@ -58,17 +58,34 @@ def solve_timed(P, h, steps, warmup, prof_level=0):
 
 def cpu_baseline():
     """The real reference (oracle/_ref/pgmg_ref, built from the reference's
-    sources) on a bounded sample: 2 V-cycles of the same 257^3 problem, one
-    MPI rank; the time to 1e-10 is that per-cycle time times the cycle count
-    the reference needs (10 at 257^3).  Falls back to the oracle (port) at
-    129^3 when the reference binary is not present."""
+    sources) on this host: the same 257^3 problem solved to 1e-10 by
+    mpiexec over up to 16 host cores (the CPU share of one GPU), set-up
+    excluded.  Multi-rank iterates differ slightly from the 1-rank ones; the
+    cycle count is reported.  Without mpiexec: one rank, 2 cycles, scaled to
+    the 10 cycles the 1-rank reference needs.  Without the reference binary:
+    the oracle (port) at 129^3, scaled."""
     ref = os.path.join(ROOT, "oracle", "_ref", "pgmg_ref")
+    env = dict(os.environ, LD_LIBRARY_PATH="/opt/conda/lib")
+    mpiexec = "/opt/conda/bin/mpiexec"
+    if os.path.exists(ref) and os.path.exists(mpiexec):
+        ranks = max(1, min(16, os.cpu_count() or 1))
+        try:
+            p = subprocess.run([mpiexec, "-n", str(ranks), ref, "time", "1", str(SEED), str(N), str(N), str(N), "gs24",
+                                str(TOL), "100"], capture_output=True, text=True, timeout=300, env=env,
+                               stdin=subprocess.DEVNULL)
+            lines = [l for l in p.stdout.splitlines() if l.startswith("{")]
+            info = json.loads(lines[-1])
+            return {"value": round(info["solve_s"], 3), "unit": "s", "cores": ranks, "kind": "reference",
+                    "sample": "reference (oracle/_ref) on %d MPI ranks (mpiexec): the full %d^3 solve to 1e-10 "
+                              "in %d cycles, rel %.2e; set-up %.2f s excluded"
+                              % (ranks, N, info["cycles"], info["rel"], info["setup_s"])}
+        except Exception as e:  # pragma: no cover - reported, not fatal
+            log("reference baseline over mpiexec failed:", e)
     if os.path.exists(ref):
-        env = dict(os.environ, LD_LIBRARY_PATH="/opt/conda/lib")
         try:
             p = subprocess.run([ref, "time", "1", str(SEED), str(N), str(N), str(N), "gs24", "1e-30", "2"],
-                               capture_output=True, text=True, timeout=240, env=env)
-            info = json.loads(p.stdout.strip().splitlines()[-1])
+                               capture_output=True, text=True, timeout=240, env=env, stdin=subprocess.DEVNULL)
+            info = json.loads([l for l in p.stdout.splitlines() if l.startswith("{")][-1])
             per_cycle = info["solve_s"] / info["cycles"]
             return {"value": round(per_cycle * 10, 3), "unit": "s", "cores": 1, "kind": "reference",
                     "sample": "reference (oracle/_ref) 2 V-cycles of the 257^3 problem on 1 MPI rank: "

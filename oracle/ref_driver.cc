@@ -238,7 +238,10 @@ int main(int argc, char **argv) {
 		int maxc = argc > 9 ? atoi(argv[9]) : 100;
 		if (bottom == "lu") enable_lu(mg);
 		if (cmd == "solve" && argc > 10 && strcmp(argv[10], "-") != 0) outf = fopen(argv[10], "wb");
+		// rhs_l2norm is the global sum on rank 0 only (region.cc:1549-1558);
+		// every rank needs it to take the same stop decision
 		double bn = mg.rhs_l2norm();
+		MPI_Bcast(&bn, 1, MPI_DOUBLE, 0, MPI_COMM_WORLD);
 		std::vector<double> res;
 		int gs0 = region::gs_num;
 		double ts = MPI_Wtime();
@@ -260,11 +263,16 @@ int main(int argc, char **argv) {
 			put_d("rhs_l2norm", std::vector<double>(1, bn));
 			fclose(outf);
 		}
-		printf("{\"cycles\": %d, \"rel\": %.17g, \"solve_s\": %.6f, \"setup_s\": %.6f, "
-		       "\"levels\": %d, \"gs_sweeps_per_cycle\": %d, \"lu\": %d}\n",
-		       c, rel, tsolve, t_setup, mg.nlevels, c ? (region::gs_num - gs0) / c : 0,
-		       mg.reg[mg.nlevels - 1]->Aexact != NULL);
-		fflush(stdout);
+		int rank = 0, size = 1;
+		MPI_Comm_rank(MPI_COMM_WORLD, &rank);
+		MPI_Comm_size(MPI_COMM_WORLD, &size);
+		if (rank == 0) {
+			printf("{\"cycles\": %d, \"rel\": %.17g, \"solve_s\": %.6f, \"setup_s\": %.6f, "
+			       "\"levels\": %d, \"gs_sweeps_per_cycle\": %d, \"lu\": %d, \"ranks\": %d}\n",
+			       c, rel, tsolve, t_setup, mg.nlevels, c ? (region::gs_num - gs0) / c : 0,
+			       mg.reg[mg.nlevels - 1]->Aexact != NULL, size);
+			fflush(stdout);
+		}
 	}
 	mg.free();
 	gm.free();
