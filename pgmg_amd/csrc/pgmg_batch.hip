@@ -143,6 +143,7 @@ WorkList wlist(pgmg_batch *b, int li) {
 	w.slot = b->d_plan + (size_t)li * b->list_stride;
 	w.prefix = w.slot + b->max_slots;
 	w.n = b->list_n[li];
+	w.total = b->h_plan[(size_t)li * b->list_stride + b->max_slots + w.n];
 	return w;
 }
 
@@ -156,6 +157,7 @@ void plan_add(pgmg_batch *b, int li, int slot, int units) {
 }
 
 int build_plan(pgmg_batch *b) {
+	const int el = b->mixed ? 4 : 8;  // the cycle's arithmetic
 	std::fill(b->list_n.begin(), b->list_n.end(), 0);
 	for (int li = 0; li < n_lists(); li++) b->h_plan[(size_t)li * b->list_stride + b->max_slots] = 0;
 	b->max_nl = 0;
@@ -164,9 +166,9 @@ int build_plan(pgmg_batch *b) {
 		if (S.state != ST_ACTIVE) continue;
 		const SlotDev &D = S.dev;
 		b->max_nl = std::max(b->max_nl, S.nl);
-		plan_add(b, L_TOP_ROWS, s, batch_units(BK_ROWS, D.lv[0]));
+		plan_add(b, L_TOP_ROWS, s, batch_units(BK_ROWS, D.lv[0], 8));
 		plan_add(b, L_TOP_ONE, s, 1);
-		plan_add(b, L_TOP_PTS, s, batch_units(BK_PTS, D.lv[0]));
+		plan_add(b, L_TOP_PTS, s, batch_units(BK_PTS, D.lv[0], 8));
 		for (int l = 0; l < S.nl; l++) {
 			const LevelView &L = D.lv[l];
 			if (l < S.nl - 1) {
@@ -174,10 +176,10 @@ int build_plan(pgmg_batch *b) {
 				if (small_lds_bytes(L, b->mixed ? 4 : 8) <= kSmallLdsBytes &&
 				    ((long long)L.m * L.n * L.o <= 1024 || small_pairs_per_pass(L) <= 1024))
 					plan_add(b, list_index(l, L_SMALL), s, 1);
-				else for (int q = 0; q < 4; q++) plan_add(b, list_index(l, L_PAIR0 + q), s, batch_units(BK_PAIR0 + q, L));
-				plan_add(b, list_index(l, L_RES), s, batch_units(BK_ROWS, L));
-				plan_add(b, list_index(l + 1, L_RESTRICT), s, batch_units(BK_PTS, D.lv[l + 1]));
-				plan_add(b, list_index(l + 1, L_INTERP), s, batch_units(BK_CUBES, L));
+				else for (int q = 0; q < 4; q++) plan_add(b, list_index(l, L_PAIR0 + q), s, batch_units(BK_PAIR0 + q, L, el));
+				plan_add(b, list_index(l, L_RES), s, batch_units(BK_ROWS, L, el));
+				plan_add(b, list_index(l + 1, L_RESTRICT), s, batch_units(BK_PTS, D.lv[l + 1], el));
+				plan_add(b, list_index(l + 1, L_INTERP), s, batch_units(BK_CUBES, L, el));
 			} else if (D.lu_active) {
 				plan_add(b, list_index(l, L_LU), s, 1);
 			} else {
@@ -224,7 +226,7 @@ uint64_t slot_bytes(const pgmg_batch *b, int nl, const int *dims) {
 	}
 	const LevelView top = make_level_view(dims[0], dims[1], dims[2]);
 	t += b->mixed ? al(top.npad * 4) : al(top.npad * 8);
-	t += al(sizeof(double) * (uint64_t)((top.n * top.o + 3) / 4 + 64));
+	t += al(sizeof(double) * (uint64_t)(residual_blocks(top, 8) + 64));
 	if (b->opts.bottom == PGMG_BOTTOM_LU) {
 		const uint64_t N = (uint64_t)dims[3 * (nl - 1)] * dims[3 * (nl - 1) + 1] * dims[3 * (nl - 1) + 2];
 		t += al(N * N * 8) + al(N * 4);
@@ -255,7 +257,7 @@ int slot_setup(pgmg_batch *b, int s, const pgmg_dims *d, int family, uint64_t se
 	}
 	const uint64_t np0 = (uint64_t)D.lv[0].npad;
 	if (b->mixed) D.res32 = (float *)take(np0 * 4); else D.res64 = (double *)take(np0 * 8);
-	D.partial = (double *)take(sizeof(double) * (uint64_t)((D.lv[0].n * D.lv[0].o + 3) / 4 + 64));
+	D.partial = (double *)take(sizeof(double) * (uint64_t)(residual_blocks(D.lv[0], 8) + 64));
 	const int bl = S.nl - 1;
 	const long long N = (long long)D.lv[bl].m * D.lv[bl].n * D.lv[bl].o;
 	if (S.want_lu) {

@@ -53,6 +53,39 @@ __host__ __device__ inline long long aidx3(const LevelView &L, int e, int i, int
 	return arow0(L, j, k) + (long long)e * L.px + i;
 }
 
+// Lane layout of the row kernels (smoother and residual).  A lane holds
+// ppl consecutive points of a grid row: 4 in FP32 (16-byte loads), 2 in
+// FP64.  A row of m = ppl*k + 1 points (the 2^k+1 sizes of the hierarchy)
+// takes k lanes, the last of which also does the point out (ext); other rows
+// take ceil(m/ppl) lanes.  When a row needs at most 64 lanes, one wavefront
+// packs rpw = 64/lpr rows; longer rows are streamed in segments by one
+// wavefront (rpw = 1, lpr > 64).
+struct RowLanes {
+	int ppl, lpr, rpw;
+	bool ext;
+};
+__host__ __device__ inline RowLanes row_lanes(int m, int elem) {
+	RowLanes w;
+	w.ppl = elem == 4 ? 4 : 2;
+	w.ext = m > w.ppl && (m - 1) % w.ppl == 0;
+	w.lpr = w.ext ? (m - 1) / w.ppl : (m + w.ppl - 1) / w.ppl;
+	w.rpw = w.lpr <= 64 ? 64 / w.lpr : 1;
+	return w;
+}
+// Wavefronts that cover `rows` grid rows of m points.
+__host__ __device__ inline int row_waves(int rows, int m, int elem) {
+	const RowLanes w = row_lanes(m, elem);
+	return (rows + w.rpw - 1) / w.rpw;
+}
+
+// Workgroups (4 wavefronts each) of a residual launch over all rows of L in
+// arithmetic of elem bytes; an FP64 residual has one partial sum per block.
+__host__ __device__ inline int residual_blocks(const LevelView &L, int elem) {
+	const int rows = L.n * L.o;
+	const int waves = row_lanes(L.m, elem).lpr <= 64 ? row_waves(rows, L.m, elem) : rows;
+	return (waves + 3) / 4;
+}
+
 // ---- batches of independent hierarchies (one per slot) ----
 const int kBatchMaxLevels = 16;
 // Device-side description of one slot's hierarchy.  Pointers of a precision
@@ -76,13 +109,15 @@ struct WorkList {
 	const int *slot;
 	const int *prefix;
 	int n;
+	int total;  // prefix[n], known to the host: the launch's grid size
 };
 // Unit kinds: the colour-pair row groups of a level, one unit per slot,
 // residual row groups, 64x4-point blocks of a level, 64x4 blocks of its
 // 2x2x2 cubes.
 enum { BK_PAIR0 = 0, BK_PAIR1, BK_PAIR2, BK_PAIR3, BK_ONE, BK_ROWS, BK_PTS, BK_CUBES };
-// Work units of one slot for a batched kernel kind at a level.
-int batch_units(int kind, const LevelView &L);
+// Work units of one slot for a batched kernel kind at a level; elem is the
+// size of the cycle's arithmetic type (4 or 8).
+int batch_units(int kind, const LevelView &L, int elem);
 
 // Batched launchers: one launch covers the level `level` of every slot in
 // the list; blocks walk the units grid-stride.
@@ -133,6 +168,7 @@ template <typename T> void launch_gs_small(hipStream_t s, const LevelView &L, co
 // of partial sums.
 template <typename T, typename TR> int launch_residual(hipStream_t s, const LevelView &L, const T *A, const T *x,
 						       const T *r, TR *res, double *partial, TR *zero_out = nullptr);
+// Partial sums (workgroups) of an FP64 residual launch over level L.
 int residual_partial_count(const LevelView &L);
 // out[0] = sum of partial[0..n) in a fixed order.
 void launch_reduce(hipStream_t s, const double *partial, int n, double *out);

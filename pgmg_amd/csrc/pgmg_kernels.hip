@@ -348,6 +348,263 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 	}
 }
 
+// Vectors of ppl elements moved with one load or store.
+template <typename T, int N> struct VecOf;
+template <> struct VecOf<float, 4> { typedef float4 type; };
+template <> struct VecOf<float, 2> { typedef float2 type; };
+template <> struct VecOf<double, 2> { typedef double2 type; };
+
+template <typename T, int N>
+__device__ __forceinline__ void ld_vec(const T *base, uint32_t off, T (&out)[N]) {
+	typedef typename VecOf<T, N>::type V;
+	const V v = ld_row<V>(base, off);
+	const T *pv = reinterpret_cast<const T *>(&v);
+#pragma unroll
+	for (int t = 0; t < N; t++) out[t] = pv[t];
+}
+// Coefficients and r: read once per launch (non-temporal in FP64).
+template <typename T, int N>
+__device__ __forceinline__ void ld_vec_stream(const T *base, uint32_t off, T (&out)[N]) {
+	typedef typename VecOf<T, N>::type V;
+	const V v = ld_stream<V>(base, off);
+	const T *pv = reinterpret_cast<const T *>(&v);
+#pragma unroll
+	for (int t = 0; t < N; t++) out[t] = pv[t];
+}
+template <> __device__ __forceinline__ float4 ld_stream<float4>(const void *base, uint32_t off) {
+	return *(const float4 *)((const char *)base + off);
+}
+template <typename T, int N>
+__device__ __forceinline__ void st_vec(T *base, uint32_t off, const T (&v)[N]) {
+	typedef typename VecOf<T, N>::type V;
+	V o;
+	T *po = reinterpret_cast<T *>(&o);
+#pragma unroll
+	for (int t = 0; t < N; t++) po[t] = v[t];
+	st_row<V>(base, off, o);
+}
+
+// Forward colour-pair update of rows that fit one wavefront's lanes
+// (row_lanes: rpw rows per wavefront, lpr lanes per row, ppl points per
+// lane, the last lane of a 2^k+1 row also doing the point out).  The
+// wavefront owns rows q0 .. q0+rpw-1 of the launch's colour pair; each array
+// is addressed from row q0's start (scalar registers) plus a per-lane 32-bit
+// offset.  Even points update from old neighbours; odd points then take
+// their own lane's new evens and the next lane's first even (shuffle), or
+// at a row's last lane the point out (or the halo).  Only the wavefront's
+// rows are written by the launch, so each lane stores its points once, at
+// the end.  Per-point arithmetic and order as in gs_point.
+template <typename T, int PPL>
+__device__ __forceinline__ void gs_lanes(const LevelView &L, const T *__restrict__ A, T *__restrict__ x,
+					 const T *__restrict__ r, int cj, int ck, int q0, int nrows) {
+	const RowLanes W = row_lanes(L.m, (int)sizeof(T));
+	const int lane = threadIdx.x;
+	const int rr = lane / W.lpr, li = lane - rr * W.lpr;
+	const int q = q0 + rr;
+	const bool live = rr < W.rpw && q < nrows;
+	const int nb = (L.n - cj + 1) >> 1;
+	const int j0 = 2 * (q0 % nb) + cj, k0 = 2 * (q0 / nb) + ck;
+	const long long xb0 = idx3(L, 0, j0, k0), ab0 = arow0(L, j0, k0);
+	const long long px = L.px, pxy = L.pxy;
+	const T *Ab = A + ab0;
+	const T *rb = r + xb0;
+	T *xb = x + xb0;
+	int i0 = 0;
+	uint32_t ox = 0, oa = 0;
+	if (live) {
+		const int j = 2 * (q % nb) + cj, k = 2 * (q / nb) + ck;
+		i0 = PPL * li;
+		ox = (uint32_t)((idx3(L, 0, j, k) - xb0 + i0) * (long long)sizeof(T));
+		oa = (uint32_t)((arow0(L, j, k) - ab0 + i0) * (long long)sizeof(T));
+	}
+	const bool ext = live && W.ext && li == W.lpr - 1;  // even point i0+PPL = m-1
+	const bool lastlane = li == W.lpr - 1;
+	const bool needxp = live && i0 + PPL <= L.m;
+	const uint32_t oe = (uint32_t)(PPL * sizeof(T));
+	T s[PPL], cc[PPL], wm[PPL], wp[PPL], rv[PPL];
+	T prod[PPL / 2][12];
+	T sx = (T)0, cx = (T)1, rx = (T)0;
+#pragma unroll
+	for (int t = 0; t < PPL; t++) { s[t] = (T)0; cc[t] = (T)1; wm[t] = (T)0; wp[t] = (T)0; rv[t] = (T)0; }
+	if (live) {
+		ld_vec_stream<T, PPL>(rb, ox, rv);
+		if (ext) rx = ld_row<T>(rb, ox + oe);
+#pragma unroll
+		for (int rq = 0; rq < 9; rq++) {
+			const int dj = rq % 3 - 1, dk = rq / 3 - 1;
+			const T *xq = xb + dj * px + dk * pxy;
+			T a[3][PPL], xv[PPL];
+#pragma unroll
+			for (int d = 0; d < 3; d++) ld_vec_stream<T, PPL>(Ab + (3 * rq + d) * px, oa, a[d]);
+			ld_vec<T, PPL>(xq, ox, xv);
+			const T xm = ld_row<T>(xq - 1, ox);
+			const T xp = needxp ? ld_row<T>(xq + PPL, ox) : (T)0;
+			T b[3] = {(T)0, (T)0, (T)0};
+			if (ext) {
+#pragma unroll
+				for (int d = 0; d < 3; d++) b[d] = ld_row<T>(Ab + (3 * rq + d) * px, oa + oe);
+			}
+#pragma unroll
+			for (int t = 0; t < PPL; t++) {
+				const T vm = t == 0 ? xm : xv[t - 1 < 0 ? 0 : t - 1];
+				const T v0 = xv[t];
+				const T vp = t == PPL - 1 ? xp : xv[t + 1 < PPL ? t + 1 : PPL - 1];
+				if ((t & 1) == 0) {
+					if (rq == 4) {
+						s[t] = s[t] + a[0][t] * vm;
+						s[t] = s[t] + a[2][t] * vp;
+						cc[t] = a[1][t];
+					} else {
+						s[t] = s[t] + a[0][t] * vm;
+						s[t] = s[t] + a[1][t] * v0;
+						s[t] = s[t] + a[2][t] * vp;
+					}
+				} else if (rq < 4) {
+					s[t] = s[t] + a[0][t] * vm;
+					s[t] = s[t] + a[1][t] * v0;
+					s[t] = s[t] + a[2][t] * vp;
+				} else if (rq == 4) {
+					wm[t] = a[0][t]; cc[t] = a[1][t]; wp[t] = a[2][t];
+				} else {
+					prod[t / 2][3 * (rq - 5)] = a[0][t] * vm;
+					prod[t / 2][3 * (rq - 5) + 1] = a[1][t] * v0;
+					prod[t / 2][3 * (rq - 5) + 2] = a[2][t] * vp;
+				}
+			}
+			// the point out: neighbours i0+PPL-1, itself, the halo i = m
+			if (rq == 4) {
+				sx = sx + b[0] * xv[PPL - 1];
+				sx = sx + b[2] * (T)0;
+				cx = b[1];
+			} else {
+				sx = sx + b[0] * xv[PPL - 1];
+				sx = sx + b[1] * xp;
+				sx = sx + b[2] * (T)0;
+			}
+		}
+	}
+	T xn[PPL];
+#pragma unroll
+	for (int t = 0; t < PPL; t++) xn[t] = (T)0;
+#pragma unroll
+	for (int t = 0; t < PPL; t += 2)
+		if (live && i0 + t < L.m) xn[t] = (rv[t] - s[t]) * ((T)1 / cc[t]);
+	T xe = (T)0;
+	if (ext) xe = (rx - sx) * ((T)1 / cx);
+	// shuffles with every lane active
+	const T nxt = __shfl_down(xn[0], 1, 64);
+	const T right = lastlane ? xe : nxt;
+#pragma unroll
+	for (int t = 1; t < PPL; t += 2) {
+		if (live && i0 + t < L.m) {
+			const T rt = t + 1 < PPL ? xn[t + 1 < PPL ? t + 1 : 0] : right;
+			T acc = s[t];
+			acc = acc + wm[t] * xn[t - 1];
+			acc = acc + wp[t] * rt;
+#pragma unroll
+			for (int q2 = 0; q2 < 12; q2++) acc = acc + prod[t / 2][q2];
+			xn[t] = (rv[t] - acc) * ((T)1 / cc[t]);
+		}
+	}
+	if (live) {
+		if (i0 + PPL <= L.m) {
+			st_vec<T, PPL>(xb, ox, xn);
+		} else {
+#pragma unroll
+			for (int t = 0; t < PPL; t++)
+				if (i0 + t < L.m) st_row<T>(xb, ox + (uint32_t)(t * sizeof(T)), xn[t]);
+		}
+		if (ext) st_row<T>(xb, ox + oe, xe);
+	}
+}
+
+// Residual of rows that fit one wavefront's lanes (layout as gs_lanes);
+// returns this lane's sum of squares.  zx (may be null) is set to 0 at the
+// same points.
+template <typename T, typename TR, int PPL>
+__device__ __forceinline__ double residual_lanes(const LevelView &L, const T *__restrict__ A, const T *__restrict__ x,
+						 const T *__restrict__ r, TR *__restrict__ res, TR *__restrict__ zx,
+						 int q0, int nrows) {
+	const RowLanes W = row_lanes(L.m, (int)sizeof(T));
+	const int lane = threadIdx.x;
+	const int rr = lane / W.lpr, li = lane - rr * W.lpr;
+	const int q = q0 + rr;
+	const bool live = rr < W.rpw && q < nrows;
+	const int j0 = q0 % L.n, k0 = q0 / L.n;
+	const long long xb0 = idx3(L, 0, j0, k0), ab0 = arow0(L, j0, k0);
+	const long long px = L.px, pxy = L.pxy;
+	double v2 = 0.0;
+	if (!live) return v2;
+	const int j = q % L.n, k = q / L.n;
+	const int i0 = PPL * li;
+	const long long dx = idx3(L, 0, j, k) - xb0 + i0;
+	const uint32_t ox = (uint32_t)(dx * (long long)sizeof(T));
+	const uint32_t oa = (uint32_t)((arow0(L, j, k) - ab0 + i0) * (long long)sizeof(T));
+	const bool ext = W.ext && li == W.lpr - 1;
+	const bool needxp = i0 + PPL <= L.m;
+	const uint32_t oe = (uint32_t)(PPL * sizeof(T));
+	const T *Ab = A + ab0, *rb = r + xb0, *xb = x + xb0;
+	T s[PPL], rv[PPL];
+	T sx = (T)0, rx = (T)0;
+#pragma unroll
+	for (int t = 0; t < PPL; t++) s[t] = (T)0;
+	ld_vec_stream<T, PPL>(rb, ox, rv);
+	if (ext) rx = ld_row<T>(rb, ox + oe);
+#pragma unroll
+	for (int rq = 0; rq < 9; rq++) {
+		const int dj = rq % 3 - 1, dk = rq / 3 - 1;
+		const T *xq = xb + dj * px + dk * pxy;
+		T a[3][PPL], xv[PPL];
+#pragma unroll
+		for (int d = 0; d < 3; d++) ld_vec_stream<T, PPL>(Ab + (3 * rq + d) * px, oa, a[d]);
+		ld_vec<T, PPL>(xq, ox, xv);
+		const T xm = ld_row<T>(xq - 1, ox);
+		const T xp = needxp ? ld_row<T>(xq + PPL, ox) : (T)0;
+#pragma unroll
+		for (int t = 0; t < PPL; t++) {
+			const T vm = t == 0 ? xm : xv[t - 1 < 0 ? 0 : t - 1];
+			const T vp = t == PPL - 1 ? xp : xv[t + 1 < PPL ? t + 1 : PPL - 1];
+			s[t] = s[t] + a[0][t] * vm;
+			s[t] = s[t] + a[1][t] * xv[t];
+			s[t] = s[t] + a[2][t] * vp;
+		}
+		if (ext) {
+			sx = sx + ld_row<T>(Ab + (3 * rq) * px, oa + oe) * xv[PPL - 1];
+			sx = sx + ld_row<T>(Ab + (3 * rq + 1) * px, oa + oe) * xp;
+			sx = sx + ld_row<T>(Ab + (3 * rq + 2) * px, oa + oe) * (T)0;  // the halo point i = m
+		}
+	}
+	TR o[PPL], z[PPL];
+#pragma unroll
+	for (int t = 0; t < PPL; t++) {
+		const T v = rv[t] - s[t];
+		o[t] = (TR)v;
+		z[t] = (TR)0;
+		if (i0 + t < L.m) v2 = v2 + (double)v * (double)v;
+	}
+	const T vx = rx - sx;
+	if (ext) v2 = v2 + (double)vx * (double)vx;
+	const uint32_t orr = (uint32_t)(dx * (long long)sizeof(TR)), oer = (uint32_t)(PPL * sizeof(TR));
+	TR *resb = res ? res + xb0 : nullptr;
+	TR *zxb = zx ? zx + xb0 : nullptr;
+	if (i0 + PPL <= L.m) {
+		if (resb) st_vec<TR, PPL>(resb, orr, o);
+		if (zxb) st_vec<TR, PPL>(zxb, orr, z);
+	} else {
+#pragma unroll
+		for (int t = 0; t < PPL; t++)
+			if (i0 + t < L.m) {
+				if (resb) st_row<TR>(resb, orr + (uint32_t)(t * sizeof(TR)), o[t]);
+				if (zxb) st_row<TR>(zxb, orr + (uint32_t)(t * sizeof(TR)), (TR)0);
+			}
+	}
+	if (ext) {
+		if (resb) st_row<TR>(resb, orr + oer, (TR)vx);
+		if (zxb) st_row<TR>(zxb, orr + oer, (TR)0);
+	}
+	return v2;
+}
+
 // Forward colour-pair update of one row: four points per lane in FP32,
 // two in FP64 (whose 16-byte pair loads already fill the registers).
 template <typename T>
@@ -364,23 +621,52 @@ __device__ __forceinline__ void gs_row<double>(const LevelView &L, const double 
 	gs_pair_row<double>(L, A, x, r, j, k);
 }
 
+template <typename T> struct Ppl { static const int v = sizeof(T) == 4 ? 4 : 2; };
+
+// Wavefront w of a colour-pair launch: rows w*rpw .. of the pair when they
+// fit the lanes, else the single long row w, streamed in segments.  w must be
+// wavefront-uniform.
+template <typename T>
+__device__ __forceinline__ void gs_wave(const LevelView &L, const T *__restrict__ A, T *__restrict__ x,
+					const T *__restrict__ r, int cj, int ck, int w) {
+	const int nb = (L.n - cj + 1) >> 1, nd = (L.o - ck + 1) >> 1, rows = nb * nd;
+	const RowLanes W = row_lanes(L.m, (int)sizeof(T));
+	if (W.lpr <= 64) {
+		const int q0 = w * W.rpw;
+		if (q0 < rows) gs_lanes<T, Ppl<T>::v>(L, A, x, r, cj, ck, q0, rows);
+	} else if (w < rows) {
+		gs_row<T>(L, A, x, r, 2 * (w % nb) + cj, 2 * (w / nb) + ck);
+	}
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void k_gs_pair(LevelView L, const T *__restrict__ A, T *__restrict__ x,
 						 const T *__restrict__ r, int cj, int ck) {
-	const int nb = (L.n - cj + 1) >> 1, nd = (L.o - ck + 1) >> 1;
-	// one row per wavefront: the row index is wavefront-uniform
-	const int q = __builtin_amdgcn_readfirstlane(xcd_block((int)blockIdx.x, (int)gridDim.x) * 4 + (int)threadIdx.y);
-	if (q >= nb * nd) return;
-	gs_row<T>(L, A, x, r, 2 * (q % nb) + cj, 2 * (q / nb) + ck);
+	const int w = __builtin_amdgcn_readfirstlane(xcd_block((int)blockIdx.x, (int)gridDim.x) * (int)blockDim.y + (int)threadIdx.y);
+	gs_wave<T>(L, A, x, r, cj, ck, w);
 }
+
+// Wavefronts of one colour-pair launch.
+template <typename T>
+__host__ __device__ inline int gs_pair_waves(const LevelView &L, int pair) {
+	const int cj = pair & 1, ck = (pair >> 1) & 1;
+	const int rows = ((L.n - cj + 1) >> 1) * ((L.o - ck + 1) >> 1);
+	return row_lanes(L.m, (int)sizeof(T)).lpr <= 64 ? row_waves(rows, L.m, (int)sizeof(T)) : rows;
+}
+
+// Wavefronts per workgroup of the row smoother.
+#ifndef PGMG_GS_ROWS
+#define PGMG_GS_ROWS 1
+#endif
+static const int kGsRows = PGMG_GS_ROWS;
 
 template <typename T>
 void launch_gs_pair(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int pair) {
 	const int cj = pair & 1, ck = (pair >> 1) & 1;
 	const int nb = (L.n - cj + 1) >> 1, nd = (L.o - ck + 1) >> 1;
 	if (nb <= 0 || nd <= 0) return;
-	const int rows = nb * nd;
-	hipLaunchKernelGGL(k_gs_pair<T>, dim3((rows + 3) / 4), dim3(64, 4, 1), 0, s, L, A, x, r, cj, ck);
+	const int waves = gs_pair_waves<T>(L, pair);
+	hipLaunchKernelGGL(k_gs_pair<T>, dim3((waves + kGsRows - 1) / kGsRows), dim3(64, kGsRows, 1), 0, s, L, A, x, r, cj, ck);
 	HIPCHECK_LAUNCH();
 }
 
@@ -548,7 +834,7 @@ __device__ __forceinline__ void gs_lds_run(const LevelView &L, const T *__restri
 	int oc = -1, ot = -1, olq = 0;
 	T ca[27], cr = (T)0;
 	if (resident && tid < npts) {
-		int i, j, k;
+		int i = 0, j = 0, k = 0;
 		if (exact) {
 			i = tid % L.m; j = (tid / L.m) % L.n; k = tid / (L.m * L.n);
 		} else {
@@ -847,27 +1133,42 @@ __device__ __forceinline__ double residual_row<float, float>(const LevelView &L,
 	return residual_row_quad_f32(L, A, x, r, res, q);  // zx: not used in FP32
 }
 
+// Wavefront w of a residual launch: rows w*rpw .. when they fit the lanes,
+// else the single long row w.  w must be wavefront-uniform.
+template <typename T, typename TR>
+__device__ __forceinline__ double residual_wave(const LevelView &L, const T *__restrict__ A, const T *__restrict__ x,
+						const T *__restrict__ r, TR *__restrict__ res, TR *__restrict__ zx, int w) {
+	const int rows = L.n * L.o;
+	const RowLanes W = row_lanes(L.m, (int)sizeof(T));
+	if (W.lpr <= 64) {
+		const int q0 = w * W.rpw;
+		return q0 < rows ? residual_lanes<T, TR, Ppl<T>::v>(L, A, x, r, res, zx, q0, rows) : 0.0;
+	}
+	return residual_row<T, TR>(L, A, x, r, res, w, zx);
+}
+
 template <typename T, typename TR>
 __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void k_residual(LevelView L, const T *__restrict__ A, const T *__restrict__ x,
 						  const T *__restrict__ r, TR *__restrict__ res, double *__restrict__ partial,
 						  TR *__restrict__ zx) {
 	const int qb = xcd_block((int)blockIdx.x, (int)gridDim.x);
-	const double v2 = residual_row<T, TR>(L, A, x, r, res, __builtin_amdgcn_readfirstlane(qb * 4 + (int)threadIdx.y), zx);
+	const double v2 = residual_wave<T, TR>(L, A, x, r, res, zx, __builtin_amdgcn_readfirstlane(qb * 4 + (int)threadIdx.y));
 	if (partial) {
-		// indexed by row group, so the sum order is that of a batched launch
+		// indexed by the block's logical number, so the sum order is that
+		// of a batched launch
 		const double sblk = block_sum_256(v2);
 		if (threadIdx.x == 0 && threadIdx.y == 0) partial[qb] = sblk;
 	}
 }
 
 int residual_partial_count(const LevelView &L) {
-	return (L.n * L.o + 3) / 4;
+	return residual_blocks(L, 8);
 }
 
 template <typename T, typename TR>
 int launch_residual(hipStream_t s, const LevelView &L, const T *A, const T *x, const T *r, TR *res, double *partial,
 		    TR *zero_out) {
-	const int G = residual_partial_count(L);
+	const int G = residual_blocks(L, (int)sizeof(T));
 	hipLaunchKernelGGL((k_residual<T, TR>), dim3(G), dim3(64, 4, 1), 0, s, L, A, x, r, res, partial, zero_out);
 	HIPCHECK_LAUNCH();
 	return G;
@@ -1514,15 +1815,14 @@ void launch_add_f32_to_f64(hipStream_t s, const LevelView &L, const float *e, do
 // are those of the single-hierarchy kernels, so a slot's arithmetic does not
 // depend on which other slots share its launches.
 
-int batch_units(int kind, const LevelView &L) {
+int batch_units(int kind, const LevelView &L, int elem) {
 	switch (kind) {
 	case BK_PAIR0: case BK_PAIR1: case BK_PAIR2: case BK_PAIR3: {
-		const int cj = kind & 1, ck = (kind >> 1) & 1;
-		const int rows = ((L.n - cj + 1) >> 1) * ((L.o - ck + 1) >> 1);
-		return (rows + 3) / 4;
+		const int waves = elem == 4 ? gs_pair_waves<float>(L, kind - BK_PAIR0) : gs_pair_waves<double>(L, kind - BK_PAIR0);
+		return (waves + 3) / 4;
 	}
 	case BK_ONE: return 1;
-	case BK_ROWS: return (L.n * L.o + 3) / 4;
+	case BK_ROWS: return residual_blocks(L, elem);
 	case BK_CUBES: return (((L.m + 1) / 2 + 63) / 64) * (((L.n + 1) / 2 + 3) / 4) * ((L.o + 1) / 2);
 	default: return ((L.m + 63) / 64) * ((L.n + 3) / 4) * L.o;
 	}
@@ -1569,11 +1869,18 @@ __device__ __forceinline__ bool cubes_unit(const LevelView &L, int lu, int &ic, 
 	return ic < ncx && jc < ncy;
 }
 
-#define BATCH_LOOP                                                   \
-	const int total_ = wl.prefix[wl.n];                          \
-	for (int u = (int)blockIdx.x; u < total_; u += (int)gridDim.x) { \
-		const int q_ = __builtin_amdgcn_readfirstlane(wl_find(wl, u)); \
-		const SlotDev *S = slots + wl.slot[q_];              \
+// Each workgroup takes one contiguous chunk of the list's units (chunks
+// numbered per XCD, so neighbouring units share an L2), finds the first
+// unit's slot by bisection and then walks forward through the slots.
+#define BATCH_LOOP                                                                         \
+	const int total_ = wl.prefix[wl.n];                                                \
+	const int per_ = (total_ + (int)gridDim.x - 1) / (int)gridDim.x;                   \
+	const int u0_ = xcd_block((int)blockIdx.x, (int)gridDim.x) * per_;                 \
+	const int u1_ = min(total_, u0_ + per_);                                           \
+	int q_ = u0_ < u1_ ? __builtin_amdgcn_readfirstlane(wl_find(wl, u0_)) : 0;         \
+	for (int u = u0_; u < u1_; u++) {                                                  \
+		while (u >= wl.prefix[q_ + 1]) q_++;                                       \
+		const SlotDev *S = slots + wl.slot[q_];                                    \
 		const int lu = u - wl.prefix[q_];
 
 template <typename T>
@@ -1581,10 +1888,8 @@ __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void kb_gs_pair(const SlotDev
 								   int level, int cj, int ck) {
 	BATCH_LOOP
 		const LevelView L = S->lv[level];
-		const int nb = (L.n - cj + 1) >> 1, nd = (L.o - ck + 1) >> 1;
-		const int row = __builtin_amdgcn_readfirstlane(lu * 4 + (int)threadIdx.y);
-		if (row < nb * nd)
-			gs_row<T>(L, sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), 2 * (row % nb) + cj, 2 * (row / nb) + ck);
+		gs_wave<T>(L, sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), cj, ck,
+			   __builtin_amdgcn_readfirstlane(lu * 4 + (int)threadIdx.y));
 	}
 }
 
@@ -1606,7 +1911,8 @@ template <typename T>
 __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void kb_residual(const SlotDev *__restrict__ slots, WorkList wl, int level) {
 	BATCH_LOOP
 		const LevelView L = S->lv[level];
-		(void)residual_row<T, T>(L, sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), sres<T>(S), __builtin_amdgcn_readfirstlane(lu * 4 + (int)threadIdx.y));
+		(void)residual_wave<T, T>(L, sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), sres<T>(S), (T *)nullptr,
+					  __builtin_amdgcn_readfirstlane(lu * 4 + (int)threadIdx.y));
 	}
 }
 
@@ -1615,9 +1921,9 @@ __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void kb_residual(const SlotDe
 __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void kb_residual_top(const SlotDev *__restrict__ slots, WorkList wl, int to_f32) {
 	BATCH_LOOP
 		const LevelView L = S->lv[0];
-		const int q = __builtin_amdgcn_readfirstlane(lu * 4 + (int)threadIdx.y);
-		const double v2 = to_f32 ? residual_row<double, float>(L, S->A[0], S->x[0], S->r[0], S->r32[0], q, S->x32[0])
-					 : residual_row<double, double>(L, S->A[0], S->x[0], S->r[0], (double *)nullptr, q);
+		const int w = __builtin_amdgcn_readfirstlane(lu * 4 + (int)threadIdx.y);
+		const double v2 = to_f32 ? residual_wave<double, float>(L, S->A[0], S->x[0], S->r[0], S->r32[0], S->x32[0], w)
+					 : residual_wave<double, double>(L, S->A[0], S->x[0], S->r[0], (double *)nullptr, (double *)nullptr, w);
 		const double sblk = block_sum_256(v2);
 		if (threadIdx.x == 0 && threadIdx.y == 0) S->partial[lu] = sblk;
 		__syncthreads();
@@ -1629,7 +1935,7 @@ __global__ __launch_bounds__(1024) void kb_reduce(const SlotDev *__restrict__ sl
 	BATCH_LOOP
 		(void)lu;
 		const LevelView L = S->lv[0];
-		const int n = (L.n * L.o + 3) / 4;
+		const int n = residual_blocks(L, 8);
 		double s = 0.0;
 		for (int t = threadIdx.x; t < n; t += 1024) s = s + S->partial[t];
 		red[threadIdx.x] = s;
@@ -1683,58 +1989,59 @@ __global__ __launch_bounds__(256) void kb_add_x(const SlotDev *__restrict__ slot
 }
 #undef BATCH_LOOP
 
-// Persistent grid sizes: enough blocks to fill 256 CUs several times over.
-static const int kBatchBlocks256 = 4096, kBatchBlocks1024 = 1024;
+// One workgroup per work unit: the hardware schedules the units of all
+// slots as it would a single hierarchy's launch.
+static unsigned batch_grid(const WorkList &wl) { return (unsigned)(wl.total > 0 ? wl.total : 1); }
 
 template <typename T>
 void launch_b_gs_pair(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int pair) {
 	if (wl.n <= 0) return;
-	hipLaunchKernelGGL(kb_gs_pair<T>, dim3(kBatchBlocks256), dim3(64, 4), 0, s, slots, wl, level, pair & 1, (pair >> 1) & 1);
+	hipLaunchKernelGGL(kb_gs_pair<T>, dim3(batch_grid(wl)), dim3(64, 4), 0, s, slots, wl, level, pair & 1, (pair >> 1) & 1);
 	HIPCHECK_LAUNCH();
 }
 template <typename T>
 void launch_b_gs_small(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int nsweeps) {
 	if (wl.n <= 0) return;
-	hipLaunchKernelGGL(kb_gs_small<T>, dim3(kBatchBlocks1024), dim3(1024), kSmallLdsBytes, s, slots, wl, level, nsweeps);
+	hipLaunchKernelGGL(kb_gs_small<T>, dim3(batch_grid(wl)), dim3(1024), kSmallLdsBytes, s, slots, wl, level, nsweeps);
 	HIPCHECK_LAUNCH();
 }
 template <typename T>
 void launch_b_residual(hipStream_t s, const SlotDev *slots, WorkList wl, int level) {
 	if (wl.n <= 0) return;
-	hipLaunchKernelGGL(kb_residual<T>, dim3(kBatchBlocks256), dim3(64, 4), 0, s, slots, wl, level);
+	hipLaunchKernelGGL(kb_residual<T>, dim3(batch_grid(wl)), dim3(64, 4), 0, s, slots, wl, level);
 	HIPCHECK_LAUNCH();
 }
 void launch_b_residual_top(hipStream_t s, const SlotDev *slots, WorkList wl, bool to_f32) {
 	if (wl.n <= 0) return;
-	hipLaunchKernelGGL(kb_residual_top, dim3(kBatchBlocks256), dim3(64, 4), 0, s, slots, wl, to_f32 ? 1 : 0);
+	hipLaunchKernelGGL(kb_residual_top, dim3(batch_grid(wl)), dim3(64, 4), 0, s, slots, wl, to_f32 ? 1 : 0);
 	HIPCHECK_LAUNCH();
 }
 void launch_b_reduce(hipStream_t s, const SlotDev *slots, WorkList wl, double *norms) {
 	if (wl.n <= 0) return;
-	hipLaunchKernelGGL(kb_reduce, dim3(kBatchBlocks1024), dim3(1024), 0, s, slots, wl, norms);
+	hipLaunchKernelGGL(kb_reduce, dim3(batch_grid(wl)), dim3(1024), 0, s, slots, wl, norms);
 	HIPCHECK_LAUNCH();
 }
 template <typename T>
 void launch_b_restrict(hipStream_t s, const SlotDev *slots, WorkList wl, int level) {
 	if (wl.n <= 0) return;
-	hipLaunchKernelGGL(kb_restrict<T>, dim3(kBatchBlocks256), dim3(64, 4), 0, s, slots, wl, level);
+	hipLaunchKernelGGL(kb_restrict<T>, dim3(batch_grid(wl)), dim3(64, 4), 0, s, slots, wl, level);
 	HIPCHECK_LAUNCH();
 }
 template <typename T>
 void launch_b_interp(hipStream_t s, const SlotDev *slots, WorkList wl, int level) {
 	if (wl.n <= 0) return;
-	hipLaunchKernelGGL(kb_interp<T>, dim3(kBatchBlocks256), dim3(64, 4), 0, s, slots, wl, level);
+	hipLaunchKernelGGL(kb_interp<T>, dim3(batch_grid(wl)), dim3(64, 4), 0, s, slots, wl, level);
 	HIPCHECK_LAUNCH();
 }
 template <typename T>
 void launch_b_lu_solve(hipStream_t s, const SlotDev *slots, WorkList wl, int level) {
 	if (wl.n <= 0) return;
-	hipLaunchKernelGGL(kb_lu_solve<T>, dim3(kBatchBlocks1024), dim3(1024), 0, s, slots, wl, level);
+	hipLaunchKernelGGL(kb_lu_solve<T>, dim3(batch_grid(wl)), dim3(1024), 0, s, slots, wl, level);
 	HIPCHECK_LAUNCH();
 }
 void launch_b_add_x(hipStream_t s, const SlotDev *slots, WorkList wl) {
 	if (wl.n <= 0) return;
-	hipLaunchKernelGGL(kb_add_x, dim3(kBatchBlocks256), dim3(64, 4), 0, s, slots, wl);
+	hipLaunchKernelGGL(kb_add_x, dim3(batch_grid(wl)), dim3(64, 4), 0, s, slots, wl);
 	HIPCHECK_LAUNCH();
 }
 
