@@ -167,3 +167,39 @@ def test_device_generator_matches_host(family):
     assert np.array_equal(h.get_operator27(0), O.gen_operator(family, 4242, m, n, o))
     assert np.array_equal(h.get_rhs(0), O.gen_rhs(4242, m, n, o))
     h.close()
+
+
+# Shapes chosen to run every lane layout of the row kernels (too large for
+# the single-workgroup small-level path): several short rows per wavefront
+# with and without the 2^k+1 point out, one row per wavefront, and long rows
+# streamed in segments (FP64 above 129 points, FP32 above 257, with the point
+# out in the last FP32 segment at 509).
+ROW_SHAPES = [(33, 30, 20, 1), (40, 26, 20, 3), (65, 20, 14, 3), (129, 12, 10, 1), (131, 12, 10, 3),
+              (257, 8, 7, 1), (300, 8, 6, 3), (509, 5, 5, 1)]
+
+
+@pytest.mark.parametrize("shape", ROW_SHAPES, ids=lambda s: "%dx%dx%d" % s[:3])
+@pytest.mark.parametrize("precision", [P.PREC_FP64, P.PREC_MIXED])
+def test_row_kernel_layouts_bitwise(shape, precision):
+    m, n, o, fam = shape
+    say("case", shape, precision)
+    A = O.gen_operator(fam, 77, m, n, o)
+    b = O.gen_rhs(77, m, n, o)
+    h = P.Hierarchy(m, n, o, precision=precision, smoother=P.SMOOTH_COLOUR8)
+    h.set_operator27(A)
+    h.compute_rats()
+    h.set_rhs(b)
+    mg = O.Multigrid(A, colour8=True)
+    mg.set_rhs(b)
+    if precision == P.PREC_FP64:
+        for c in range(2):
+            h.v_cycle()
+            mg.v_cycle()
+        assert abs(h.l2_error_all() / mg.l2_error_all() - 1) < NORM_RTOL
+    else:
+        c, hist = h.solve(1e-30, 2, check=False)
+        c2, hist2 = mg.solve(1e-30, 2, mixed=True)
+        assert c == c2 == 2
+        np.testing.assert_allclose(hist, hist2, rtol=1e-9)
+    assert np.array_equal(h.get_x(0), mg.get_x(0))
+    h.close()
