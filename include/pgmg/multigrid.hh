@@ -222,8 +222,26 @@ class multigrid {
 			return rc;
 		}
 
-		/** PCG helpers (multigrid.hh:222-251). */
+		/** PCG helpers (multigrid.hh:222-251), on the top level. */
 		inline void compute_residual(double *ext_r) {reg[0]->compute_residual(ext_r);}
+		void Ax(const double *in, double *out) {reg[0]->Ax(in, out);}
+		void unpadded_Ax(const double *in, double *out, const int lev = 0) {
+			if (lev >= nlevels) p_fatal_error("Unpadded_Ax only available for single MPI process, within multigrid hierarchy.", 1);
+			reg[lev]->unpadded_Ax(in, out);
+		}
+		void copy_rhs_to_soln() {reg[0]->copy_rhs_to_soln();}
+		double source_soln_dot() {return reg[0]->source_soln_dot();}
+		template <class p_class>
+		void update_r_field(p_class &pr) {reg[0]->update_r_field(pr);}
+		template <class p_class>
+		void overwrite_r_field(p_class &pr) {reg[0]->overwrite_r_field(pr);}
+		double soln_diff(double *ext_x) {return reg[0]->soln_diff(ext_x);}
+		/** z-slice output (multigrid.cc:158-176). */
+		void output_x(const char *filename, int k, int level = 0) {if (level < nlevels) reg[level]->output_x(filename, k);}
+		void output_r(const char *filename, int k, int level = 0) {if (level < nlevels) reg[level]->output_r(filename, k);}
+		void output_residual(const char *filename, int k, int level = 0) {
+			if (level < nlevels) reg[level]->output_residual(filename, k);
+		}
 		inline void free() {}
 		inline void setup_test(int ca, bool xfield, int level = 0) {
 			if (level < nlevels) reg[level]->setup_test(ca, xfield);

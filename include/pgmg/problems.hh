@@ -46,6 +46,9 @@ class synthetic_problem {
 		inline double *external_ptr(int, int, int) {return NULL;}
 		inline double x_field(int, int, int) {return 0;}
 		inline double r_field(int i, int j, int k) {return pgmg_rhs(seed, m, n, o, i, j, k);}
+		/** For overwrite_r_field / update_r_field (region.hh:207-224). */
+		inline double reset_r_field(int i, int j, int k) {return r_field(i, j, k);}
+		inline double update_r_field(int i, int j, int k) {return 0.5 * r_field(i, j, k);}
 };
 
 class problem_simple {

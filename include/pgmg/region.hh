@@ -15,6 +15,7 @@
 #define PGMG_AMD_REGION_HH
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -196,6 +197,52 @@ class region : public pgmg_region_counter<0> {
 			}
 			sync_to_device();
 		}
+		/** Operations on outside arrays of the level's size, for
+		 * preconditioned CG: region::Ax (region.cc:1907-1916) and, at one
+		 * rank without periodic axes the same, unpadded_Ax
+		 * (region.cc:1880-1905). */
+		void Ax(const double *in, double *out) {pgmg_check(pgmg_apply_operator(hier, level, in, out), "Ax");}
+		void unpadded_Ax(const double *in, double *out) {Ax(in, out);}
+		/** region::source_soln_dot (region.cc:1918-1930). */
+		double source_soln_dot() {double v; pgmg_check(pgmg_rhs_x_dot(hier, level, &v), "source_soln_dot"); return v;}
+		/** region::copy_rhs_to_soln (region.cc:1932-1941). */
+		void copy_rhs_to_soln() {pgmg_check(pgmg_copy_rhs_to_x(hier, level), "copy_rhs_to_soln");}
+		/** region::soln_diff (region.hh:226-236). */
+		double soln_diff(double *ext_x) {double v; pgmg_check(pgmg_x_diff2(hier, level, ext_x, &v), "soln_diff"); return v;}
+		/** region::overwrite_r_field / update_r_field (region.hh:207-224). */
+		template <class p_class>
+		void overwrite_r_field(p_class &pr) {
+			double *rp = r0;
+			for (int k = 0; k < so; k++) for (int j = 0; j < sn; j++) for (int i = 0; i < sm; i++)
+				*(rp++) = pr.reset_r_field(i + ai, j + aj, k + ak);
+			sync_r_to_device();
+		}
+		template <class p_class>
+		void update_r_field(p_class &pr) {
+			pgmg_check(pgmg_get_rhs(hier, level, r0), "update_r_field");
+			double *rp = r0;
+			for (int k = 0; k < so; k++) for (int j = 0; j < sn; j++) for (int i = 0; i < sm; i++)
+				*(rp++) += pr.update_r_field(i + ai, j + aj, k + ak);
+			sync_r_to_device();
+		}
+		/** z-slices of x, r and the residual as a binary float matrix
+		 * (region.cc:1562-1660): a header row m, 0 .. m-1, then per row j the
+		 * value j followed by the m values of the row. */
+		void output_x(const char *filename, int k) {
+			std::vector<double> f((size_t)smno);
+			pgmg_check(pgmg_get_x(hier, level, &f[0]), "output_x");
+			output_slice(filename, k, &f[0]);
+		}
+		void output_r(const char *filename, int k) {
+			std::vector<double> f((size_t)smno);
+			pgmg_check(pgmg_get_rhs(hier, level, &f[0]), "output_r");
+			output_slice(filename, k, &f[0]);
+		}
+		void output_residual(const char *filename, int k) {
+			std::vector<double> f((size_t)smno);
+			pgmg_check(pgmg_residual(hier, level, &f[0]), "output_residual");
+			output_slice(filename, k, &f[0]);
+		}
 		inline int total() {return m * n * o;}
 		inline int min_size() {return sm < sn ? (sm < so ? sm : so) : (sn < so ? sn : so);}
 		/** The level operator as 27-entry rows (the reference exposes A and S
@@ -203,6 +250,21 @@ class region : public pgmg_region_counter<0> {
 		void get_operator27(double *A27) {pgmg_check(pgmg_get_operator27(hier, level, A27), "get_operator27");}
 	private:
 		bool owns;
+		void output_slice(const char *filename, int k, const double *f) {
+			if (k < 0 || k >= so) p_fatal_error("Output slice out of range", 1);
+			FILE *outf = fopen(filename, "wb");
+			if (!outf) p_fatal_error("Can't open file", PGMG_FILE_ERROR);
+			std::vector<float> buf((size_t)m + 1);
+			buf[0] = (float)m;
+			for (int i = 0; i < m; i++) buf[i + 1] = (float)i;
+			if (fwrite(&buf[0], sizeof(float), m + 1, outf) != (size_t)m + 1) p_fatal_error("File output error", 1);
+			for (int j = 0; j < n; j++) {
+				buf[0] = (float)j;
+				for (int i = 0; i < m; i++) buf[i + 1] = (float)f[i + (size_t)m * (j + (size_t)n * k)];
+				if (fwrite(&buf[0], sizeof(float), m + 1, outf) != (size_t)m + 1) p_fatal_error("File output error", 1);
+			}
+			fclose(outf);
+		}
 		void init_fields() {
 			ip = jp = kp = 0; mp = np = op = 1;
 			ai = aj = ak = 0; bi = m; bj = n; bk = o;

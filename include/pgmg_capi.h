@@ -143,6 +143,16 @@ int pgmg_rhs_l2norm(pgmg_hier *h, int level, double *out);
 /* region::compute_residual, region.cc:1848-1854: the residual r - A x of a
  * level into a host array (point-major). */
 int pgmg_residual(pgmg_hier *h, int level, double *res);
+/* Operations on outside arrays, for preconditioned CG (multigrid.hh:222-251).
+ * Arrays are point-major, p = i + m*(j + n*k), at the level's size.
+ * region::Ax / unpadded_Ax, region.cc:1880-1916: out = A in. */
+int pgmg_apply_operator(pgmg_hier *h, int level, const double *in, double *out);
+/* region::copy_rhs_to_soln, region.cc:1932-1941: x = r. */
+int pgmg_copy_rhs_to_x(pgmg_hier *h, int level);
+/* region::source_soln_dot, region.cc:1918-1930: sum of x*r. */
+int pgmg_rhs_x_dot(pgmg_hier *h, int level, double *out);
+/* region::soln_diff, region.hh:226-236: sum of (ext_x - x)^2. */
+int pgmg_x_diff2(pgmg_hier *h, int level, const double *ext_x, double *out);
 /* Solve to a relative residual: V-cycles (FP64) or FP64 defect correction
  * around FP32 V-cycles (MIXED) from the current x until
  * sqrt(l2_error_all/rhs_l2norm) < tol.  hist (max_cycles+1 doubles, may be

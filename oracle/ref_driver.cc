@@ -232,6 +232,22 @@ int main(int argc, char **argv) {
 		fclose(outf);
 		printf("{\"ok\": true, \"levels\": %d}\n", mg.nlevels);
 		fflush(stdout);
+	} else if (cmd == "slices") {
+		/* Two V-cycles from x = 0, then z-slice k of x, r and the residual
+		 * at levels 0 and 1 through the reference's own output routines. */
+		int k = atoi(argv[7]);
+		std::string prefix = argv[8];
+		mg.v_cycle();
+		mg.v_cycle();
+		for (int l = 0; l < 2 && l < mg.nlevels; l++) {
+			int kl = l == 0 ? k : k / 2;
+			char nm[512];
+			snprintf(nm, sizeof nm, "%s_x%d.bin", prefix.c_str(), l); mg.output_x(nm, kl, l);
+			snprintf(nm, sizeof nm, "%s_r%d.bin", prefix.c_str(), l); mg.output_r(nm, kl, l);
+			snprintf(nm, sizeof nm, "%s_res%d.bin", prefix.c_str(), l); mg.output_residual(nm, kl, l);
+		}
+		printf("{\"ok\": true}\n");
+		fflush(stdout);
 	} else if (cmd == "solve" || cmd == "time") {
 		std::string bottom = argc > 7 ? argv[7] : "gs24";
 		double tol = argc > 8 ? atof(argv[8]) : 1e-10;

@@ -76,6 +76,10 @@ _SIGS = {
     "pgmg_rhs_l2norm": (_i, [_VP, _i, _D]),
     "pgmg_solve": (_i, [_VP, ctypes.c_double, _i, _I, _D]),
     "pgmg_residual": (_i, [_VP, _i, _D]),
+    "pgmg_apply_operator": (_i, [_VP, _i, _D, _D]),
+    "pgmg_copy_rhs_to_x": (_i, [_VP, _i]),
+    "pgmg_rhs_x_dot": (_i, [_VP, _i, _D]),
+    "pgmg_x_diff2": (_i, [_VP, _i, _D, _D]),
     "pgmg_synchronize": (_i, [_VP]),
     "pgmg_gs_count": (_i, [_VP, ctypes.POINTER(ctypes.c_longlong)]),
     "pgmg_prof_enable": (_i, [_VP, _i, _i]),
@@ -255,6 +259,30 @@ class Hierarchy:
         out = np.zeros(self.shape(level))
         _check(lib().pgmg_residual(self.h, level, _dptr(out)))
         return out
+
+    # operations on outside arrays (multigrid.hh:222-251)
+    def apply_operator(self, v, level=0):
+        """A v at a level (region::Ax)."""
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        assert v.size == int(np.prod(self.shape(level)))
+        out = np.zeros(self.shape(level))
+        _check(lib().pgmg_apply_operator(self.h, level, _dptr(v), _dptr(out)))
+        return out
+
+    def copy_rhs_to_x(self, level=0):
+        _check(lib().pgmg_copy_rhs_to_x(self.h, level))
+
+    def rhs_x_dot(self, level=0):
+        v = ctypes.c_double()
+        _check(lib().pgmg_rhs_x_dot(self.h, level, ctypes.byref(v)))
+        return v.value
+
+    def x_diff2(self, y, level=0):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        assert y.size == int(np.prod(self.shape(level)))
+        v = ctypes.c_double()
+        _check(lib().pgmg_x_diff2(self.h, level, _dptr(y), ctypes.byref(v)))
+        return v.value
 
     def synchronize(self):
         _check(lib().pgmg_synchronize(self.h))

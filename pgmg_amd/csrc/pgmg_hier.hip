@@ -64,6 +64,7 @@ struct pgmg_hier {
 	int nl = 0;
 	Level lev[kMaxLevels];
 	double *res64 = nullptr;   // residual scratch, top-level size
+	double *aux64 = nullptr;   // two top-level vectors for the outside-array operations
 	float *res32 = nullptr;
 	double *partial = nullptr;
 	int partial_cap = 0;
@@ -452,7 +453,7 @@ int pgmg_hier_destroy(pgmg_hier *h) {
 		hipFree(L.A); hipFree(L.x); hipFree(L.r);
 		hipFree(L.A32); hipFree(L.x32); hipFree(L.r32);
 	}
-	hipFree(h->res64); hipFree(h->res32); hipFree(h->partial); hipFree(h->dscalar);
+	hipFree(h->res64); hipFree(h->res32); hipFree(h->partial); hipFree(h->dscalar); hipFree(h->aux64);
 	if (h->hscalar) hipHostFree(h->hscalar);
 	hipFree(h->LU); hipFree(h->ipiv); hipFree(h->lu_info);
 	for (auto e : h->prof_ev) hipEventDestroy(e);
@@ -737,6 +738,73 @@ int pgmg_residual(pgmg_hier *h, int level, double *res) {
 	Level &L = h->lev[level];
 	launch_residual<double, double>(h->stream, L.v, L.A, L.x, L.r, h->res64, (double *)nullptr);
 	return vec_io(h, level, h->res64, nullptr, res);
+}
+
+// Two level-sized device vectors with zero halos for the operations on
+// outside arrays (allocated on first use).
+static int aux_vectors(pgmg_hier *h, int level, double **a, double **b) {
+	const long long np0 = h->lev[0].v.npad;
+	if (!h->aux64) HIP_TRY(hipMalloc(&h->aux64, sizeof(double) * 2 * (size_t)np0));
+	const long long np = h->lev[level].v.npad;
+	launch_fill<double>(h->stream, h->aux64, np, 0.0);
+	launch_fill<double>(h->stream, h->aux64 + np0, np, 0.0);
+	*a = h->aux64;
+	*b = h->aux64 + np0;
+	return PGMG_OK;
+}
+
+int pgmg_apply_operator(pgmg_hier *h, int level, const double *in, double *out) {
+	int rc = check_level(h, level);
+	if (rc) return rc;
+	if (!in || !out) return fail(PGMG_ERR_ARG, "null array");
+	double *din, *dout;
+	rc = aux_vectors(h, level, &din, &dout);
+	if (rc) return rc;
+	rc = vec_io(h, level, din, in, nullptr);
+	if (rc) return rc;
+	Level &L = h->lev[level];
+	launch_apply(h->stream, L.v, L.A, din, dout);
+	return vec_io(h, level, dout, nullptr, out);
+}
+
+int pgmg_copy_rhs_to_x(pgmg_hier *h, int level) {
+	int rc = check_level(h, level);
+	if (rc) return rc;
+	Level &L = h->lev[level];
+	HIP_TRY(hipMemcpyAsync(L.x, L.r, sizeof(double) * (size_t)L.v.npad, hipMemcpyDeviceToDevice, h->stream));
+	HIP_TRY(hipStreamSynchronize(h->stream));
+	return PGMG_OK;
+}
+
+int pgmg_rhs_x_dot(pgmg_hier *h, int level, double *out) {
+	int rc = check_level(h, level);
+	if (rc) return rc;
+	if (!out) return fail(PGMG_ERR_ARG, "null output");
+	Level &L = h->lev[level];
+	const int n = ((L.v.m + 63) / 64) * ((L.v.n + 3) / 4) * L.v.o;
+	rc = ensure_partial(h, n);
+	if (rc) return rc;
+	launch_pair_sum(h->stream, L.v, L.x, L.r, 0, h->partial);
+	launch_reduce(h->stream, h->partial, n, h->dscalar);
+	return read_scalar(h, out);
+}
+
+int pgmg_x_diff2(pgmg_hier *h, int level, const double *ext_x, double *out) {
+	int rc = check_level(h, level);
+	if (rc) return rc;
+	if (!ext_x || !out) return fail(PGMG_ERR_ARG, "null argument");
+	double *dext, *unused;
+	rc = aux_vectors(h, level, &dext, &unused);
+	if (rc) return rc;
+	rc = vec_io(h, level, dext, ext_x, nullptr);
+	if (rc) return rc;
+	Level &L = h->lev[level];
+	const int n = ((L.v.m + 63) / 64) * ((L.v.n + 3) / 4) * L.v.o;
+	rc = ensure_partial(h, n);
+	if (rc) return rc;
+	launch_pair_sum(h->stream, L.v, dext, L.x, 1, h->partial);
+	launch_reduce(h->stream, h->partial, n, h->dscalar);
+	return read_scalar(h, out);
 }
 
 int pgmg_synchronize(pgmg_hier *h) {

@@ -170,6 +170,10 @@ template <typename T, typename TR> int launch_residual(hipStream_t s, const Leve
 						       const T *r, TR *res, double *partial, TR *zero_out = nullptr);
 // Partial sums (workgroups) of an FP64 residual launch over level L.
 int residual_partial_count(const LevelView &L);
+// out = A in at interior points (in with a zero halo).
+void launch_apply(hipStream_t s, const LevelView &L, const double *A, const double *in, double *out);
+// Block sums of a*b (diff = 0) or (a-b)^2 (diff = 1); returns the count.
+int launch_pair_sum(hipStream_t s, const LevelView &L, const double *a, const double *b, int diff, double *partial);
 // out[0] = sum of partial[0..n) in a fixed order.
 void launch_reduce(hipStream_t s, const double *partial, int n, double *out);
 // partial sums of r^2 (rhs_l2norm); returns count.

@@ -1195,6 +1195,53 @@ int launch_sumsq(hipStream_t s, const LevelView &L, const T *r, double *partial)
 	return ((L.m + 63) / 64) * ((L.n + 3) / 4) * L.o;
 }
 
+// out = A in at the interior points of L (region::mul_A with an outside
+// array, region.cc:1861-1869, used by region::Ax): the 27-entry sum in box
+// order from zero.  in must have a zero halo.
+__global__ __launch_bounds__(256) void k_apply(LevelView L, const double *__restrict__ A, const double *__restrict__ in,
+					       double *__restrict__ out) {
+	const int i = (int)(blockIdx.x * 64 + threadIdx.x), j = (int)(blockIdx.y * 4 + threadIdx.y), k = (int)blockIdx.z;
+	if (i >= L.m || j >= L.n) return;
+	const long long p = idx3(L, i, j, k), px = L.px, pxy = L.pxy;
+	const double *Ap = A + aidx3(L, 0, i, j, k);
+	double ax = 0.0;
+#pragma unroll
+	for (int e = 0; e < 27; e++) {
+		const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
+		ax = ax + Ap[e * px] * in[p + di + dj * px + dk * pxy];
+	}
+	out[p] = ax;
+}
+
+void launch_apply(hipStream_t s, const LevelView &L, const double *A, const double *in, double *out) {
+	dim3 grid((L.m + 63) / 64, (L.n + 3) / 4, L.o), block(64, 4, 1);
+	hipLaunchKernelGGL(k_apply, grid, block, 0, s, L, A, in, out);
+	HIPCHECK_LAUNCH();
+}
+
+// Block sums over the interior points of L of a*b (diff = 0) or (a-b)^2
+// (diff = 1).
+__global__ __launch_bounds__(256) void k_pair_sum(LevelView L, const double *__restrict__ a, const double *__restrict__ b,
+						  int diff, double *__restrict__ partial) {
+	const int i = (int)(blockIdx.x * 64 + threadIdx.x), j = (int)(blockIdx.y * 4 + threadIdx.y), k = (int)blockIdx.z;
+	double v = 0.0;
+	if (i < L.m && j < L.n) {
+		const long long p = idx3(L, i, j, k);
+		if (diff) { const double d = a[p] - b[p]; v = d * d; }
+		else v = a[p] * b[p];
+	}
+	const double sblk = block_sum_256(v);
+	if (threadIdx.x == 0 && threadIdx.y == 0)
+		partial[blockIdx.x + gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z)] = sblk;
+}
+
+int launch_pair_sum(hipStream_t s, const LevelView &L, const double *a, const double *b, int diff, double *partial) {
+	dim3 grid((L.m + 63) / 64, (L.n + 3) / 4, L.o), block(64, 4, 1);
+	hipLaunchKernelGGL(k_pair_sum, grid, block, 0, s, L, a, b, diff, partial);
+	HIPCHECK_LAUNCH();
+	return ((L.m + 63) / 64) * ((L.n + 3) / 4) * L.o;
+}
+
 // Fixed-order reduction: thread t sums partial[t], partial[t+1024], ... in
 // order, then a fixed binary tree.
 __global__ __launch_bounds__(1024) void k_reduce(const double *__restrict__ partial, int n, double *__restrict__ out) {

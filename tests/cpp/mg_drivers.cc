@@ -6,8 +6,17 @@
 //  - guide: guide.set_tol(scale) then solve_v_cycles() three times
 //    (guide_test.cc:26-50), printing the return code, residual and cycle
 //    prediction;
-//  - fmg: one fmg_cycle() from x = 0 (multigrid.cc:109-134), then l2.
-//   mg_drivers <cycles|guide|fmg> <family> <seed> <m> <n> <o> <x_out> [smoother] [bottom]
+//  - fmg: one fmg_cycle() from x = 0 (multigrid.cc:109-134), then l2;
+//  - slices: two V-cycles, then output_x / output_r / output_residual of
+//    z-slice k (argv[8]) at level 0 and k/2 at level 1, files <x_out>_x0.bin
+//    etc. (as the reference's output routines name nothing, the names are
+//    the caller's);
+//  - pcg: conjugate gradients preconditioned by one V-cycle from x = 0
+//    (the way PGMG preconditions CG, README.md:55-61), with Ax and
+//    source_soln_dot, printing |r|^2 per iteration; then the other
+//    outside-array helpers (copy_rhs_to_soln, soln_diff, overwrite_r_field,
+//    update_r_field) on the result.
+//   mg_drivers <cycles|guide|fmg|slices|pcg> <family> <seed> <m> <n> <o> <x_out> [smoother|k] [bottom]
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -24,7 +33,7 @@ int main(int argc, char **argv) {
 	comm_buffer com;
 	pgmg_opts opts;
 	pgmg_default_opts(&opts);
-	opts.smoother = argc > 8 ? atoi(argv[8]) : PGMG_SMOOTH_EXACT;
+	opts.smoother = argc > 8 && strcmp(mode, "slices") != 0 ? atoi(argv[8]) : PGMG_SMOOTH_EXACT;
 	opts.bottom = argc > 9 ? atoi(argv[9]) : PGMG_BOTTOM_GS24;
 	multigrid mg(p, gm, com, &opts);
 	mg.setup_matrices(p);
@@ -57,6 +66,67 @@ int main(int argc, char **argv) {
 			int rc = mg.solve_v_cycles(res);
 			printf("guide %d rc %d res %.6g pred %d next %d\n", t, rc, res, pred, mg.guide.pred_iters());
 		}
+	} else if (strcmp(mode, "slices") == 0) {
+		const int k = argc > 8 ? atoi(argv[8]) : 0;
+		mg.v_cycle();
+		mg.v_cycle();
+		for (int l = 0; l < 2 && l < mg.nlevels; l++) {
+			const int kl = l == 0 ? k : k / 2;
+			char nm[512];
+			snprintf(nm, sizeof nm, "%s_x%d.bin", argv[7], l); mg.output_x(nm, kl, l);
+			snprintf(nm, sizeof nm, "%s_r%d.bin", argv[7], l); mg.output_r(nm, kl, l);
+			snprintf(nm, sizeof nm, "%s_res%d.bin", argv[7], l); mg.output_residual(nm, kl, l);
+		}
+		printf("slices done\n");
+	} else if (strcmp(mode, "pcg") == 0) {
+		region &R = *mg.reg[0];
+		const int N = R.smno;
+		R.sync_from_device();
+		double *b = new double[N], *u = new double[N], *r = new double[N], *z = new double[N];
+		double *pv = new double[N], *Ap = new double[N];
+		double bb = 0;
+		for (int q = 0; q < N; q++) {b[q] = R.r0[q]; u[q] = 0; r[q] = b[q]; bb += b[q] * b[q];}
+		double rz_old = 0, rr = bb;
+		printf("pcg 0 %.17g\n", rr);
+		int it = 0;
+		while (it < 40 && rr > 1e-20 * bb) {
+			// z = M^-1 r: one V-cycle on A z = r from z = 0
+			memcpy(R.r0, r, sizeof(double) * N);
+			R.sync_r_to_device();
+			R.allclear_x_field();
+			mg.v_cycle();
+			const double rz = mg.source_soln_dot();
+			R.sync_from_device();
+			memcpy(z, R.x0, sizeof(double) * N);
+			const double beta = it == 0 ? 0 : rz / rz_old;
+			for (int q = 0; q < N; q++) pv[q] = z[q] + beta * pv[q];
+			mg.Ax(pv, Ap);
+			double pAp = 0;
+			for (int q = 0; q < N; q++) pAp += pv[q] * Ap[q];
+			const double alpha = rz / pAp;
+			rr = 0;
+			for (int q = 0; q < N; q++) {u[q] += alpha * pv[q]; r[q] -= alpha * Ap[q]; rr += r[q] * r[q];}
+			rz_old = rz;
+			it++;
+			printf("pcg %d %.17g\n", it, rr);
+		}
+		// check the solution against the operator: |b - A u|^2
+		mg.Ax(u, Ap);
+		double tr = 0;
+		for (int q = 0; q < N; q++) tr += (b[q] - Ap[q]) * (b[q] - Ap[q]);
+		printf("true %.17g bb %.17g\n", tr, bb);
+		// x = u on the device: soln_diff against u is 0
+		memcpy(R.x0, u, sizeof(double) * N);
+		R.sync_x_to_device();
+		printf("diff_self %.17g\n", mg.soln_diff(u));
+		// source back to b, then x = r and soln_diff against b is 0
+		mg.overwrite_r_field(p);
+		mg.copy_rhs_to_soln();
+		printf("diff_rhs %.17g\n", mg.soln_diff(b));
+		// update_r_field adds 0.5 b: rhs_l2norm grows by 2.25
+		mg.update_r_field(p);
+		printf("rhs_ratio %.17g\n", mg.rhs_l2norm() / bb);
+		delete[] b; delete[] u; delete[] r; delete[] z; delete[] pv; delete[] Ap;
 	} else {
 		mg.fmg_cycle();
 		printf("fmg %.17g rhs %.17g\n", mg.l2_error_all(), mg.rhs_l2norm());

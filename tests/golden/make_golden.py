@@ -44,6 +44,14 @@ SOLVES = [
 ]
 
 
+# z-slice files written by the reference's output_x / output_r /
+# output_residual after two V-cycles: (name, family, seed, m, n, o, k).
+SLICES = [
+    ("slices_f7u_20x17x30", 1, 7, 20, 17, 30, 11),
+    ("slices_q1_18x13x22", 3, 3, 18, 13, 22, 9),
+]
+
+
 def read_records(path):
     out = {}
     with open(path, "rb") as f:
@@ -111,6 +119,17 @@ def main():
                             tol=np.array(tol), res=rec["res"], x=rec["x"], rhs_l2norm=rec["rhs_l2norm"],
                             info=np.array(json.dumps(info)))
         print(name, info, flush=True)
+    for name, fam, seed, m, n, o, k in SLICES:
+        prefix = os.path.join(ROOT, "_work", "golden_slice")
+        run(["slices", fam, seed, m, n, o, k, prefix])
+        save = {"meta": np.array([fam, seed, m, n, o, k], dtype=np.int64)}
+        for field in ("x", "r", "res"):
+            for lev in (0, 1):
+                path = "%s_%s%d.bin" % (prefix, field, lev)
+                save["%s%d" % (field, lev)] = np.frombuffer(open(path, "rb").read(), dtype=np.uint8)
+                os.remove(path)
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), **save)
+        print(name, flush=True)
     os.remove(tmp)
 
 

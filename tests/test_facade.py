@@ -119,3 +119,38 @@ def test_fmg_cycle_against_oracle_composition():
             for _ in range(2):
                 mg.gauss_seidel(0, i - 1)
     assert abs(l2 / mg.l2_error_all() - 1) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["slices_f7u_20x17x30", "slices_q1_18x13x22"])
+def test_slice_output_matches_reference_bytes(tmp_path, name):
+    """output_x / output_r / output_residual after two exact-mode V-cycles,
+    byte for byte against the files the reference's own output routines
+    wrote for the same problem (region.cc:1562-1660)."""
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    fam, seed, m, n, o, k = [int(v) for v in z["meta"]]
+    prefix = str(tmp_path / "s")
+    out = run(["mg_drivers", "slices", fam, seed, m, n, o, prefix, k])
+    assert "slices done" in out
+    for field in ("x", "r", "res"):
+        for lev in (0, 1):
+            got = np.fromfile("%s_%s%d.bin" % (prefix, field, lev), dtype=np.uint8)
+            assert np.array_equal(got, z["%s%d" % (field, lev)]), (field, lev)
+
+
+@pytest.mark.gpu
+def test_pcg_with_vcycle_preconditioner():
+    """CG preconditioned by one V-cycle, driven through Ax and
+    source_soln_dot as PGMG's CG users do; then the other outside-array
+    helpers on the result."""
+    out = run(["mg_drivers", "pcg", 1, 5, 33, 29, 40, "/dev/null"]).splitlines()
+    hist = [float(l.split()[2]) for l in out if l.startswith("pcg ")]
+    bb = hist[0]
+    assert hist[-1] < 1e-20 * bb and len(hist) <= 16
+    assert all(hist[q + 1] < hist[q] for q in range(len(hist) - 1))
+    vals = dict((l.split()[0], l.split()[1:]) for l in out if not l.startswith(("pcg", "levels")))
+    vals = {k: [float(t) for t in v if t != "bb"] for k, v in vals.items()}
+    true_res, bb2 = vals["true"][0], vals["true"][1]
+    assert bb2 == bb and true_res < 1e-18 * bb
+    assert vals["diff_self"][0] == 0.0 and vals["diff_rhs"][0] == 0.0
+    assert abs(vals["rhs_ratio"][0] - 2.25) < 1e-12

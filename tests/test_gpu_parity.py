@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 FIXTURES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
-                  if not os.path.basename(p).startswith("solve_"))
+                  if not os.path.basename(p).startswith(("solve_", "slices_")))
 NORM_RTOL = 1e-12
 
 
@@ -202,4 +202,43 @@ def test_row_kernel_layouts_bitwise(shape, precision):
         assert c == c2 == 2
         np.testing.assert_allclose(hist, hist2, rtol=1e-9)
     assert np.array_equal(h.get_x(0), mg.get_x(0))
+    h.close()
+
+
+def apply_27(A, x):
+    """A x over the 27-entry rows in box order from zero (region::mul_A)."""
+    o, n, m = x.shape
+    xp = np.zeros((o + 2, n + 2, m + 2))
+    xp[1:-1, 1:-1, 1:-1] = x
+    ax = np.zeros_like(x)
+    for e in range(27):
+        di, dj, dk = e % 3 - 1, (e // 3) % 3 - 1, e // 9 - 1
+        ax = ax + A[..., e] * xp[1 + dk:o + 1 + dk, 1 + dj:n + 1 + dj, 1 + di:m + 1 + di]
+    return ax
+
+
+@pytest.mark.parametrize("level", [0, 1])
+def test_outside_array_operations(level):
+    """apply_operator bit for bit against the box-order sum; the dot products
+    against numpy sums to 1e-12."""
+    m, n, o = 20, 17, 30
+    A = O.gen_operator(3, 21, m, n, o)
+    b = O.gen_rhs(21, m, n, o)
+    h = P.Hierarchy(m, n, o)
+    h.set_operator27(A)
+    h.compute_rats()
+    Al = h.get_operator27(level)
+    ol, nl, ml = Al.shape[:3]
+    rng = np.random.default_rng(level)
+    v = rng.uniform(-1, 1, (ol, nl, ml))
+    assert np.array_equal(h.apply_operator(v, level), apply_27(Al, v))
+    x = rng.uniform(-1, 1, (ol, nl, ml))
+    r = rng.uniform(-1, 1, (ol, nl, ml))
+    h.set_x(x, level)
+    h.set_rhs(r, level)
+    assert abs(h.rhs_x_dot(level) / float(np.sum(x * r)) - 1) < 1e-12
+    y = rng.uniform(-1, 1, (ol, nl, ml))
+    assert abs(h.x_diff2(y, level) / float(np.sum((y - x) ** 2)) - 1) < 1e-12
+    h.copy_rhs_to_x(level)
+    assert np.array_equal(h.get_x(level), r)
     h.close()
