@@ -5,6 +5,7 @@
 // where the operation order is the same.  All kernels are memory-bound
 // (0.225 flop/byte in FP64); none is a dense contraction, so no MFMA is used.
 #include "pgmg_internal.hpp"
+#include <utility>
 #include "../../include/pgmg_problems.h"
 
 namespace pgmg {
@@ -1500,83 +1501,90 @@ __device__ __forceinline__ void psupport(int J, int M, int &lo, int &hi, int &c)
 	else { lo = j2 == 0 ? 0 : j2 - 1; hi = j2 == M - 2 ? M - 1 : j2 + 2; c = j2; }
 }
 
-// Galerkin coefficient Ac(I, I+d) for one coarse point I and one entry d.
+// Galerkin coefficients Ac(I, I+d) of one coarse point I, all 27 entries.
 // The reference (region.cc:1008-1192) takes each fine point f in cube-scan
 // order, forms At[J] = sum_g 0.125*Af(f,g)*P(g,J) over g in f's box in
 // (k,j,i) order (A_red), scales At by f's restriction weight (A_mul/A_sca)
-// and adds it to the coarse rows of f (A_add).  Here the sum for one (I,J)
-// visits the same f in the same order, and forms each inner sum the same way.
-__global__ __launch_bounds__(256) void k_rat(LevelView F, LevelView C, const double *__restrict__ Af,
-					     double *__restrict__ Ac) {
-	const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	const long long ncp = (long long)C.m * C.n * C.o;
-	if (tid >= ncp * 27) return;
-	const int eJ = (int)(tid % 27);
-	const long long cp = tid / 27;
-	const int I = (int)(cp % C.m), Jy = (int)((cp / C.m) % C.n), K = (int)(cp / ((long long)C.m * C.n));
-	const int dI = eJ % 3 - 1, dJ = (eJ / 3) % 3 - 1, dK = eJ / 9 - 1;
-	const int JI = I + dI, JJ = Jy + dJ, JK = K + dK;
-	double acc = 0.0;
-	if (JI >= 0 && JI < C.m && JJ >= 0 && JJ < C.n && JK >= 0 && JK < C.o) {
-		int plx, phx, pcx, ply, phy, pcy, plz, phz, pcz;
-		psupport(JI, F.m, plx, phx, pcx);
-		psupport(JJ, F.n, ply, phy, pcy);
-		psupport(JK, F.o, plz, phz, pcz);
-		const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
-		const bool evx = !(F.m & 1), evy = !(F.n & 1), evz = !(F.o & 1);
-		for (int cz = 0; cz < 2; cz++) {
-			const int kc = K - 1 + cz;
-			if (kc < 0 || kc > lastz) continue;
-			const bool sz = evz && kc == lastz;
-			for (int cy = 0; cy < 2; cy++) {
-				const int jc = Jy - 1 + cy;
-				if (jc < 0 || jc > lasty) continue;
-				const bool sy = evy && jc == lasty;
-				for (int cx = 0; cx < 2; cx++) {
-					const int ic = I - 1 + cx;
-					if (ic < 0 || ic > lastx) continue;
-					const bool sx = evx && ic == lastx;
-					for (int lz = cz ? 0 : 1; lz < 2; lz++) {
-						const int fk = 2 * kc + lz;
-						if (fk >= F.o) continue;
-						double wz;
-						if (lz == 0) wz = 1.0;
-						else if (cz == 0) wz = sz ? 1.0 : 0.5;
-						else { if (sz) continue; wz = 0.5; }
-						const int gzl = max(plz, fk - 1), gzh = min(phz, min(fk + 2, F.o));
-						if (gzl >= gzh) continue;
-						for (int ly = cy ? 0 : 1; ly < 2; ly++) {
-							const int fj = 2 * jc + ly;
-							if (fj >= F.n) continue;
-							double wy;
-							if (ly == 0) wy = 1.0;
-							else if (cy == 0) wy = sy ? 1.0 : 0.5;
-							else { if (sy) continue; wy = 0.5; }
-							const int gyl = max(ply, fj - 1), gyh = min(phy, min(fj + 2, F.n));
-							if (gyl >= gyh) continue;
-							for (int lx = cx ? 0 : 1; lx < 2; lx++) {
-								const int fi = 2 * ic + lx;
-								if (fi >= F.m) continue;
-								double wx;
-								if (lx == 0) wx = 1.0;
-								else if (cx == 0) wx = sx ? 1.0 : 0.5;
-								else { if (sx) continue; wx = 0.5; }
-								const int gxl = max(plx, fi - 1), gxh = min(phx, min(fi + 2, F.m));
-								if (gxl >= gxh) continue;
-								const double *Afr = Af + aidx3(F, 0, fi, fj, fk);
+// and adds it to the coarse rows of f (A_add).  Here each entry (I,J) visits
+// the same f in the same order and forms each inner sum the same way; f's 27
+// coefficients are loaded once for all J.
+__device__ __noinline__ void rat_row_point(const LevelView &F, const LevelView &C, const double *__restrict__ Af,
+					     double *__restrict__ Ac, int I, int Jy, int K) {
+	// interpolation supports of the coarse neighbours I-1..I+1 per axis
+	int lx[3], hx[3], cx_[3], ly[3], hy[3], cy_[3], lz[3], hz[3], cz_[3];
+	bool okx[3], oky[3], okz[3];
+#pragma unroll
+	for (int d = 0; d < 3; d++) {
+		const int JI = I + d - 1, JJ = Jy + d - 1, JK = K + d - 1;
+		okx[d] = JI >= 0 && JI < C.m; oky[d] = JJ >= 0 && JJ < C.n; okz[d] = JK >= 0 && JK < C.o;
+		lx[d] = hx[d] = cx_[d] = ly[d] = hy[d] = cy_[d] = lz[d] = hz[d] = cz_[d] = 0;
+		if (okx[d]) psupport(JI, F.m, lx[d], hx[d], cx_[d]);
+		if (oky[d]) psupport(JJ, F.n, ly[d], hy[d], cy_[d]);
+		if (okz[d]) psupport(JK, F.o, lz[d], hz[d], cz_[d]);
+	}
+	double acc[27];
+#pragma unroll
+	for (int e = 0; e < 27; e++) acc[e] = 0.0;
+	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
+	const bool evx = !(F.m & 1), evy = !(F.n & 1), evz = !(F.o & 1);
+	for (int cz = 0; cz < 2; cz++) {
+		const int kc = K - 1 + cz;
+		if (kc < 0 || kc > lastz) continue;
+		const bool sz = evz && kc == lastz;
+		for (int cy = 0; cy < 2; cy++) {
+			const int jc = Jy - 1 + cy;
+			if (jc < 0 || jc > lasty) continue;
+			const bool sy = evy && jc == lasty;
+			for (int cx = 0; cx < 2; cx++) {
+				const int ic = I - 1 + cx;
+				if (ic < 0 || ic > lastx) continue;
+				const bool sx = evx && ic == lastx;
+				for (int lzz = cz ? 0 : 1; lzz < 2; lzz++) {
+					const int fk = 2 * kc + lzz;
+					if (fk >= F.o) continue;
+					double wz;
+					if (lzz == 0) wz = 1.0;
+					else if (cz == 0) wz = sz ? 1.0 : 0.5;
+					else { if (sz) continue; wz = 0.5; }
+					for (int lyy = cy ? 0 : 1; lyy < 2; lyy++) {
+						const int fj = 2 * jc + lyy;
+						if (fj >= F.n) continue;
+						double wy;
+						if (lyy == 0) wy = 1.0;
+						else if (cy == 0) wy = sy ? 1.0 : 0.5;
+						else { if (sy) continue; wy = 0.5; }
+						for (int lxx = cx ? 0 : 1; lxx < 2; lxx++) {
+							const int fi = 2 * ic + lxx;
+							if (fi >= F.m) continue;
+							double wx;
+							if (lxx == 0) wx = 1.0;
+							else if (cx == 0) wx = sx ? 1.0 : 0.5;
+							else { if (sx) continue; wx = 0.5; }
+							const double wr = wx * wy * wz;
+							const double *Afr = Af + aidx3(F, 0, fi, fj, fk);
+							double a[27];
+#pragma unroll
+							for (int e = 0; e < 27; e++) a[e] = Afr[e * F.px];
+#pragma unroll
+							for (int eJ = 0; eJ < 27; eJ++) {
+								const int dx = eJ % 3, dy = (eJ / 3) % 3, dz = eJ / 9;
+								if (!(okx[dx] && oky[dy] && okz[dz])) continue;
+								const int gxl = max(lx[dx], fi - 1), gxh = min(hx[dx], min(fi + 2, F.m));
+								const int gyl = max(ly[dy], fj - 1), gyh = min(hy[dy], min(fj + 2, F.n));
+								const int gzl = max(lz[dz], fk - 1), gzh = min(hz[dz], min(fk + 2, F.o));
+								if (gxl >= gxh || gyl >= gyh || gzl >= gzh) continue;
 								double inner = 0.0;
 								for (int kk = gzl; kk < gzh; kk++) {
-									const double kfac = kk == pcz ? 1.0 : 0.5;
+									const double kfac = kk == cz_[dz] ? 1.0 : 0.5;
 									for (int jj = gyl; jj < gyh; jj++) {
-										const double jkfac = jj == pcy ? kfac : 0.5 * kfac;
+										const double jkfac = jj == cy_[dy] ? kfac : 0.5 * kfac;
 										for (int ii = gxl; ii < gxh; ii++) {
 											const int ef = (kk - fk + 1) * 9 + (jj - fj + 1) * 3 + (ii - fi + 1);
-											inner = inner + (0.125 * Afr[ef * F.px]) * (ii == pcx ? jkfac : 0.5 * jkfac);
+											inner = inner + (0.125 * a[ef]) * (ii == cx_[dx] ? jkfac : 0.5 * jkfac);
 										}
 									}
 								}
-								const double wr = wx * wy * wz;
-								acc = acc + inner * wr;
+								acc[eJ] = acc[eJ] + inner * wr;
 							}
 						}
 					}
@@ -1584,12 +1592,162 @@ __global__ __launch_bounds__(256) void k_rat(LevelView F, LevelView C, const dou
 			}
 		}
 	}
-	Ac[aidx3(C, eJ, I, Jy, K)] = acc;
+	double *Acr = Ac + aidx3(C, 0, I, Jy, K);
+#pragma unroll
+	for (int e = 0; e < 27; e++) Acr[e * C.px] = acc[e];
+}
+
+// Interior coarse points (1 <= I < lastx, likewise J, K: every cube of I's
+// restriction full, every fine box and interpolation support unclipped): the
+// terms k_rat_row visits are fixed by the offsets alone, so they are
+// expanded at compile time.  For fine point f = 2I + df (df per axis -1, 0,
+// 1 in the cube-scan order) and entry J = I + d, the term of box entry ef
+// (offset eg) enters with P weight pw(df+eg, 2d): 1 at J's fine point, 1/2
+// next to it, else absent; the products and sums are those of k_rat_row in
+// the same order.
+namespace rat {
+__host__ __device__ constexpr double pw(int dg, int c) {
+	return dg == c ? 1.0 : ((dg - c == 1 || c - dg == 1) ? 0.5 : 0.0);
+}
+template <int DFX, int DFY, int DFZ, int J, int EF>
+__device__ __forceinline__ void term(const double (&a8)[27], double &inner) {
+	constexpr int dx = J % 3 - 1, dy = (J / 3) % 3 - 1, dz = J / 9 - 1;
+	constexpr int ex = EF % 3 - 1, ey = (EF / 3) % 3 - 1, ez = EF / 9 - 1;
+	constexpr double wx = pw(DFX + ex, 2 * dx), wy = pw(DFY + ey, 2 * dy), wz = pw(DFZ + ez, 2 * dz);
+	if constexpr (wx != 0.0 && wy != 0.0 && wz != 0.0) {
+		constexpr double kfac = wz;
+		constexpr double jkfac = wy == 1.0 ? kfac : 0.5 * kfac;
+		constexpr double w = wx == 1.0 ? jkfac : 0.5 * jkfac;
+		inner = inner + a8[EF] * w;
+	}
+}
+template <int DFX, int DFY, int DFZ, int J>
+__host__ __device__ constexpr bool touches() {
+	constexpr int dx = J % 3 - 1, dy = (J / 3) % 3 - 1, dz = J / 9 - 1;
+	bool any = false;
+	for (int EF = 0; EF < 27; EF++)
+		if (pw(DFX + EF % 3 - 1, 2 * dx) != 0.0 && pw(DFY + (EF / 3) % 3 - 1, 2 * dy) != 0.0 &&
+		    pw(DFZ + EF / 9 - 1, 2 * dz) != 0.0)
+			any = true;
+	return any;
+}
+template <int DFX, int DFY, int DFZ, int J, int... EF>
+__device__ __forceinline__ void entry(const double (&a8)[27], double wr, double (&acc)[27],
+				      std::integer_sequence<int, EF...>) {
+	if constexpr (touches<DFX, DFY, DFZ, J>()) {
+		double inner = 0.0;
+		(term<DFX, DFY, DFZ, J, EF>(a8, inner), ...);
+		acc[J] = acc[J] + inner * wr;
+	}
+}
+template <int DFX, int DFY, int DFZ, int... J>
+__device__ __forceinline__ void fine_point(const LevelView &F, const double *__restrict__ Af, int fi, int fj, int fk,
+					   double (&acc)[27], std::integer_sequence<int, J...>) {
+	// restriction weight of f: 1 at the cube's local 0, 1/2 at local 1
+	constexpr double wx = DFX == 0 ? 1.0 : 0.5, wy = DFY == 0 ? 1.0 : 0.5, wz = DFZ == 0 ? 1.0 : 0.5;
+	const double wr = wx * wy * wz;
+	const double *Afr = Af + aidx3(F, 0, fi + DFX, fj + DFY, fk + DFZ);
+	double a8[27];
+#pragma unroll
+	for (int e = 0; e < 27; e++) a8[e] = 0.125 * Afr[e * F.px];
+	(entry<DFX, DFY, DFZ, J>(a8, wr, acc, std::make_integer_sequence<int, 27>()), ...);
+}
+// Cube q (cz = q>>2, cy, cx) and its local point l (lz = l>>2, ly, lx), in
+// the scan order cube first, then local; local 0 of a cube at offset 0 on
+// an axis exists only in the cube at offset 1 (restrict_point's loops).
+template <int Q, int L>
+__device__ __forceinline__ void cube_local(const LevelView &F, const double *__restrict__ Af, int fi, int fj, int fk,
+					   double (&acc)[27]) {
+	constexpr int cz = Q >> 2, cy = (Q >> 1) & 1, cx = Q & 1;
+	constexpr int lz = L >> 2, ly = (L >> 1) & 1, lx = L & 1;
+	if constexpr ((cz || lz) && (cy || ly) && (cx || lx))
+		fine_point<2 * (cx - 1) + lx, 2 * (cy - 1) + ly, 2 * (cz - 1) + lz>(F, Af, fi, fj, fk, acc,
+										      std::make_integer_sequence<int, 27>());
+}
+template <int Q, int... L>
+__device__ __forceinline__ void cube(const LevelView &F, const double *__restrict__ Af, int fi, int fj, int fk,
+				     double (&acc)[27], std::integer_sequence<int, L...>) {
+	(cube_local<Q, L>(F, Af, fi, fj, fk, acc), ...);
+}
+template <int... Q>
+__device__ __forceinline__ void all(const LevelView &F, const double *__restrict__ Af, int fi, int fj, int fk,
+				    double (&acc)[27], std::integer_sequence<int, Q...>) {
+	(cube<Q>(F, Af, fi, fj, fk, acc, std::make_integer_sequence<int, 8>()), ...);
+}
+}  // namespace rat
+
+__global__ __launch_bounds__(256) void k_rat_interior(LevelView F, LevelView C, const double *__restrict__ Af,
+						      double *__restrict__ Ac) {
+	const int I = 1 + (int)(blockIdx.x * 64 + threadIdx.x), Jy = 1 + (int)(blockIdx.y * 4 + threadIdx.y), K = 1 + (int)blockIdx.z;
+	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1;
+	if (I >= lastx || Jy >= lasty) return;
+	double acc[27];
+#pragma unroll
+	for (int e = 0; e < 27; e++) acc[e] = 0.0;
+	rat::all(F, Af, 2 * I, 2 * Jy, 2 * K, acc, std::make_integer_sequence<int, 8>());
+	double *Acr = Ac + aidx3(C, 0, I, Jy, K);
+#pragma unroll
+	for (int e = 0; e < 27; e++) Acr[e * C.px] = acc[e];
+}
+
+// The coarse points k_rat_interior leaves out, enumerated densely: those
+// with K in the z boundary set {0} u [lastz, C.o), then (K interior) those
+// with J in the y set, then (K, J interior) those with I in the x set.
+struct RatEdges {
+	int nbx, nby, nbz;     // sizes of the boundary index sets
+	long long na, nb, nc;  // points in the three groups
+};
+__host__ __device__ inline int edge_index(int q, int last) { return q == 0 ? 0 : last + q - 1; }
+__host__ __device__ inline RatEdges rat_edges(const LevelView &F, const LevelView &C) {
+	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
+	RatEdges r;
+	r.nbx = lastx > 1 ? 1 + C.m - lastx : C.m;
+	r.nby = lasty > 1 ? 1 + C.n - lasty : C.n;
+	r.nbz = lastz > 1 ? 1 + C.o - lastz : C.o;
+	const int ix = C.m - r.nbx, iy = C.n - r.nby, iz = C.o - r.nbz;  // interior counts
+	r.na = (long long)r.nbz * C.m * C.n;
+	r.nb = (long long)iz * r.nby * C.m;
+	r.nc = (long long)iz * iy * r.nbx;
+	(void)ix;
+	return r;
+}
+
+__global__ __launch_bounds__(256) void k_rat_boundary(LevelView F, LevelView C, const double *__restrict__ Af,
+						      double *__restrict__ Ac, RatEdges E) {
+	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
+	const bool allx = E.nbx == C.m, ally = E.nby == C.n, allz = E.nbz == C.o;
+	int I, Jy, K;
+	if (t < E.na) {
+		I = (int)(t % C.m); Jy = (int)((t / C.m) % C.n);
+		const int q = (int)(t / ((long long)C.m * C.n));
+		K = allz ? q : edge_index(q, lastz);
+	} else if ((t -= E.na) < E.nb) {
+		I = (int)(t % C.m);
+		const int q = (int)((t / C.m) % E.nby);
+		Jy = ally ? q : edge_index(q, lasty);
+		K = 1 + (int)(t / ((long long)C.m * E.nby));
+	} else if ((t -= E.nb) < E.nc) {
+		const int q = (int)(t % E.nbx);
+		I = allx ? q : edge_index(q, lastx);
+		Jy = 1 + (int)((t / E.nbx) % (C.n - E.nby));
+		K = 1 + (int)(t / ((long long)E.nbx * (C.n - E.nby)));
+	} else {
+		return;
+	}
+	rat_row_point(F, C, Af, Ac, I, Jy, K);
 }
 
 void launch_rat(hipStream_t s, const LevelView &F, const LevelView &C, const double *Af, double *Ac) {
-	const long long total = (long long)C.m * C.n * C.o * 27;
-	hipLaunchKernelGGL(k_rat, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, F, C, Af, Ac);
+	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
+	if (lastx > 1 && lasty > 1 && lastz > 1) {
+		dim3 gi((lastx - 1 + 63) / 64, (lasty - 1 + 3) / 4, lastz - 1), block(64, 4, 1);
+		hipLaunchKernelGGL(k_rat_interior, gi, block, 0, s, F, C, Af, Ac);
+		HIPCHECK_LAUNCH();
+	}
+	const RatEdges E = rat_edges(F, C);
+	const long long nbnd = E.na + E.nb + E.nc;
+	hipLaunchKernelGGL(k_rat_boundary, dim3((unsigned)((nbnd + 255) / 256)), dim3(256), 0, s, F, C, Af, Ac, E);
 	HIPCHECK_LAUNCH();
 }
 
