@@ -31,6 +31,7 @@ def lib():
         sigs = {
             "orc_hierarchy": (_i, [_i, _i, _i, _I]),
             "orc_gen_operator": (None, [_i, _u64, _i, _i, _i, _D]),
+            "orc_gen_operator_planes": (None, [_i, _u64, _i, _i, _i, _i, _i, _D]),
             "orc_gen_rhs": (None, [_u64, _i, _i, _i, _D]),
             "orc_gs_lex": (None, [_i, _i, _i, _D, _D, _D, _i]),
             "orc_gs_colour8": (None, [_i, _i, _i, _D, _D, _D, _i]),
@@ -92,6 +93,13 @@ def hierarchy(m, n, o):
 def gen_operator(family, seed, m, n, o):
     A = np.zeros((o, n, m, 27), dtype=np.float64)
     lib().orc_gen_operator(family, seed, m, n, o, _p(A))
+    return A
+
+
+def gen_operator_planes(family, seed, m, n, o, k0, k1):
+    """Planes k0..k1-1 of gen_operator(family, seed, m, n, o)."""
+    A = np.zeros((k1 - k0, n, m, 27), dtype=np.float64)
+    lib().orc_gen_operator_planes(family, seed, m, n, o, k0, k1, _p(A))
     return A
 
 

@@ -43,6 +43,14 @@ void orc_gen_operator(int family, uint64_t seed, int m, int n, int o, double *A2
 		pgmg_row27(family, seed, m, n, o, i, j, k, A27 + 27 * IDX(i, j, k, m, n));
 }
 
+/* Planes k0 <= k < k1 of the operator of an m x n x o grid, into
+ * A27[(k-k0), j, i, 27]: checks large grids a few planes at a time. */
+void orc_gen_operator_planes(int family, uint64_t seed, int m, int n, int o, int k0, int k1, double *A27) {
+	int i, j, k;
+	for (k = k0; k < k1; k++) for (j = 0; j < n; j++) for (i = 0; i < m; i++)
+		pgmg_row27(family, seed, m, n, o, i, j, k, A27 + 27 * IDX(i, j, k - k0, m, n));
+}
+
 void orc_gen_rhs(uint64_t seed, int m, int n, int o, double *b) {
 	int i, j, k;
 	for (k = 0; k < o; k++) for (j = 0; j < n; j++) for (i = 0; i < m; i++)

@@ -26,6 +26,7 @@ int orc_hierarchy(int m, int n, int o, int *dims /* 3*ORC_MAX_LEVELS */);
 
 /* Synthetic problems (include/pgmg_problems.h). */
 void orc_gen_operator(int family, uint64_t seed, int m, int n, int o, double *A27);
+void orc_gen_operator_planes(int family, uint64_t seed, int m, int n, int o, int k0, int k1, double *A27);
 void orc_gen_rhs(uint64_t seed, int m, int n, int o, double *b);
 
 /* Level kernels, FP64. */

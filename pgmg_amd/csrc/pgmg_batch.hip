@@ -486,7 +486,8 @@ int pgmg_batch_get_x(pgmg_batch *b, int slot, double *x) {
 	if (!b || slot < 0 || slot >= b->max_slots || !x) return bfail(PGMG_ERR_ARG, "bad argument");
 	SlotHost &S = b->slots[slot];
 	if (S.state == ST_FREE) return bfail(PGMG_ERR_ARG, "slot %d is free", slot);
-	BHIP(hipStreamSynchronize(b->setup));
+	// a slot still in set-up waits for its own set-up only, not for later admissions
+	if (S.state == ST_SETUP) BHIP(hipEventSynchronize(S.ready));
 	const LevelView &L = S.dev.lv[0];
 	const long long N = (long long)L.m * L.n * L.o;
 	double *stage = nullptr;

@@ -1501,30 +1501,22 @@ __device__ __forceinline__ void psupport(int J, int M, int &lo, int &hi, int &c)
 	else { lo = j2 == 0 ? 0 : j2 - 1; hi = j2 == M - 2 ? M - 1 : j2 + 2; c = j2; }
 }
 
-// Galerkin coefficients Ac(I, I+d) of one coarse point I, all 27 entries.
+// Galerkin coefficient Ac(I, I+d) of one coarse point I and one entry d.
 // The reference (region.cc:1008-1192) takes each fine point f in cube-scan
 // order, forms At[J] = sum_g 0.125*Af(f,g)*P(g,J) over g in f's box in
 // (k,j,i) order (A_red), scales At by f's restriction weight (A_mul/A_sca)
-// and adds it to the coarse rows of f (A_add).  Here each entry (I,J) visits
-// the same f in the same order and forms each inner sum the same way; f's 27
-// coefficients are loaded once for all J.
-__device__ __noinline__ void rat_row_point(const LevelView &F, const LevelView &C, const double *__restrict__ Af,
-					     double *__restrict__ Ac, int I, int Jy, int K) {
-	// interpolation supports of the coarse neighbours I-1..I+1 per axis
-	int lx[3], hx[3], cx_[3], ly[3], hy[3], cy_[3], lz[3], hz[3], cz_[3];
-	bool okx[3], oky[3], okz[3];
-#pragma unroll
-	for (int d = 0; d < 3; d++) {
-		const int JI = I + d - 1, JJ = Jy + d - 1, JK = K + d - 1;
-		okx[d] = JI >= 0 && JI < C.m; oky[d] = JJ >= 0 && JJ < C.n; okz[d] = JK >= 0 && JK < C.o;
-		lx[d] = hx[d] = cx_[d] = ly[d] = hy[d] = cy_[d] = lz[d] = hz[d] = cz_[d] = 0;
-		if (okx[d]) psupport(JI, F.m, lx[d], hx[d], cx_[d]);
-		if (oky[d]) psupport(JJ, F.n, ly[d], hy[d], cy_[d]);
-		if (okz[d]) psupport(JK, F.o, lz[d], hz[d], cz_[d]);
-	}
-	double acc[27];
-#pragma unroll
-	for (int e = 0; e < 27; e++) acc[e] = 0.0;
+// and adds it to the coarse rows of f (A_add).  Here the sum for one (I,J)
+// visits the same f in the same order, and forms each inner sum the same way.
+__device__ __forceinline__ double rat_entry(const LevelView &F, const LevelView &C, const double *__restrict__ Af,
+					    int I, int Jy, int K, int eJ) {
+	const int dI = eJ % 3 - 1, dJ = (eJ / 3) % 3 - 1, dK = eJ / 9 - 1;
+	const int JI = I + dI, JJ = Jy + dJ, JK = K + dK;
+	double acc = 0.0;
+	if (JI < 0 || JI >= C.m || JJ < 0 || JJ >= C.n || JK < 0 || JK >= C.o) return acc;
+	int plx, phx, pcx, ply, phy, pcy, plz, phz, pcz;
+	psupport(JI, F.m, plx, phx, pcx);
+	psupport(JJ, F.n, ply, phy, pcy);
+	psupport(JK, F.o, plz, phz, pcz);
 	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
 	const bool evx = !(F.m & 1), evy = !(F.n & 1), evz = !(F.o & 1);
 	for (int cz = 0; cz < 2; cz++) {
@@ -1539,71 +1531,63 @@ __device__ __noinline__ void rat_row_point(const LevelView &F, const LevelView &
 				const int ic = I - 1 + cx;
 				if (ic < 0 || ic > lastx) continue;
 				const bool sx = evx && ic == lastx;
-				for (int lzz = cz ? 0 : 1; lzz < 2; lzz++) {
-					const int fk = 2 * kc + lzz;
+				for (int lz = cz ? 0 : 1; lz < 2; lz++) {
+					const int fk = 2 * kc + lz;
 					if (fk >= F.o) continue;
 					double wz;
-					if (lzz == 0) wz = 1.0;
+					if (lz == 0) wz = 1.0;
 					else if (cz == 0) wz = sz ? 1.0 : 0.5;
 					else { if (sz) continue; wz = 0.5; }
-					for (int lyy = cy ? 0 : 1; lyy < 2; lyy++) {
-						const int fj = 2 * jc + lyy;
+					const int gzl = max(plz, fk - 1), gzh = min(phz, min(fk + 2, F.o));
+					if (gzl >= gzh) continue;
+					for (int ly = cy ? 0 : 1; ly < 2; ly++) {
+						const int fj = 2 * jc + ly;
 						if (fj >= F.n) continue;
 						double wy;
-						if (lyy == 0) wy = 1.0;
+						if (ly == 0) wy = 1.0;
 						else if (cy == 0) wy = sy ? 1.0 : 0.5;
 						else { if (sy) continue; wy = 0.5; }
-						for (int lxx = cx ? 0 : 1; lxx < 2; lxx++) {
-							const int fi = 2 * ic + lxx;
+						const int gyl = max(ply, fj - 1), gyh = min(phy, min(fj + 2, F.n));
+						if (gyl >= gyh) continue;
+						for (int lx = cx ? 0 : 1; lx < 2; lx++) {
+							const int fi = 2 * ic + lx;
 							if (fi >= F.m) continue;
 							double wx;
-							if (lxx == 0) wx = 1.0;
+							if (lx == 0) wx = 1.0;
 							else if (cx == 0) wx = sx ? 1.0 : 0.5;
 							else { if (sx) continue; wx = 0.5; }
-							const double wr = wx * wy * wz;
+							const int gxl = max(plx, fi - 1), gxh = min(phx, min(fi + 2, F.m));
+							if (gxl >= gxh) continue;
 							const double *Afr = Af + aidx3(F, 0, fi, fj, fk);
-							double a[27];
-#pragma unroll
-							for (int e = 0; e < 27; e++) a[e] = Afr[e * F.px];
-#pragma unroll
-							for (int eJ = 0; eJ < 27; eJ++) {
-								const int dx = eJ % 3, dy = (eJ / 3) % 3, dz = eJ / 9;
-								if (!(okx[dx] && oky[dy] && okz[dz])) continue;
-								const int gxl = max(lx[dx], fi - 1), gxh = min(hx[dx], min(fi + 2, F.m));
-								const int gyl = max(ly[dy], fj - 1), gyh = min(hy[dy], min(fj + 2, F.n));
-								const int gzl = max(lz[dz], fk - 1), gzh = min(hz[dz], min(fk + 2, F.o));
-								if (gxl >= gxh || gyl >= gyh || gzl >= gzh) continue;
-								double inner = 0.0;
-								for (int kk = gzl; kk < gzh; kk++) {
-									const double kfac = kk == cz_[dz] ? 1.0 : 0.5;
-									for (int jj = gyl; jj < gyh; jj++) {
-										const double jkfac = jj == cy_[dy] ? kfac : 0.5 * kfac;
-										for (int ii = gxl; ii < gxh; ii++) {
-											const int ef = (kk - fk + 1) * 9 + (jj - fj + 1) * 3 + (ii - fi + 1);
-											inner = inner + (0.125 * a[ef]) * (ii == cx_[dx] ? jkfac : 0.5 * jkfac);
-										}
+							double inner = 0.0;
+							for (int kk = gzl; kk < gzh; kk++) {
+								const double kfac = kk == pcz ? 1.0 : 0.5;
+								for (int jj = gyl; jj < gyh; jj++) {
+									const double jkfac = jj == pcy ? kfac : 0.5 * kfac;
+									for (int ii = gxl; ii < gxh; ii++) {
+										const int ef = (kk - fk + 1) * 9 + (jj - fj + 1) * 3 + (ii - fi + 1);
+										inner = inner + (0.125 * Afr[ef * F.px]) * (ii == pcx ? jkfac : 0.5 * jkfac);
 									}
 								}
-								acc[eJ] = acc[eJ] + inner * wr;
 							}
+							const double wr = wx * wy * wz;
+							acc = acc + inner * wr;
 						}
 					}
 				}
 			}
 		}
 	}
-	double *Acr = Ac + aidx3(C, 0, I, Jy, K);
-#pragma unroll
-	for (int e = 0; e < 27; e++) Acr[e * C.px] = acc[e];
+	return acc;
 }
 
 // Interior coarse points (1 <= I < lastx, likewise J, K: every cube of I's
 // restriction full, every fine box and interpolation support unclipped): the
-// terms k_rat_row visits are fixed by the offsets alone, so they are
+// terms rat_entry visits are fixed by the offsets alone, so they are
 // expanded at compile time.  For fine point f = 2I + df (df per axis -1, 0,
 // 1 in the cube-scan order) and entry J = I + d, the term of box entry ef
 // (offset eg) enters with P weight pw(df+eg, 2d): 1 at J's fine point, 1/2
-// next to it, else absent; the products and sums are those of k_rat_row in
+// next to it, else absent; the products and sums are those of rat_entry in
 // the same order.
 namespace rat {
 __host__ __device__ constexpr double pw(int dg, int c) {
@@ -1712,9 +1696,13 @@ __host__ __device__ inline RatEdges rat_edges(const LevelView &F, const LevelVie
 	return r;
 }
 
+// One thread per (boundary point, entry): small levels have few interior
+// points, so the boundary work is spread as widely as possible.
 __global__ __launch_bounds__(256) void k_rat_boundary(LevelView F, LevelView C, const double *__restrict__ Af,
 						      double *__restrict__ Ac, RatEdges E) {
-	long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	const long long tt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	const int eJ = (int)(tt % 27);
+	long long t = tt / 27;
 	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
 	const bool allx = E.nbx == C.m, ally = E.nby == C.n, allz = E.nbz == C.o;
 	int I, Jy, K;
@@ -1735,7 +1723,7 @@ __global__ __launch_bounds__(256) void k_rat_boundary(LevelView F, LevelView C, 
 	} else {
 		return;
 	}
-	rat_row_point(F, C, Af, Ac, I, Jy, K);
+	Ac[aidx3(C, eJ, I, Jy, K)] = rat_entry(F, C, Af, I, Jy, K, eJ);
 }
 
 void launch_rat(hipStream_t s, const LevelView &F, const LevelView &C, const double *Af, double *Ac) {
@@ -1746,7 +1734,7 @@ void launch_rat(hipStream_t s, const LevelView &F, const LevelView &C, const dou
 		HIPCHECK_LAUNCH();
 	}
 	const RatEdges E = rat_edges(F, C);
-	const long long nbnd = E.na + E.nb + E.nc;
+	const long long nbnd = (E.na + E.nb + E.nc) * 27;
 	hipLaunchKernelGGL(k_rat_boundary, dim3((unsigned)((nbnd + 255) / 256)), dim3(256), 0, s, F, C, Af, Ac, E);
 	HIPCHECK_LAUNCH();
 }
@@ -1918,9 +1906,94 @@ __global__ __launch_bounds__(256) void k_generate(LevelView L, int family, uint6
 	r[p] = pgmg_rhs(seed, L.m, L.n, L.o, i, j, k);
 }
 
+// Q1 family: a point's row sums element matrices of its 8 cells, and each
+// cell tensor (two 10^u series, a rotation) is shared by 8 points.  A block
+// owns a 64 x 4 column of points over kz planes and keeps the two cell planes
+// under the current point plane in LDS (struct of arrays, 6 tensor entries),
+// so each tensor is formed about once instead of 8 times.  The row is summed
+// exactly as pgmg_row27 sums it: cells in (k,j,i) order, local node lb
+// ascending, each entry from pgmg_q1_entry with the same operation order, so
+// the rows are bit-identical.
+constexpr int kGenCx = 65, kGenCy = 5, kGenPlane = kGenCx * kGenCy;
+
+__device__ __forceinline__ void gen_cell_plane(double *Kp, uint64_t seed, int m, int n, int o, int i0, int j0, int ck) {
+	const int t0 = threadIdx.y * 64 + threadIdx.x;
+	if (ck > o) return;
+	for (int t = t0; t < kGenPlane; t += 256) {
+		const int ci = i0 + t % kGenCx, cj = j0 + t / kGenCx;
+		if (ci > m || cj > n) continue;
+		double K[6];
+		pgmg_cell_tensor(seed, m, n, ci, cj, ck, K);
+#pragma unroll
+		for (int e = 0; e < 6; e++) Kp[e * kGenPlane + t] = K[e];
+	}
+}
+
+template <int LA>
+__device__ __forceinline__ void gen_q1_cell(const double *Kp, int c, int i, int j, int k, int m, int n, int o,
+					    double (&row)[27]) {
+	PGMG_NO_CONTRACT
+	constexpr int dc0 = 1 - (LA & 1), dc1 = 1 - ((LA >> 1) & 1), dc2 = 1 - ((LA >> 2) & 1);
+	double K[6];
+#pragma unroll
+	for (int e = 0; e < 6; e++) K[e] = Kp[e * kGenPlane + c];
+#pragma unroll
+	for (int lb = 0; lb < 8; lb++) {
+		const int bi = i + dc0 - 1 + (lb & 1), bj = j + dc1 - 1 + ((lb >> 1) & 1), bk = k + dc2 - 1 + ((lb >> 2) & 1);
+		if (bi < 0 || bi >= m || bj < 0 || bj >= n || bk < 0 || bk >= o) continue;
+		const int idx = (dc2 + ((lb >> 2) & 1)) * 9 + (dc1 + ((lb >> 1) & 1)) * 3 + (dc0 + (lb & 1));
+		row[idx] = row[idx] + pgmg_q1_entry(K, LA, lb);
+	}
+}
+
+__global__ __launch_bounds__(256) void k_generate_q1(LevelView L, uint64_t seed, int kz, double *__restrict__ A,
+						     double *__restrict__ r) {
+	__shared__ double Ks[2][6 * kGenPlane];
+	const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4;
+	const int kb = blockIdx.z * kz, ke = min(L.o, kb + kz);
+	const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+	const bool in = i < L.m && j < L.n;
+	gen_cell_plane(Ks[kb & 1], seed, L.m, L.n, L.o, i0, j0, kb);
+	for (int k = kb; k < ke; k++) {
+		gen_cell_plane(Ks[(k + 1) & 1], seed, L.m, L.n, L.o, i0, j0, k + 1);
+		__syncthreads();
+		if (in) {
+			double row[27];
+#pragma unroll
+			for (int e = 0; e < 27; e++) row[e] = 0.0;
+			const double *lo = Ks[k & 1], *hi = Ks[(k + 1) & 1];
+			const int c = threadIdx.y * kGenCx + threadIdx.x;
+			// cells (ci, cj, ck) = (i + dc0, j + dc1, k + dc2) in (k,j,i) order
+			gen_q1_cell<7>(lo, c, i, j, k, L.m, L.n, L.o, row);
+			gen_q1_cell<6>(lo, c + 1, i, j, k, L.m, L.n, L.o, row);
+			gen_q1_cell<5>(lo, c + kGenCx, i, j, k, L.m, L.n, L.o, row);
+			gen_q1_cell<4>(lo, c + kGenCx + 1, i, j, k, L.m, L.n, L.o, row);
+			gen_q1_cell<3>(hi, c, i, j, k, L.m, L.n, L.o, row);
+			gen_q1_cell<2>(hi, c + 1, i, j, k, L.m, L.n, L.o, row);
+			gen_q1_cell<1>(hi, c + kGenCx, i, j, k, L.m, L.n, L.o, row);
+			gen_q1_cell<0>(hi, c + kGenCx + 1, i, j, k, L.m, L.n, L.o, row);
+			double *Ap = A + aidx3(L, 0, i, j, k);
+#pragma unroll
+			for (int e = 0; e < 27; e++) Ap[e * L.px] = row[e];
+			r[idx3(L, i, j, k)] = pgmg_rhs(seed, L.m, L.n, L.o, i, j, k);
+		}
+		__syncthreads();
+	}
+}
+
 void launch_generate(hipStream_t s, const LevelView &L, int family, uint64_t seed, double *A, double *r) {
-	dim3 grid((L.m + 63) / 64, (L.n + 3) / 4, L.o), block(64, 4, 1);
-	hipLaunchKernelGGL(k_generate, grid, block, 0, s, L, family, seed, A, r);
+	dim3 block(64, 4, 1);
+	if (family == PGMG_FAM_Q1_ANISO) {
+		// planes per block: as many as keep >= 2048 blocks, at most 8
+		const long long cols = (long long)((L.m + 63) / 64) * ((L.n + 3) / 4);
+		int kz = 8;
+		while (kz > 1 && cols * ((L.o + kz - 1) / kz) < 2048) kz >>= 1;
+		dim3 grid((L.m + 63) / 64, (L.n + 3) / 4, (L.o + kz - 1) / kz);
+		hipLaunchKernelGGL(k_generate_q1, grid, block, 0, s, L, seed, kz, A, r);
+	} else {
+		dim3 grid((L.m + 63) / 64, (L.n + 3) / 4, L.o);
+		hipLaunchKernelGGL(k_generate, grid, block, 0, s, L, family, seed, A, r);
+	}
 	HIPCHECK_LAUNCH();
 }
 

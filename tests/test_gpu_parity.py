@@ -169,6 +169,29 @@ def test_device_generator_matches_host(family):
     h.close()
 
 
+@pytest.mark.parametrize("shape,kz", [((65, 255, 130), 8), ((70, 255, 70), 4), ((129, 9, 21), 1)],
+                         ids=["kz8", "kz4", "kz1"])
+def test_device_q1_generator_plane_blocks(shape, kz):
+    """The Q1 generator shares cell tensors across a block's planes (kz per
+    block, set by the grid size): partial x/y tiles and a partial last plane
+    block, checked on the planes around every block edge it has."""
+    m, n, o = shape
+    cols = ((m + 63) // 64) * ((n + 3) // 4)
+    k = 8
+    while k > 1 and cols * ((o + k - 1) // k) < 2048:
+        k //= 2
+    assert k == kz
+    say("case", shape, "kz", kz)
+    h = P.Hierarchy(m, n, o)
+    h.generate_problem(P.FAM_Q1_ANISO, 515)
+    A = h.get_operator27(0)
+    edges = sorted({0, 1, kz - 1, kz, kz + 1, o - kz - 1, o - 2, o - 1} & set(range(o)))
+    for k0 in edges:
+        assert np.array_equal(A[k0:k0 + 1], O.gen_operator_planes(P.FAM_Q1_ANISO, 515, m, n, o, k0, k0 + 1)), k0
+    assert np.array_equal(h.get_rhs(0), O.gen_rhs(515, m, n, o))
+    h.close()
+
+
 # Shapes chosen to run every lane layout of the row kernels (too large for
 # the single-workgroup small-level path): several short rows per wavefront
 # with and without the 2^k+1 point out, one row per wavefront, and long rows

@@ -266,3 +266,19 @@ def test_lu_matches_dense_solve_and_fallback_rule():
     # singular (all zero) matrix: the reference falls back to Gauss-Seidel
     ok0, _, _ = O.lu_factor(np.zeros((o, n, m, 27)))
     assert not ok0
+
+
+@pytest.mark.parametrize("family", [1, 2, 3])
+def test_generator_planes_match_whole_grid(family):
+    """The plane-range generator returns the same rows as the whole-grid one;
+    the Q1 operator is symmetric with zero row sums away from the boundary
+    (a sum of element stiffness matrices of constant tensors)."""
+    m, n, o = 11, 9, 8
+    A = O.gen_operator(family, 17, m, n, o)
+    for k0, k1 in [(0, 3), (2, 8), (7, 8)]:
+        assert np.array_equal(O.gen_operator_planes(family, 17, m, n, o, k0, k1), A[k0:k1])
+    if family == 3:
+        D = dense(A)
+        np.testing.assert_allclose(D, D.T, rtol=0, atol=1e-12)
+        interior = A[1:-1, 1:-1, 1:-1].sum(axis=-1)
+        np.testing.assert_allclose(interior, 0.0, atol=1e-12)
