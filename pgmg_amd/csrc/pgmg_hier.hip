@@ -130,7 +130,9 @@ struct ProfScope {
 		if (h->prof_used + 2 > h->prof_ev.size()) {
 			for (int q = 0; q < 64; q++) {
 				hipEvent_t e;
-				if (hipEventCreate(&e) != hipSuccess) return;
+				// timing only: no system-scope fence, whose cache write-back
+				// would otherwise land between the timed launches
+				if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return;
 				h->prof_ev.push_back(e);
 			}
 		}

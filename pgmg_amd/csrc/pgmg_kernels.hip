@@ -93,47 +93,68 @@ template <typename V>
 __device__ __forceinline__ V ld_row(const void *base, uint32_t off) {
 	return *(const V *)((const char *)base + off);
 }
+// Raw buffer descriptor word 3 for gfx950 (no format conversion, no
+// swizzle) and the cache-policy bits of buffer stores (bit 4 = sc1).
+constexpr int kBufferWord3 = 0x00020000;
+constexpr int kStoreSc1 = 16;
 template <typename V>
 __device__ __forceinline__ void st_row(void *base, uint32_t off, V v) {
-	*(V *)((char *)base + off) = v;
+	// Row results go out as buffer stores with the device-scope (sc1) cache
+	// policy: in these read-dominated kernels each written byte costs
+	// several times a read byte of HBM time, and this policy measured the
+	// cheapest (level-0 FP32 smoother -2 %, FP64 -2 %).  The descriptor spans
+	// 2 GiB from the row base; row offsets are far below that.
+	typedef unsigned u4v __attribute__((ext_vector_type(4)));
+	typedef unsigned u2v __attribute__((ext_vector_type(2)));
+	const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(base, 0, 0x7fffffff, kBufferWord3);
+	if constexpr (sizeof(V) == 16) {
+		u4v t; __builtin_memcpy(&t, &v, 16); __builtin_amdgcn_raw_buffer_store_b128(t, rs, (int)off, 0, kStoreSc1);
+	} else if constexpr (sizeof(V) == 8) {
+		u2v t; __builtin_memcpy(&t, &v, 8); __builtin_amdgcn_raw_buffer_store_b64(t, rs, (int)off, 0, kStoreSc1);
+	} else {
+		static_assert(sizeof(V) == 4, "row stores move 4, 8 or 16 bytes");
+		unsigned t; __builtin_memcpy(&t, &v, 4); __builtin_amdgcn_raw_buffer_store_b32(t, rs, (int)off, 0, kStoreSc1);
+	}
 }
 
-// Coefficient planes and r are read once per launch.  In FP64 their loads
-// carry the non-temporal hint, so the streamed planes do not evict from L2
-// the x rows that neighbouring grid rows read again (measured: fewer L2
-// misses in FP64, more in FP32, which keeps plain loads).
-#ifndef PGMG_NT_COEF
-#define PGMG_NT_COEF 1
-#endif
+// Coefficient planes and r are read once per launch: their loads carry the
+// non-temporal hint, so the streamed planes do not evict from L2 the x rows
+// that neighbouring grid rows read again (measured on the level-0 smoother:
+// FP32 113.8 -> 104.8 us per launch; x loads stay plain).
 typedef double d2_nt __attribute__((ext_vector_type(2)));
 template <typename V> __device__ __forceinline__ V ld_stream(const void *base, uint32_t off);
 template <> __device__ __forceinline__ float2 ld_stream<float2>(const void *base, uint32_t off) {
 	return *(const float2 *)((const char *)base + off);
 }
+typedef float f4_nt __attribute__((ext_vector_type(4)));
+template <> __device__ __forceinline__ float4 ld_stream<float4>(const void *base, uint32_t off) {
+	const f4_nt v = __builtin_nontemporal_load((const f4_nt *)((const char *)base + off));
+	return make_float4(v.x, v.y, v.z, v.w);
+}
 template <> __device__ __forceinline__ double2 ld_stream<double2>(const void *base, uint32_t off) {
-#if PGMG_NT_COEF
 	const d2_nt v = __builtin_nontemporal_load((const d2_nt *)((const char *)base + off));
 	return make_double2(v.x, v.y);
-#else
-	return *(const double2 *)((const char *)base + off);
-#endif
 }
 
 // Colour classes (0,cj,ck) and then (1,cj,ck) in one launch.  One wavefront
 // owns one grid row (j,k) with j = cj (mod 2), k = ck (mod 2); only that
 // row is both read and written by the launch, because every other row a
 // stencil reaches has the other parity in j or k.  The row is processed in
-// segments that advance by 126 points: lane l holds the pair (i0, i0+1),
-// i0 = s0 + 2l, read with one vector load per coefficient plane, so the
-// whole row is streamed once with no half-used cache lines.  Lanes 0..63
-// update the even point i0 (lane 63 is the lookahead: the first even point
-// of the next segment), then lanes 0..62 update the odd point i0+1, whose
-// two even neighbours are the lane's own new value and lane l+1's (shuffle).
-// The lookahead value is written at once and handed to lane 0 of the next
-// segment, which must not recompute it: by then lane 62's odd neighbour has
-// changed.  Per-point arithmetic and the colour order are those of
-// k_gs_colour, so the result is bit-identical.  j and k must be
-// wavefront-uniform.
+// segments of 128 points: lane l holds the pair (i0, i0+1), i0 = s0 + 2l,
+// read with one vector load per coefficient plane.  Lanes update the even
+// point i0, then the odd point i0+1, whose two even neighbours are the lane's
+// own new value and lane l+1's (shuffle).  Lane 63's right even neighbour
+// s0+128 (the first point of the next segment, or the last point of a row of
+// 128q+1 points) is updated by lane 63 itself from scalar loads, with the
+// evens; its value is handed to lane 0 of the next segment, which must not
+// recompute it: by then its odd neighbour s0+127 has changed.  A row of
+// 2^k+1 points thus takes 2^k/128 segments.  A segment's stores are issued
+// after the next segment's loads (one counter tracks loads and stores in
+// issue order, so a load issued after a store would wait for it); the only
+// loads they overtake are the next segment's lane-0 reads of s0-1 and s0,
+// which feed the even point that lane takes from the hand-over.  Per-point
+// arithmetic and the colour order are those of k_gs_colour, so the result is
+// bit-identical.  j and k must be wavefront-uniform.
 template <typename T>
 __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restrict__ A, T *__restrict__ x,
 					    const T *__restrict__ r, int j, int k) {
@@ -145,24 +166,32 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 	const T *rr_ = r + row;
 	T *xr = x + row;
 	T carry = (T)0;
-	for (int s0 = 0; s0 < L.m; s0 += 126) {
+	// the previous segment's results, stored after this segment's loads
+	bool pE = false, pO = false, pZ = false;
+	uint32_t pob = 0;
+	T pe = (T)0, po = (T)0, pz = (T)0;
+	for (int s0 = 0;; s0 += 128) {
+		const bool last = L.m - s0 <= 129;
 		const int i0 = s0 + 2 * lane;
 		const bool live = i0 < L.m;
 		const bool doE = live && !(lane == 0 && s0 > 0);
-		const bool doO = lane < 63 && i0 + 1 < L.m;
+		const bool doO = live && i0 + 1 < L.m;
+		const bool ext = lane == 63 && i0 + 2 < L.m;  // point out s0+128
 		const uint32_t ob = (uint32_t)i0 * (uint32_t)sizeof(T);
+		const uint32_t oz = ob + 2u * (uint32_t)sizeof(T);
 		// Streamed row by row (dj,dk): the even point's sum is formed as the
 		// loads arrive; the odd point's terms before its own row are summed
 		// at once, its own-row coefficients are kept, and its 12 terms after
 		// the own row are kept as products, to be added in box order once
 		// the new even values exist.  Products are the same roundings the
 		// in-order sum would form.
-		T ax = (T)0, ay = (T)0, ac_e = (T)1, ac_o = (T)1, a12 = (T)0, a14 = (T)0, rx = (T)0, ry = (T)0;
+		T ax = (T)0, ay = (T)0, az = (T)0, ac_e = (T)1, ac_o = (T)1, ac_z = (T)1, a12 = (T)0, a14 = (T)0;
+		T rx = (T)0, ry = (T)0, rz = (T)0;
 		T prod[12];
-		T xe = (T)0;
 		if (live) {
 			const V2 rr = ld_stream<V2>(rr_, ob);
 			rx = rr.x; ry = rr.y;
+			if (ext) rz = ld_row<T>(rr_, oz);
 #pragma unroll
 			for (int rq = 0; rq < 9; rq++) {
 				const int dj = rq % 3 - 1, dk = rq / 3 - 1;
@@ -191,41 +220,61 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 						prod[3 * (rq - 5) + 2] = ap.y * xp;
 					}
 				}
+				if (ext) {
+					// point out i0+2: neighbours i0+1 (xv.y), i0+2 (xp), i0+3
+					const T zm = ld_row<T>(Ar + (3 * rq) * px, oz);
+					const T z0 = ld_row<T>(Ar + (3 * rq + 1) * px, oz);
+					const T zp = ld_row<T>(Ar + (3 * rq + 2) * px, oz);
+					const T x3 = ld_row<T>(xq + 3, ob);
+					az = az + zm * xv.y;
+					if (rq == 4) ac_z = z0;
+					else az = az + z0 * xp;
+					az = az + zp * x3;
+				}
 			}
 		}
-		if (doE) {
-			// even point i0: every neighbour still held its old value
-			xe = (rx - ax) * ((T)1 / ac_e);
-			st_row<T>(xr, ob, xe);
-		} else if (live) {
-			xe = carry;  // lane 0: computed by lane 63 of the previous segment
-		}
-		// Shuffles run with every lane active; lanes past the row end hold 0.
-		const T xe_next = __shfl_down(xe, 1, 64);
+		if (pE) st_row<T>(xr, pob, pe);
+		if (pO) st_row<T>(xr + 1, pob, po);
+		if (pZ) st_row<T>(xr, pob + 2u * (uint32_t)sizeof(T), pz);
+		// even points: every neighbour still holds its old value
+		T xe = (T)0, xz = (T)0, xo = (T)0;
+		if (doE) xe = (rx - ax) * ((T)1 / ac_e);
+		else if (live) xe = carry;
+		if (ext) xz = (rz - az) * ((T)1 / ac_z);
+		// Shuffles run with every lane active; lanes past the row end hold 0,
+		// as does lane 63's point out when it lies in the halo.
+		const T xs = __shfl_down(xe, 1, 64);
+		const T xe_next = lane == 63 ? xz : xs;
 		if (doO) {
 			// odd point i0+1: its even neighbours in the own row are new
 			ay = ay + a12 * xe;
 			ay = ay + a14 * xe_next;
 #pragma unroll
 			for (int q2 = 0; q2 < 12; q2++) ay = ay + prod[q2];
-			st_row<T>(xr + 1, ob, (ry - ay) * ((T)1 / ac_o));
+			xo = (ry - ay) * ((T)1 / ac_o);
 		}
-		carry = __shfl(xe, 63, 64);
+		carry = __shfl(xz, 63, 64);
+		pE = doE; pO = doO; pZ = ext; pob = ob; pe = xe; po = xo; pz = xz;
+		if (last) break;
 	}
+	if (pE) st_row<T>(xr, pob, pe);
+	if (pO) st_row<T>(xr + 1, pob, po);
+	if (pZ) st_row<T>(xr, pob + 2u * (uint32_t)sizeof(T), pz);
 }
 
 // FP32 variant of gs_pair_row with four points per lane: lane l holds
 // i0 = s0 + 4l .. i0+3 (one 16-byte load per plane, i0 a multiple of 4 and
 // so 16-byte aligned).  Even points i0, i0+2 update first from old
 // neighbours; odd i0+1 then takes the lane's own new evens, odd i0+3 its own
-// i0+2 and lane l+1's i0+4 (shuffle).  A segment that does not reach the row
-// end advances by 252 points: lane 63's i0+3 waits for the next segment,
-// whose lane 0 re-reads that quad and updates only i0+3, with the carried
-// new i0+2.  The last segment covers up to 257 points: when one point is
-// left past lane 63's quad (a row of 2^k + 1 points), lane 63 also updates
-// that even point, from scalar loads, with the evens.  So a 257-point row
-// takes one segment.  Per-point arithmetic and order as in gs_point.  j and
-// k must be wavefront-uniform.
+// i0+2 and lane l+1's i0+4 (shuffle).  Segments are 256 points; lane 63's
+// right even neighbour s0+256 (the first point of the next segment, or the
+// last point of a row of 256q+1 points) is updated by lane 63 itself from
+// scalar loads, with the evens, and handed to lane 0 of the next segment,
+// which must not recompute it (its odd neighbour s0+255 has changed by
+// then).  A 257-point row takes one segment, 513 points two.  A segment's
+// stores are issued after the next segment's loads, as in gs_pair_row.
+// Per-point arithmetic and order as in gs_point.  j and k must be
+// wavefront-uniform.
 __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float *__restrict__ A, float *__restrict__ x,
 						const float *__restrict__ r, int j, int k) {
 	const int lane = threadIdx.x;
@@ -235,16 +284,19 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 	const float *rr_ = r + row;
 	float *xr = x + row;
 	float carry = 0.0f;
-	for (int s0 = 0;; s0 += 252) {
+	// the previous segment's results, stored after this segment's loads
+	bool pE0 = false, pE2 = false, pO1 = false, pO3 = false, pX = false;
+	uint32_t pob = 0;
+	float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f, v3 = 0.0f, v4 = 0.0f;
+	for (int s0 = 0;; s0 += 256) {
 		const int i0 = s0 + 4 * lane;
 		const bool last = L.m - s0 <= 257;
 		const bool live = i0 < L.m;
-		const bool first = lane == 0 && s0 > 0;  // only i0+3 is left to do
-		const bool doE0 = live && !first;
-		const bool doE2 = doE0 && i0 + 2 < L.m;
-		const bool doO1 = doE0 && i0 + 1 < L.m;
-		const bool doO3 = live && i0 + 3 < L.m && (lane < 63 || last);
-		const bool ext = last && lane == 63 && L.m - s0 == 257;  // even point i0+4 = m-1
+		const bool doE0 = live && !(lane == 0 && s0 > 0);  // lane 0 takes i0 from the hand-over
+		const bool doE2 = live && i0 + 2 < L.m;
+		const bool doO1 = live && i0 + 1 < L.m;
+		const bool doO3 = live && i0 + 3 < L.m;
+		const bool ext = lane == 63 && i0 + 4 < L.m;  // even point out i0+4 = s0+256
 		const uint32_t ob = (uint32_t)i0 * 4u;
 		// even sums e0 (i0), e2 (i0+2), ex (i0+4, ext); odd sums o1 (i0+1),
 		// o3 (i0+3) up to the own row; own-row coefficients kept; the odd
@@ -256,24 +308,25 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 		float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f, rx = 0.0f;
 		float p1[12], p3[12];
 		if (live) {
-			const float4 rv = ld_row<float4>(rr_, ob);
+			const float4 rv = ld_stream<float4>(rr_, ob);
 			r0 = rv.x; r1 = rv.y; r2 = rv.z; r3 = rv.w;
 			if (ext) rx = ld_row<float>(rr_, ob + 16u);
 #pragma unroll
 			for (int rq = 0; rq < 9; rq++) {
 				const int dj = rq % 3 - 1, dk = rq / 3 - 1;
 				const float *xq = xr + dj * px + dk * pxy;
-				const float4 am = ld_row<float4>(Ar + (3 * rq) * px, ob);
-				const float4 a0 = ld_row<float4>(Ar + (3 * rq + 1) * px, ob);
-				const float4 ap = ld_row<float4>(Ar + (3 * rq + 2) * px, ob);
+				const float4 am = ld_stream<float4>(Ar + (3 * rq) * px, ob);
+				const float4 a0 = ld_stream<float4>(Ar + (3 * rq + 1) * px, ob);
+				const float4 ap = ld_stream<float4>(Ar + (3 * rq + 2) * px, ob);
 				const float4 xv = ld_row<float4>(xq, ob);
 				const float xm = ld_row<float>(xq - 1, ob);
 				const float xp = doO3 ? ld_row<float>(xq + 4, ob) : 0.0f;
-				float bm = 0.0f, b0 = 0.0f, bp = 0.0f;
+				float bm = 0.0f, b0 = 0.0f, bp = 0.0f, x5 = 0.0f;
 				if (ext) {
 					bm = ld_row<float>(Ar + (3 * rq) * px, ob + 16u);
 					b0 = ld_row<float>(Ar + (3 * rq + 1) * px, ob + 16u);
 					bp = ld_row<float>(Ar + (3 * rq + 2) * px, ob + 16u);
+					x5 = ld_row<float>(xq + 5, ob);  // the halo point when i0+4 = m-1
 				}
 				if (rq == 4) {
 					e0 = e0 + am.x * xm;
@@ -281,7 +334,7 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 					e2 = e2 + am.z * xv.y;
 					e2 = e2 + ap.z * xv.w;
 					ex = ex + bm * xv.w;
-					ex = ex + bp * 0.0f;  // the halo point i = m
+					ex = ex + bp * x5;
 					c0 = a0.x; c1 = a0.y; c2 = a0.z; c3 = a0.w; cx = b0;
 					w1m = am.y; w1p = ap.y; w3m = am.w; w3p = ap.w;
 				} else {
@@ -293,7 +346,7 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 					e2 = e2 + ap.z * xv.w;
 					ex = ex + bm * xv.w;
 					ex = ex + b0 * xp;
-					ex = ex + bp * 0.0f;
+					ex = ex + bp * x5;
 					if (rq < 4) {
 						o1 = o1 + am.y * xv.x;
 						o1 = o1 + a0.y * xv.y;
@@ -312,41 +365,43 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 				}
 			}
 		}
-		float x0 = 0.0f, x2 = 0.0f, xx = 0.0f;
-		if (doE0) {
-			x0 = (r0 - e0) * (1.0f / c0);
-			st_row<float>(xr, ob, x0);
-			if (doE2) {
-				x2 = (r2 - e2) * (1.0f / c2);
-				st_row<float>(xr, ob + 8u, x2);
-			}
-			if (ext) {
-				xx = (rx - ex) * (1.0f / cx);
-				st_row<float>(xr, ob + 16u, xx);
-			}
-		} else if (first) {
-			x2 = carry;  // i0+2, updated by lane 63 of the previous segment
-		}
+		if (pE0) st_row<float>(xr, pob, v0);
+		if (pO1) st_row<float>(xr, pob + 4u, v1);
+		if (pE2) st_row<float>(xr, pob + 8u, v2);
+		if (pO3) st_row<float>(xr, pob + 12u, v3);
+		if (pX) st_row<float>(xr, pob + 16u, v4);
+		float x0 = 0.0f, x2 = 0.0f, xx = 0.0f, y1 = 0.0f, y3 = 0.0f;
+		if (doE0) x0 = (r0 - e0) * (1.0f / c0);
+		else if (live) x0 = carry;
+		if (doE2) x2 = (r2 - e2) * (1.0f / c2);
+		if (ext) xx = (rx - ex) * (1.0f / cx);
 		// shuffles with every lane active; lanes past the row end hold 0
 		const float xn = __shfl_down(x0, 1, 64);
-		const float x4 = lane == 63 ? xx : xn;  // lane 63: the extra point or the halo
+		const float x4 = lane == 63 ? xx : xn;  // lane 63: the point out, or the halo
 		if (doO1) {
 			o1 = o1 + w1m * x0;
 			o1 = o1 + w1p * x2;
 #pragma unroll
 			for (int q2 = 0; q2 < 12; q2++) o1 = o1 + p1[q2];
-			st_row<float>(xr, ob + 4u, (r1 - o1) * (1.0f / c1));
+			y1 = (r1 - o1) * (1.0f / c1);
 		}
 		if (doO3) {
 			o3 = o3 + w3m * x2;
 			o3 = o3 + w3p * x4;
 #pragma unroll
 			for (int q2 = 0; q2 < 12; q2++) o3 = o3 + p3[q2];
-			st_row<float>(xr, ob + 12u, (r3 - o3) * (1.0f / c3));
+			y3 = (r3 - o3) * (1.0f / c3);
 		}
+		pE0 = doE0; pE2 = doE2; pO1 = doO1; pO3 = doO3; pX = ext; pob = ob;
+		v0 = x0; v1 = y1; v2 = x2; v3 = y3; v4 = xx;
 		if (last) break;
-		carry = __shfl(x2, 63, 64);
+		carry = __shfl(xx, 63, 64);
 	}
+	if (pE0) st_row<float>(xr, pob, v0);
+	if (pO1) st_row<float>(xr, pob + 4u, v1);
+	if (pE2) st_row<float>(xr, pob + 8u, v2);
+	if (pO3) st_row<float>(xr, pob + 12u, v3);
+	if (pX) st_row<float>(xr, pob + 16u, v4);
 }
 
 // Vectors of ppl elements moved with one load or store.
@@ -371,9 +426,6 @@ __device__ __forceinline__ void ld_vec_stream(const T *base, uint32_t off, T (&o
 	const T *pv = reinterpret_cast<const T *>(&v);
 #pragma unroll
 	for (int t = 0; t < N; t++) out[t] = pv[t];
-}
-template <> __device__ __forceinline__ float4 ld_stream<float4>(const void *base, uint32_t off) {
-	return *(const float4 *)((const char *)base + off);
 }
 template <typename T, int N>
 __device__ __forceinline__ void st_vec(T *base, uint32_t off, const T (&v)[N]) {
@@ -990,17 +1042,37 @@ __device__ __forceinline__ double residual_row_pair(const LevelView &L, const T 
 		TR *resr = res ? res + row : nullptr;
 		TR *zxr = zx ? zx + row : nullptr;
 		// segments of 128 points; the last one takes up to 129, lane 63
-		// doing the odd point out from scalar loads
+		// doing the odd point out from scalar loads.  A segment's stores are
+		// issued after the next segment's loads, which then never wait for
+		// them (one counter tracks loads and stores in issue order).
+		bool pend = false, ptwo = false, pext = false;
+		uint32_t pobr = 0;
+		T p0 = (T)0, p1 = (T)0, p2 = (T)0;
+		auto flush = [&]() {
+			if (!pend) return;
+			if (resr) {
+				if (ptwo) { R2 o; o.x = (TR)p0; o.y = (TR)p1; st_row<R2>(resr, pobr, o); }
+				else st_row<TR>(resr, pobr, (TR)p0);
+				if (pext) st_row<TR>(resr, pobr + 2u * (uint32_t)sizeof(TR), (TR)p2);
+			}
+			if (zxr) {
+				if (ptwo) { R2 o; o.x = (TR)0; o.y = (TR)0; st_row<R2>(zxr, pobr, o); }
+				else st_row<TR>(zxr, pobr, (TR)0);
+				if (pext) st_row<TR>(zxr, pobr + 2u * (uint32_t)sizeof(TR), (TR)0);
+			}
+		};
 		for (int s0 = 0;; s0 += 128) {
 			const bool last = L.m - s0 <= 129;
 			const int i0 = s0 + 2 * lane;
 			const bool ext = last && lane == 63 && L.m - s0 == 129;  // point i0+2 = m-1
-			if (i0 < L.m) {
-				const bool two = i0 + 1 < L.m;
-				const uint32_t ob = (uint32_t)i0 * (uint32_t)sizeof(T);
+			const bool live = i0 < L.m;
+			const bool two = i0 + 1 < L.m;
+			T ax = (T)0, ay = (T)0, az = (T)0, r0 = (T)0, r1 = (T)0, rz = (T)0;
+			const uint32_t ob = (uint32_t)i0 * (uint32_t)sizeof(T);
+			if (live) {
 				const V2 rr = ld_stream<V2>(rr_, ob);
-				const T rz = ext ? ld_row<T>(rr_, ob + 2u * (uint32_t)sizeof(T)) : (T)0;
-				T ax = (T)0, ay = (T)0, az = (T)0;
+				r0 = rr.x; r1 = rr.y;
+				if (ext) rz = ld_row<T>(rr_, ob + 2u * (uint32_t)sizeof(T));
 #pragma unroll
 				for (int rq = 0; rq < 9; rq++) {
 					const int dj = rq % 3 - 1, dk = rq / 3 - 1;
@@ -1024,25 +1096,20 @@ __device__ __forceinline__ double residual_row_pair(const LevelView &L, const T 
 						az = az + ld_row<T>(Ar + (3 * rq + 2) * px, oz) * (T)0;  // the halo point i = m
 					}
 				}
-				const T rv0 = rr.x - ax, rv1 = rr.y - ay, rv2 = rz - az;
-				if (resr) {
-					const uint32_t obr = (uint32_t)i0 * (uint32_t)sizeof(TR);
-					if (two) { R2 o; o.x = (TR)rv0; o.y = (TR)rv1; st_row<R2>(resr, obr, o); }
-					else st_row<TR>(resr, obr, (TR)rv0);
-					if (ext) st_row<TR>(resr, obr + 2u * (uint32_t)sizeof(TR), (TR)rv2);
-				}
-				if (zxr) {
-					const uint32_t obr = (uint32_t)i0 * (uint32_t)sizeof(TR);
-					if (two) { R2 o; o.x = (TR)0; o.y = (TR)0; st_row<R2>(zxr, obr, o); }
-					else st_row<TR>(zxr, obr, (TR)0);
-					if (ext) st_row<TR>(zxr, obr + 2u * (uint32_t)sizeof(TR), (TR)0);
-				}
+			}
+			flush();
+			if (live) {
+				const T rv0 = r0 - ax, rv1 = r1 - ay, rv2 = rz - az;
 				v2 = v2 + (double)rv0 * (double)rv0;
 				if (two) v2 = v2 + (double)rv1 * (double)rv1;
 				if (ext) v2 = v2 + (double)rv2 * (double)rv2;
+				p0 = rv0; p1 = rv1; p2 = rv2;
 			}
+			pend = live; ptwo = two; pext = ext;
+			pobr = (uint32_t)i0 * (uint32_t)sizeof(TR);
 			if (last) break;
 		}
+		flush();
 	}
 	return v2;
 }
@@ -1060,23 +1127,40 @@ __device__ __forceinline__ double residual_row_quad_f32(const LevelView &L, cons
 		const long long row = idx3(L, 0, j, k);
 		const float *Ar = A + arow0(L, j, k), *xr = x + row, *rr_ = r + row;
 		float *resr = res ? res + row : nullptr;
+		// a segment's stores are issued after the next segment's loads
+		int pnv = 0;
+		bool pext = false;
+		uint32_t pob = 0;
+		float q0 = 0.0f, q1 = 0.0f, q2 = 0.0f, q3 = 0.0f, qz = 0.0f;
+		auto flush = [&]() {
+			if (!resr || pnv == 0) return;
+			if (pnv == 4) st_row<float4>(resr, pob, make_float4(q0, q1, q2, q3));
+			else {
+				st_row<float>(resr, pob, q0);
+				if (pnv > 1) st_row<float>(resr, pob + 4u, q1);
+				if (pnv > 2) st_row<float>(resr, pob + 8u, q2);
+			}
+			if (pext) st_row<float>(resr, pob + 16u, qz);
+		};
 		for (int s0 = 0;; s0 += 256) {
 			const bool last = L.m - s0 <= 257;
 			const int i0 = s0 + 4 * lane;
 			const bool ext = last && lane == 63 && L.m - s0 == 257;  // point i0+4 = m-1
-			if (i0 < L.m) {
-				const int nv = min(4, L.m - i0);
-				const uint32_t ob = (uint32_t)i0 * 4u;
-				const float4 rr = ld_row<float4>(rr_, ob);
-				const float rz = ext ? ld_row<float>(rr_, ob + 16u) : 0.0f;
-				float a0s = 0.0f, a1s = 0.0f, a2s = 0.0f, a3s = 0.0f, azs = 0.0f;
+			const int nv = i0 < L.m ? min(4, L.m - i0) : 0;
+			const uint32_t ob = (uint32_t)i0 * 4u;
+			float a0s = 0.0f, a1s = 0.0f, a2s = 0.0f, a3s = 0.0f, azs = 0.0f;
+			float4 rr = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+			float rz = 0.0f;
+			if (nv > 0) {
+				rr = ld_stream<float4>(rr_, ob);
+				rz = ext ? ld_row<float>(rr_, ob + 16u) : 0.0f;
 #pragma unroll
 				for (int rq = 0; rq < 9; rq++) {
 					const int dj = rq % 3 - 1, dk = rq / 3 - 1;
 					const float *xq = xr + dj * px + dk * pxy;
-					const float4 am = ld_row<float4>(Ar + (3 * rq) * px, ob);
-					const float4 ac = ld_row<float4>(Ar + (3 * rq + 1) * px, ob);
-					const float4 ap = ld_row<float4>(Ar + (3 * rq + 2) * px, ob);
+					const float4 am = ld_stream<float4>(Ar + (3 * rq) * px, ob);
+					const float4 ac = ld_stream<float4>(Ar + (3 * rq + 1) * px, ob);
+					const float4 ap = ld_stream<float4>(Ar + (3 * rq + 2) * px, ob);
 					const float4 xv = ld_row<float4>(xq, ob);
 					const float xm = ld_row<float>(xq - 1, ob);
 					const float xp = nv == 4 ? ld_row<float>(xq + 4, ob) : 0.0f;
@@ -1098,24 +1182,21 @@ __device__ __forceinline__ double residual_row_quad_f32(const LevelView &L, cons
 						azs = azs + ld_row<float>(Ar + (3 * rq + 2) * px, ob + 16u) * 0.0f;  // halo
 					}
 				}
+			}
+			flush();
+			if (nv > 0) {
 				const float v0 = rr.x - a0s, v1 = rr.y - a1s, v2f = rr.z - a2s, v3 = rr.w - a3s, vz = rz - azs;
-				if (resr) {
-					if (nv == 4) st_row<float4>(resr, ob, make_float4(v0, v1, v2f, v3));
-					else {
-						st_row<float>(resr, ob, v0);
-						if (nv > 1) st_row<float>(resr, ob + 4u, v1);
-						if (nv > 2) st_row<float>(resr, ob + 8u, v2f);
-					}
-					if (ext) st_row<float>(resr, ob + 16u, vz);
-				}
 				v2 = v2 + (double)v0 * (double)v0;
 				if (nv > 1) v2 = v2 + (double)v1 * (double)v1;
 				if (nv > 2) v2 = v2 + (double)v2f * (double)v2f;
 				if (nv > 3) v2 = v2 + (double)v3 * (double)v3;
 				if (ext) v2 = v2 + (double)vz * (double)vz;
+				q0 = v0; q1 = v1; q2 = v2f; q3 = v3; qz = vz;
 			}
+			pnv = nv; pext = ext; pob = ob;
 			if (last) break;
 		}
+		flush();
 	}
 	return v2;
 }

@@ -195,10 +195,10 @@ def test_device_q1_generator_plane_blocks(shape, kz):
 # Shapes chosen to run every lane layout of the row kernels (too large for
 # the single-workgroup small-level path): several short rows per wavefront
 # with and without the 2^k+1 point out, one row per wavefront, and long rows
-# streamed in segments (FP64 above 129 points, FP32 above 257, with the point
-# out in the last FP32 segment at 509).
+# streamed in segments (FP64 above 129 points, FP32 above 257), with and
+# without a point out at the end of the last segment.
 ROW_SHAPES = [(33, 30, 20, 1), (40, 26, 20, 3), (65, 20, 14, 3), (129, 12, 10, 1), (131, 12, 10, 3),
-              (257, 8, 7, 1), (300, 8, 6, 3), (509, 5, 5, 1)]
+              (256, 9, 6, 1), (257, 8, 7, 1), (258, 7, 6, 3), (300, 8, 6, 3), (509, 5, 5, 1), (513, 5, 4, 3)]
 
 
 @pytest.mark.parametrize("shape", ROW_SHAPES, ids=lambda s: "%dx%dx%d" % s[:3])
