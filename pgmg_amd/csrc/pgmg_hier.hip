@@ -126,7 +126,7 @@ struct ProfScope {
 	bool on;
 	long long pts;
 	ProfScope(pgmg_hier *h_, int kc, int level, long long points) : h(h_), on(false), pts(points) {
-		if (h->prof_class != kc || h->prof_level != level || h->capturing) return;
+		if (kc == KC_NONE || h->prof_class != kc || h->prof_level != level || h->capturing) return;
 		if (h->prof_used + 2 > h->prof_ev.size()) {
 			for (int q = 0; q < 64; q++) {
 				hipEvent_t e;
@@ -172,20 +172,33 @@ template <typename T> T *res_scratch(pgmg_hier *h);
 template <> double *res_scratch<double>(pgmg_hier *h) { return h->res64; }
 template <> float *res_scratch<float>(pgmg_hier *h) { return h->res32; }
 
+// One workgroup smooths a small level when that beats per-class (or
+// per-hyperplane) launches: with x in LDS, for exact mode, and for forward
+// colour sweeps when each thread holds one point or one point pair.
 template <typename T>
-void op_gs(pgmg_hier *h, int level, int mode, int nsweeps) {
+bool gs_use_small(const pgmg_hier *h, const Level &L, int mode) {
+	const bool exact = h->opts.smoother == PGMG_SMOOTH_EXACT;
+	const bool lds = small_lds_bytes(L.v, (int)sizeof(T)) <= kSmallLdsBytes;
+	if (exact) return lds || L.points() <= kSmallExact;
+	if (mode == 0) return lds && (L.points() <= 1024 || small_pairs_per_pass(L.v) <= 1024);
+	return L.points() <= kSmallColour;
+}
+
+// Whether op_gs(..., from_zero) starts a forward FP32 colour sweep without
+// reading x (then x need not be cleared before it).
+bool gs_from_zero_ok(const pgmg_hier *h, int level, int mode) {
+	return h->opts.smoother != PGMG_SMOOTH_EXACT && mode == 0 && !gs_use_small<float>(h, h->lev[level], mode);
+}
+
+// from_zero: x is 0 on entry (and may not be cleared in memory); only with
+// gs_from_zero_ok.
+template <typename T>
+void op_gs(pgmg_hier *h, int level, int mode, int nsweeps, bool from_zero = false) {
 	Level &L = h->lev[level];
 	const bool exact = h->opts.smoother == PGMG_SMOOTH_EXACT;
 	h->gs_count += nsweeps;
 	if (nsweeps <= 0) return;
-	// One workgroup smooths a small level when that beats per-class (or
-	// per-hyperplane) launches: with x in LDS, for exact mode, and for forward
-	// colour sweeps when each thread holds one point or one point pair.
-	const bool lds = small_lds_bytes(L.v, (int)sizeof(T)) <= kSmallLdsBytes;
-	bool use_small;
-	if (exact) use_small = lds || L.points() <= kSmallExact;
-	else if (mode == 0) use_small = lds && (L.points() <= 1024 || small_pairs_per_pass(L.v) <= 1024);
-	else use_small = L.points() <= kSmallColour;
+	const bool use_small = gs_use_small<T>(h, L, mode);
 	if (use_small) {
 		ProfScope ps(h, KC_GS, level, L.points() * nsweeps * (mode ? 2 : 1));
 		launch_gs_small<T>(h->stream, L.v, lA<T>(L), lx<T>(L), lr<T>(L), nsweeps, mode, exact ? 1 : 0);
@@ -202,8 +215,12 @@ void op_gs(pgmg_hier *h, int level, int mode, int nsweeps) {
 				// forward: colour pairs (0,1), (2,3), (4,5), (6,7)
 				for (int q = 0; q < 4; q++) {
 					const int cj = q & 1, ck = q >> 1;
-					ProfScope ps(h, KC_GS, level, (long long)L.v.m * ((L.v.n - cj + 1) / 2) * ((L.v.o - ck + 1) / 2));
-					launch_gs_pair<T>(h->stream, L.v, lA<T>(L), lx<T>(L), lr<T>(L), q);
+					// a sweep from zero reads less x than a regular one: its
+					// launches stay out of the per-launch byte count
+					const bool fz = from_zero && s == 0;
+					ProfScope ps(h, fz ? KC_NONE : KC_GS, level,
+						     (long long)L.v.m * ((L.v.n - cj + 1) / 2) * ((L.v.o - ck + 1) / 2));
+					launch_gs_pair<T>(h->stream, L.v, lA<T>(L), lx<T>(L), lr<T>(L), q, fz);
 				}
 			} else {
 				// backward: colours 7..0
@@ -245,12 +262,13 @@ void op_solve_exact(pgmg_hier *h, int level, bool smooth) {
 	}
 }
 
-// multigrid::v_cycle, multigrid.cc:88-106.
+// multigrid::v_cycle, multigrid.cc:88-106.  top_zero: the level-0 x is 0 on
+// entry without having been cleared (the mixed-precision correction).
 template <typename T>
-void op_v_cycle(pgmg_hier *h, int mode) {
+void op_v_cycle(pgmg_hier *h, int mode, bool top_zero = false) {
 	const int L = h->nl;
 	for (int i = 0; i < L - 1; i++) {
-		op_gs<T>(h, i, mode, h->opts.v_down + i);
+		op_gs<T>(h, i, mode, h->opts.v_down + i, top_zero && i == 0);
 		op_restriction<T>(h, i + 1);
 	}
 	op_solve_exact<T>(h, L - 1, false);
@@ -300,7 +318,7 @@ int run_cycle(pgmg_hier *h, int mode, bool fp32) {
 			HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
 			h->capturing = true;
 			long long gc = h->gs_count;
-			if (fp32) op_v_cycle<float>(h, mode); else op_v_cycle<double>(h, mode);
+			if (fp32) op_v_cycle<float>(h, mode, gs_from_zero_ok(h, 0, mode)); else op_v_cycle<double>(h, mode);
 			h->gs_count = gc;
 			h->capturing = false;
 			HIP_TRY(hipStreamEndCapture(h->stream, &g));
@@ -315,7 +333,7 @@ int run_cycle(pgmg_hier *h, int mode, bool fp32) {
 		h->gs_count += per;
 		HIP_TRY(hipGraphLaunch(h->graph[mi][pi], h->stream));
 	} else {
-		if (fp32) op_v_cycle<float>(h, mode); else op_v_cycle<double>(h, mode);
+		if (fp32) op_v_cycle<float>(h, mode, gs_from_zero_ok(h, 0, mode)); else op_v_cycle<double>(h, mode);
 	}
 	return PGMG_OK;
 }
@@ -708,17 +726,19 @@ int pgmg_solve(pgmg_hier *h, double tol, int max_cycles, int *cycles, double *hi
 	int c = 0;
 	double e = 0;
 	for (;;) {
-		// mixed: the FP64 defect goes to r32 and the FP32 correction x32 is
-		// cleared in the same pass
-		rc = residual_norm(h, 0, mixed ? T0.r32 : nullptr, &e, mixed ? T0.x32 : nullptr);
+		// mixed: the FP64 defect goes to r32, and the FP32 correction x32 is
+		// cleared in the same pass unless the cycle's first sweep starts from
+		// zero without reading it
+		float *zero32 = mixed && !gs_from_zero_ok(h, 0, 0) ? T0.x32 : nullptr;
+		rc = residual_norm(h, 0, mixed ? T0.r32 : nullptr, &e, zero32);
 		if (rc) return rc;
 		if (hist) hist[c] = e;
 		if (std::isnan(e)) { if (cycles) *cycles = c; return fail(PGMG_ERR_NAN, "NaN residual after %d cycles", c); }
 		if (sqrt(e / bn) < tol) break;
 		if (c >= max_cycles) { if (cycles) *cycles = c; return fail(PGMG_ERR_NOT_CONVERGED, "not converged in %d cycles", c); }
 		if (mixed) {
-			// defect correction: r32 holds (float)(b - A x), x32 = 0; one
-			// FP32 V-cycle; x += x32.
+			// defect correction: r32 holds (float)(b - A x), x32 = 0 (in
+			// memory or by the first sweep); one FP32 V-cycle; x += x32.
 			rc = run_cycle(h, 0, true);
 			if (rc) return rc;
 			launch_add_f32_to_f64(h->stream, T0.v, T0.x32, T0.x);

@@ -155,7 +155,10 @@ enum { KC_NONE = 0, KC_GS = 1, KC_RESIDUAL = 2, KC_RESTRICT = 3, KC_INTERP = 4 }
 template <typename T> void launch_gs_colour(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int colour);
 // Forward Gauss-Seidel of the two colour classes (0,cj,ck) then (1,cj,ck)
 // in one launch, one wavefront per grid row.
-template <typename T> void launch_gs_pair(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int pair);
+// from_zero (FP32 only): the first sweep of a call that starts from x = 0;
+// x need not be cleared beforehand.
+template <typename T> void launch_gs_pair(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int pair,
+					  bool from_zero = false);
 // Gauss-Seidel on the hyperplane i+2j+4k = t.
 template <typename T> void launch_gs_plane(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int t);
 // nsweeps Gauss-Seidel sweeps of a small level by one workgroup, with x

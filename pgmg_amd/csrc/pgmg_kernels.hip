@@ -136,6 +136,14 @@ template <> __device__ __forceinline__ double2 ld_stream<double2>(const void *ba
 	return make_double2(v.x, v.y);
 }
 
+// First sweep from x = 0 (template flag Z of the FP32 row kernels): the rows of this colour pair and of the pairs
+// after it have not been updated yet in the sweep, so they still hold 0;
+// their x is not read (the caller need not clear x), and the terms see the
+// same +0.0 a cleared array would give.
+__device__ __forceinline__ bool zero_row(int cj, int ck, int dj, int dk) {
+	return ((cj ^ (dj & 1)) + 2 * (ck ^ (dk & 1))) >= cj + 2 * ck;
+}
+
 // Colour classes (0,cj,ck) and then (1,cj,ck) in one launch.  One wavefront
 // owns one grid row (j,k) with j = cj (mod 2), k = ck (mod 2); only that
 // row is both read and written by the launch, because every other row a
@@ -275,8 +283,10 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 // stores are issued after the next segment's loads, as in gs_pair_row.
 // Per-point arithmetic and order as in gs_point.  j and k must be
 // wavefront-uniform.
+template <bool Z = false>
 __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float *__restrict__ A, float *__restrict__ x,
 						const float *__restrict__ r, int j, int k) {
+	const int cj = j & 1, ck = k & 1;
 	const int lane = threadIdx.x;
 	const long long row = idx3(L, 0, j, k);
 	const long long px = L.px, pxy = L.pxy;
@@ -318,15 +328,20 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 				const float4 am = ld_stream<float4>(Ar + (3 * rq) * px, ob);
 				const float4 a0 = ld_stream<float4>(Ar + (3 * rq + 1) * px, ob);
 				const float4 ap = ld_stream<float4>(Ar + (3 * rq + 2) * px, ob);
-				const float4 xv = ld_row<float4>(xq, ob);
-				const float xm = ld_row<float>(xq - 1, ob);
-				const float xp = doO3 ? ld_row<float>(xq + 4, ob) : 0.0f;
+				const bool zr = Z && zero_row(cj, ck, dj, dk);
+				float4 xv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+				float xm = 0.0f, xp = 0.0f;
+				if (!zr) {
+					xv = ld_row<float4>(xq, ob);
+					xm = ld_row<float>(xq - 1, ob);
+					xp = doO3 ? ld_row<float>(xq + 4, ob) : 0.0f;
+				}
 				float bm = 0.0f, b0 = 0.0f, bp = 0.0f, x5 = 0.0f;
 				if (ext) {
 					bm = ld_row<float>(Ar + (3 * rq) * px, ob + 16u);
 					b0 = ld_row<float>(Ar + (3 * rq + 1) * px, ob + 16u);
 					bp = ld_row<float>(Ar + (3 * rq + 2) * px, ob + 16u);
-					x5 = ld_row<float>(xq + 5, ob);  // the halo point when i0+4 = m-1
+					if (!zr) x5 = ld_row<float>(xq + 5, ob);  // the halo point when i0+4 = m-1
 				}
 				if (rq == 4) {
 					e0 = e0 + am.x * xm;
@@ -447,7 +462,7 @@ __device__ __forceinline__ void st_vec(T *base, uint32_t off, const T (&v)[N]) {
 // at a row's last lane the point out (or the halo).  Only the wavefront's
 // rows are written by the launch, so each lane stores its points once, at
 // the end.  Per-point arithmetic and order as in gs_point.
-template <typename T, int PPL>
+template <typename T, int PPL, bool Z = false>
 __device__ __forceinline__ void gs_lanes(const LevelView &L, const T *__restrict__ A, T *__restrict__ x,
 					 const T *__restrict__ r, int cj, int ck, int q0, int nrows) {
 	const RowLanes W = row_lanes(L.m, (int)sizeof(T));
@@ -489,9 +504,15 @@ __device__ __forceinline__ void gs_lanes(const LevelView &L, const T *__restrict
 			T a[3][PPL], xv[PPL];
 #pragma unroll
 			for (int d = 0; d < 3; d++) ld_vec_stream<T, PPL>(Ab + (3 * rq + d) * px, oa, a[d]);
-			ld_vec<T, PPL>(xq, ox, xv);
-			const T xm = ld_row<T>(xq - 1, ox);
-			const T xp = needxp ? ld_row<T>(xq + PPL, ox) : (T)0;
+			T xm = (T)0, xp = (T)0;
+			if (Z && zero_row(cj, ck, dj, dk)) {
+#pragma unroll
+				for (int t = 0; t < PPL; t++) xv[t] = (T)0;
+			} else {
+				ld_vec<T, PPL>(xq, ox, xv);
+				xm = ld_row<T>(xq - 1, ox);
+				xp = needxp ? ld_row<T>(xq + PPL, ox) : (T)0;
+			}
 			T b[3] = {(T)0, (T)0, (T)0};
 			if (ext) {
 #pragma unroll
@@ -660,18 +681,16 @@ __device__ __forceinline__ double residual_lanes(const LevelView &L, const T *__
 
 // Forward colour-pair update of one row: four points per lane in FP32,
 // two in FP64 (whose 16-byte pair loads already fill the registers).
-template <typename T>
+// The first-sweep-from-zero form (Z) exists in FP32 only.
+template <typename T, bool Z>
 __device__ __forceinline__ void gs_row(const LevelView &L, const T *__restrict__ A, T *__restrict__ x,
-				       const T *__restrict__ r, int j, int k);
-template <>
-__device__ __forceinline__ void gs_row<float>(const LevelView &L, const float *__restrict__ A, float *__restrict__ x,
-					      const float *__restrict__ r, int j, int k) {
-	gs_quad_row_f32(L, A, x, r, j, k);
-}
-template <>
-__device__ __forceinline__ void gs_row<double>(const LevelView &L, const double *__restrict__ A, double *__restrict__ x,
-					       const double *__restrict__ r, int j, int k) {
-	gs_pair_row<double>(L, A, x, r, j, k);
+				       const T *__restrict__ r, int j, int k) {
+	if constexpr (sizeof(T) == 4) {
+		gs_quad_row_f32<Z>(L, A, x, r, j, k);
+	} else {
+		static_assert(!Z, "first sweep from zero: FP32 only");
+		gs_pair_row<T>(L, A, x, r, j, k);
+	}
 }
 
 template <typename T> struct Ppl { static const int v = sizeof(T) == 4 ? 4 : 2; };
@@ -679,24 +698,24 @@ template <typename T> struct Ppl { static const int v = sizeof(T) == 4 ? 4 : 2; 
 // Wavefront w of a colour-pair launch: rows w*rpw .. of the pair when they
 // fit the lanes, else the single long row w, streamed in segments.  w must be
 // wavefront-uniform.
-template <typename T>
+template <typename T, bool Z = false>
 __device__ __forceinline__ void gs_wave(const LevelView &L, const T *__restrict__ A, T *__restrict__ x,
 					const T *__restrict__ r, int cj, int ck, int w) {
 	const int nb = (L.n - cj + 1) >> 1, nd = (L.o - ck + 1) >> 1, rows = nb * nd;
 	const RowLanes W = row_lanes(L.m, (int)sizeof(T));
 	if (W.lpr <= 64) {
 		const int q0 = w * W.rpw;
-		if (q0 < rows) gs_lanes<T, Ppl<T>::v>(L, A, x, r, cj, ck, q0, rows);
+		if (q0 < rows) gs_lanes<T, Ppl<T>::v, Z>(L, A, x, r, cj, ck, q0, rows);
 	} else if (w < rows) {
-		gs_row<T>(L, A, x, r, 2 * (w % nb) + cj, 2 * (w / nb) + ck);
+		gs_row<T, Z>(L, A, x, r, 2 * (w % nb) + cj, 2 * (w / nb) + ck);
 	}
 }
 
-template <typename T>
+template <typename T, bool Z = false>
 __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void k_gs_pair(LevelView L, const T *__restrict__ A, T *__restrict__ x,
 						 const T *__restrict__ r, int cj, int ck) {
 	const int w = __builtin_amdgcn_readfirstlane(xcd_block((int)blockIdx.x, (int)gridDim.x) * (int)blockDim.y + (int)threadIdx.y);
-	gs_wave<T>(L, A, x, r, cj, ck, w);
+	gs_wave<T, Z>(L, A, x, r, cj, ck, w);
 }
 
 // Wavefronts of one colour-pair launch.
@@ -714,12 +733,20 @@ __host__ __device__ inline int gs_pair_waves(const LevelView &L, int pair) {
 static const int kGsRows = PGMG_GS_ROWS;
 
 template <typename T>
-void launch_gs_pair(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int pair) {
+void launch_gs_pair(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int pair, bool from_zero) {
 	const int cj = pair & 1, ck = (pair >> 1) & 1;
 	const int nb = (L.n - cj + 1) >> 1, nd = (L.o - ck + 1) >> 1;
 	if (nb <= 0 || nd <= 0) return;
 	const int waves = gs_pair_waves<T>(L, pair);
-	hipLaunchKernelGGL(k_gs_pair<T>, dim3((waves + kGsRows - 1) / kGsRows), dim3(64, kGsRows, 1), 0, s, L, A, x, r, cj, ck);
+	const dim3 grid((waves + kGsRows - 1) / kGsRows), block(64, kGsRows, 1);
+	if constexpr (sizeof(T) == 4) {
+		if (from_zero) {
+			hipLaunchKernelGGL((k_gs_pair<T, true>), grid, block, 0, s, L, A, x, r, cj, ck);
+			HIPCHECK_LAUNCH();
+			return;
+		}
+	}
+	hipLaunchKernelGGL((k_gs_pair<T, false>), grid, block, 0, s, L, A, x, r, cj, ck);
 	HIPCHECK_LAUNCH();
 }
 
@@ -2407,7 +2434,7 @@ void launch_b_add_x(hipStream_t s, const SlotDev *slots, WorkList wl) {
 // ------------------------------------------------------ instantiations
 #define INST(T) \
 	template void launch_gs_colour<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int); \
-	template void launch_gs_pair<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int); \
+	template void launch_gs_pair<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int, bool); \
 	template void launch_gs_plane<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int); \
 	template void launch_gs_small<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int, int, int); \
 	template int launch_sumsq<T>(hipStream_t, const LevelView &, const T *, double *); \
