@@ -442,6 +442,20 @@ __device__ __forceinline__ void ld_vec_stream(const T *base, uint32_t off, T (&o
 #pragma unroll
 	for (int t = 0; t < N; t++) out[t] = pv[t];
 }
+// A level whose coefficients fit the 256 MB memory-side cache (MALL) keeps
+// them there from one sweep to the next with plain loads; a larger level
+// streams them with the non-temporal hint (measured on 257^3: levels 1-2 in
+// FP32 and level 2 in FP64 faster with plain loads, level 0 and FP64 level 1
+// with non-temporal ones).
+template <typename T>
+__device__ __forceinline__ bool coef_streamed(const LevelView &L) {
+	return 27LL * L.m * L.n * L.o * (long long)sizeof(T) > (256LL << 20);
+}
+template <typename T, int N>
+__device__ __forceinline__ void ld_vec_coef(const T *base, uint32_t off, T (&out)[N], bool nt) {
+	if (nt) ld_vec_stream<T, N>(base, off, out);
+	else ld_vec<T, N>(base, off, out);
+}
 template <typename T, int N>
 __device__ __forceinline__ void st_vec(T *base, uint32_t off, const T (&v)[N]) {
 	typedef typename VecOf<T, N>::type V;
@@ -474,6 +488,7 @@ __device__ __forceinline__ void gs_lanes(const LevelView &L, const T *__restrict
 	const int j0 = 2 * (q0 % nb) + cj, k0 = 2 * (q0 / nb) + ck;
 	const long long xb0 = idx3(L, 0, j0, k0), ab0 = arow0(L, j0, k0);
 	const long long px = L.px, pxy = L.pxy;
+	const bool nt = coef_streamed<T>(L);
 	const T *Ab = A + ab0;
 	const T *rb = r + xb0;
 	T *xb = x + xb0;
@@ -503,7 +518,7 @@ __device__ __forceinline__ void gs_lanes(const LevelView &L, const T *__restrict
 			const T *xq = xb + dj * px + dk * pxy;
 			T a[3][PPL], xv[PPL];
 #pragma unroll
-			for (int d = 0; d < 3; d++) ld_vec_stream<T, PPL>(Ab + (3 * rq + d) * px, oa, a[d]);
+			for (int d = 0; d < 3; d++) ld_vec_coef<T, PPL>(Ab + (3 * rq + d) * px, oa, a[d], nt);
 			T xm = (T)0, xp = (T)0;
 			if (Z && zero_row(cj, ck, dj, dk)) {
 #pragma unroll
@@ -607,6 +622,7 @@ __device__ __forceinline__ double residual_lanes(const LevelView &L, const T *__
 	const int j0 = q0 % L.n, k0 = q0 / L.n;
 	const long long xb0 = idx3(L, 0, j0, k0), ab0 = arow0(L, j0, k0);
 	const long long px = L.px, pxy = L.pxy;
+	const bool nt = coef_streamed<T>(L);
 	double v2 = 0.0;
 	if (!live) return v2;
 	const int j = q % L.n, k = q / L.n;
@@ -630,7 +646,7 @@ __device__ __forceinline__ double residual_lanes(const LevelView &L, const T *__
 		const T *xq = xb + dj * px + dk * pxy;
 		T a[3][PPL], xv[PPL];
 #pragma unroll
-		for (int d = 0; d < 3; d++) ld_vec_stream<T, PPL>(Ab + (3 * rq + d) * px, oa, a[d]);
+		for (int d = 0; d < 3; d++) ld_vec_coef<T, PPL>(Ab + (3 * rq + d) * px, oa, a[d], nt);
 		ld_vec<T, PPL>(xq, ox, xv);
 		const T xm = ld_row<T>(xq - 1, ox);
 		const T xp = needxp ? ld_row<T>(xq + PPL, ox) : (T)0;
