@@ -173,8 +173,7 @@ int build_plan(pgmg_batch *b) {
 			const LevelView &L = D.lv[l];
 			if (l < S.nl - 1) {
 				// as the lone hierarchy's forward colour sweeps
-				if (small_lds_bytes(L, b->mixed ? 4 : 8) <= kSmallLdsBytes &&
-				    ((long long)L.m * L.n * L.o <= 1024 || small_pairs_per_pass(L) <= 1024))
+				if (batch_level_small(L, el))
 					plan_add(b, list_index(l, L_SMALL), s, 1);
 				else for (int q = 0; q < 4; q++) plan_add(b, list_index(l, L_PAIR0 + q), s, batch_units(BK_PAIR0 + q, L, el));
 				plan_add(b, list_index(l, L_RES), s, batch_units(BK_ROWS, L, el));
@@ -192,11 +191,14 @@ int build_plan(pgmg_batch *b) {
 	return PGMG_OK;
 }
 
+// from_zero: level 0 of the mixed correction, whose pair-list slots did not
+// clear x32 (top_zero); the first sweep reads none of it.
 template <typename T>
-void batched_gs(pgmg_batch *b, int l, int sweeps) {
+void batched_gs(pgmg_batch *b, int l, int sweeps, bool from_zero = false) {
 	if (sweeps <= 0) return;
 	for (int s = 0; s < sweeps; s++)
-		for (int q = 0; q < 4; q++) launch_b_gs_pair<T>(b->stream, b->d_slots, wlist(b, list_index(l, L_PAIR0 + q)), l, q);
+		for (int q = 0; q < 4; q++)
+			launch_b_gs_pair<T>(b->stream, b->d_slots, wlist(b, list_index(l, L_PAIR0 + q)), l, q, from_zero && s == 0);
 	launch_b_gs_small<T>(b->stream, b->d_slots, wlist(b, list_index(l, L_SMALL)), l, sweeps);
 }
 
@@ -205,7 +207,7 @@ template <typename T>
 void batched_cycle(pgmg_batch *b) {
 	const int L = b->max_nl;
 	for (int l = 0; l < L; l++) {
-		batched_gs<T>(b, l, b->opts.v_down + l);
+		batched_gs<T>(b, l, b->opts.v_down + l, b->mixed && l == 0);
 		launch_b_residual<T>(b->stream, b->d_slots, wlist(b, list_index(l, L_RES)), l);
 		if (l + 1 < kBatchMaxLevels) launch_b_restrict<T>(b->stream, b->d_slots, wlist(b, list_index(l + 1, L_RESTRICT)), l + 1);
 		launch_b_lu_solve<T>(b->stream, b->d_slots, wlist(b, list_index(l, L_LU)), l);
@@ -241,6 +243,9 @@ int slot_setup(pgmg_batch *b, int s, const pgmg_dims *d, int family, uint64_t se
 	SlotDev &D = S.dev;
 	memset(&D, 0, sizeof D);
 	D.nl = S.nl;
+	// level 0 on the colour-pair kernels: the mixed correction's first sweep
+	// starts from zero without x32 being cleared
+	D.top_zero = b->mixed && S.nl > 1 && !batch_level_small(make_level_view(S.dims[0], S.dims[1], S.dims[2]), 4) ? 1 : 0;
 	char *p = b->arena.base + S.off;
 	auto take = [&](uint64_t bytes) { char *q = p; p += al(bytes); return (void *)q; };
 	for (int l = 0; l < S.nl; l++) {

@@ -93,7 +93,7 @@ const int kBatchMaxLevels = 16;
 struct SlotDev {
 	int nl;
 	int lu_active, lu_n;
-	int pad_;
+	int top_zero;  // mixed: the first level-0 sweep starts from x32 = 0 without reading it
 	LevelView lv[kBatchMaxLevels];
 	double *A[kBatchMaxLevels], *x[kBatchMaxLevels], *r[kBatchMaxLevels];
 	float *A32[kBatchMaxLevels], *x32[kBatchMaxLevels], *r32[kBatchMaxLevels];
@@ -121,7 +121,10 @@ int batch_units(int kind, const LevelView &L, int elem);
 
 // Batched launchers: one launch covers the level `level` of every slot in
 // the list; blocks walk the units grid-stride.
-template <typename T> void launch_b_gs_pair(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int pair);
+// from_zero (FP32): the first sweep of level 0 of a mixed correction (slots
+// with top_zero set; the others are in the small-level list).
+template <typename T> void launch_b_gs_pair(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int pair,
+					    bool from_zero = false);
 template <typename T> void launch_b_gs_small(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int nsweeps);
 template <typename T> void launch_b_residual(hipStream_t s, const SlotDev *slots, WorkList wl, int level);
 void launch_b_residual_top(hipStream_t s, const SlotDev *slots, WorkList wl, bool to_f32);
@@ -145,6 +148,11 @@ const int kSmallLdsBytes = 64 * 1024;
 // at most (cj = ck = 0).
 __host__ __device__ inline long long small_pairs_per_pass(const LevelView &L) {
 	return (long long)((L.m + 1) / 2) * ((L.n + 1) / 2) * ((L.o + 1) / 2);
+}
+// A batched level goes to the one-workgroup small-level kernel (as the lone
+// hierarchy's forward colour sweeps), else to the colour-pair kernels.
+inline bool batch_level_small(const LevelView &L, int elem) {
+	return small_lds_bytes(L, elem) <= kSmallLdsBytes && ((long long)L.m * L.n * L.o <= 1024 || small_pairs_per_pass(L) <= 1024);
 }
 
 // Kernel classes for timing.

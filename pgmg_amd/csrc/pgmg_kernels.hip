@@ -2285,13 +2285,13 @@ __device__ __forceinline__ bool cubes_unit(const LevelView &L, int lu, int &ic, 
 		const SlotDev *S = slots + wl.slot[q_];                                    \
 		const int lu = u - wl.prefix[q_];
 
-template <typename T>
+template <typename T, bool Z = false>
 __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void kb_gs_pair(const SlotDev *__restrict__ slots, WorkList wl,
 								   int level, int cj, int ck) {
 	BATCH_LOOP
 		const LevelView L = S->lv[level];
-		gs_wave<T>(L, sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), cj, ck,
-			   __builtin_amdgcn_readfirstlane(lu * 4 + (int)threadIdx.y));
+		gs_wave<T, Z>(L, sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), cj, ck,
+			      __builtin_amdgcn_readfirstlane(lu * 4 + (int)threadIdx.y));
 	}
 }
 
@@ -2324,7 +2324,8 @@ __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void kb_residual_top(const Sl
 	BATCH_LOOP
 		const LevelView L = S->lv[0];
 		const int w = __builtin_amdgcn_readfirstlane(lu * 4 + (int)threadIdx.y);
-		const double v2 = to_f32 ? residual_wave<double, float>(L, S->A[0], S->x[0], S->r[0], S->r32[0], S->x32[0], w)
+		const double v2 = to_f32 ? residual_wave<double, float>(L, S->A[0], S->x[0], S->r[0], S->r32[0],
+									 S->top_zero ? (float *)nullptr : S->x32[0], w)
 					 : residual_wave<double, double>(L, S->A[0], S->x[0], S->r[0], (double *)nullptr, (double *)nullptr, w);
 		const double sblk = block_sum_256(v2);
 		if (threadIdx.x == 0 && threadIdx.y == 0) S->partial[lu] = sblk;
@@ -2396,9 +2397,18 @@ __global__ __launch_bounds__(256) void kb_add_x(const SlotDev *__restrict__ slot
 static unsigned batch_grid(const WorkList &wl) { return (unsigned)(wl.total > 0 ? wl.total : 1); }
 
 template <typename T>
-void launch_b_gs_pair(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int pair) {
+void launch_b_gs_pair(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int pair, bool from_zero) {
 	if (wl.n <= 0) return;
-	hipLaunchKernelGGL(kb_gs_pair<T>, dim3(batch_grid(wl)), dim3(64, 4), 0, s, slots, wl, level, pair & 1, (pair >> 1) & 1);
+	if constexpr (sizeof(T) == 4) {
+		if (from_zero) {
+			hipLaunchKernelGGL((kb_gs_pair<T, true>), dim3(batch_grid(wl)), dim3(64, 4), 0, s, slots, wl, level, pair & 1,
+					   (pair >> 1) & 1);
+			HIPCHECK_LAUNCH();
+			return;
+		}
+	}
+	hipLaunchKernelGGL((kb_gs_pair<T, false>), dim3(batch_grid(wl)), dim3(64, 4), 0, s, slots, wl, level, pair & 1,
+			   (pair >> 1) & 1);
 	HIPCHECK_LAUNCH();
 }
 template <typename T>
@@ -2458,7 +2468,7 @@ void launch_b_add_x(hipStream_t s, const SlotDev *slots, WorkList wl) {
 	template void launch_interp<T>(hipStream_t, const LevelView &, const LevelView &, const T *, T *); \
 	template void launch_lu_solve<T>(hipStream_t, const LevelView &, int, const double *, const int *, const T *, T *); \
 	template void launch_fill<T>(hipStream_t, T *, long long, T); \
-	template void launch_b_gs_pair<T>(hipStream_t, const SlotDev *, WorkList, int, int); \
+	template void launch_b_gs_pair<T>(hipStream_t, const SlotDev *, WorkList, int, int, bool); \
 	template void launch_b_gs_small<T>(hipStream_t, const SlotDev *, WorkList, int, int); \
 	template void launch_b_residual<T>(hipStream_t, const SlotDev *, WorkList, int); \
 	template void launch_b_restrict<T>(hipStream_t, const SlotDev *, WorkList, int); \
