@@ -44,6 +44,15 @@ SOLVES = [
 ]
 
 
+# Full-size solves kept as summaries (the iterate is too large to store):
+# every cycle's sum of squared residuals, the SHA-256 of the final x and a
+# few of its values.  (name, family, seed, m, n, o, bottom, tol)
+SUMMARIES = [
+    ("summary_f7u_257_gs24", 1, 12345, 257, 257, 257, "gs24", 1e-10),
+]
+SAMPLE_POINTS = [0, 1, 257 * 128 + 77, 257 * 257 * 128 + 257 * 128 + 128, 257 ** 3 - 1]
+
+
 # z-slice files written by the reference's output_x / output_r /
 # output_residual after two V-cycles: (name, family, seed, m, n, o, k).
 SLICES = [
@@ -71,7 +80,7 @@ def read_records(path):
 def run(args):
     env = dict(os.environ)
     env["LD_LIBRARY_PATH"] = "/opt/conda/lib"
-    p = subprocess.run([REF] + [str(a) for a in args], capture_output=True, text=True, env=env, timeout=600)
+    p = subprocess.run([REF] + [str(a) for a in args], capture_output=True, text=True, env=env, timeout=1800)
     if p.returncode != 0:
         sys.stderr.write(p.stdout + p.stderr)
         raise RuntimeError("reference run failed: %s" % args)
@@ -85,9 +94,15 @@ def sha(a):
 def main():
     if not os.path.exists(REF):
         raise SystemExit("build the reference first: make -f oracle/Makefile.ref")
+    only = set(sys.argv[1:])  # optional: names of the fixtures to (re)write
+
+    def want(name):
+        return not only or name in only
     tmp = os.path.join(ROOT, "_work", "golden_tmp.bin")
     os.makedirs(os.path.dirname(tmp), exist_ok=True)
     for name, fam, seed, m, n, o in FIXTURES:
+        if not want(name):
+            continue
         run(["fixture", fam, seed, m, n, o, tmp])
         rec = read_records(tmp)
         dims = rec["dims"].reshape(-1, 3)
@@ -111,7 +126,20 @@ def main():
             save[k] = v
         np.savez_compressed(os.path.join(OUT, name + ".npz"), **save)
         print(name, "levels", dims.tolist(), flush=True)
+    for name, fam, seed, m, n, o, bottom, tol in SUMMARIES:
+        if not want(name):
+            continue
+        info = run(["solve", fam, seed, m, n, o, bottom, tol, 100, tmp])
+        rec = read_records(tmp)
+        x = rec["x"]
+        np.savez_compressed(os.path.join(OUT, name + ".npz"),
+                            meta=np.array([fam, seed, m, n, o], dtype=np.int64), tol=np.array(tol),
+                            res=rec["res"], sha_x=np.array(sha(x)), sample_idx=np.array(SAMPLE_POINTS, dtype=np.int64),
+                            sample_x=x[SAMPLE_POINTS], rhs_l2norm=rec["rhs_l2norm"], info=np.array(json.dumps(info)))
+        print(name, info, flush=True)
     for name, fam, seed, m, n, o, bottom, tol in SOLVES:
+        if not want(name):
+            continue
         info = run(["solve", fam, seed, m, n, o, bottom, tol, 100, tmp])
         rec = read_records(tmp)
         np.savez_compressed(os.path.join(OUT, name + ".npz"),
@@ -120,6 +148,8 @@ def main():
                             info=np.array(json.dumps(info)))
         print(name, info, flush=True)
     for name, fam, seed, m, n, o, k in SLICES:
+        if not want(name):
+            continue
         prefix = os.path.join(ROOT, "_work", "golden_slice")
         run(["slices", fam, seed, m, n, o, k, prefix])
         save = {"meta": np.array([fam, seed, m, n, o, k], dtype=np.int64)}

@@ -17,7 +17,7 @@ from oracle import oracle as O
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 FIXTURES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
-                  if not os.path.basename(p).startswith(("solve_", "slices_")))
+                  if not os.path.basename(p).startswith(("solve_", "slices_", "summary_")))
 SOLVES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "solve_*.npz")))
 
 
@@ -153,3 +153,17 @@ def test_constructor_lu_falls_back():
     default solve uses the 24-sweep bottom."""
     for name in FIXTURES:
         assert int(load(name)["aexact_after_ctor"][0]) == 0
+
+
+def test_full_size_summary_fixture():
+    """The 257^3 reference run kept as a summary: ten cycles, each cutting
+    the residual, below 1e-10 relative at the end; the source norm is the
+    oracle's (same generator)."""
+    z = load("summary_f7u_257_gs24")
+    fam, seed, m, n, o = [int(v) for v in z["meta"]]
+    res = z["res"]
+    assert len(res) == 11 and np.all(np.diff(res) < 0)
+    assert np.sqrt(res[-1] / z["rhs_l2norm"][0]) < float(z["tol"]) <= np.sqrt(res[-2] / z["rhs_l2norm"][0])
+    b = O.gen_rhs(seed, m, n, o)
+    np.testing.assert_allclose(float(np.sum(b * b)), float(z["rhs_l2norm"][0]), rtol=1e-12)
+    assert len(str(z["sha_x"])) == 64 and len(z["sample_x"]) == len(z["sample_idx"])
