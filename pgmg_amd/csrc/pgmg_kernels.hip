@@ -12,19 +12,11 @@ namespace pgmg {
 
 #define HIPCHECK_LAUNCH() (void)hipGetLastError()
 
-// Optional minimum of resident wavefronts per SIMD for the streaming stencil
-// kernels (0: the compiler's choice).  The row kernels issue every load of a
-// row segment at once, so their latency is covered within the wavefront;
-// forcing more wavefronts only makes the compiler spill (FP32 row kernels
-// use about 100 VGPRs, 4 wavefronts; FP64 about 190, 2 wavefronts).
-#ifndef PGMG_STREAM_WAVES
-#define PGMG_STREAM_WAVES 0
-#endif
-#if PGMG_STREAM_WAVES > 0
-#define PGMG_STREAM_ATTR __attribute__((amdgpu_waves_per_eu(PGMG_STREAM_WAVES)))
-#else
-#define PGMG_STREAM_ATTR
-#endif
+// The streaming stencil kernels leave occupancy to the compiler: a row
+// kernel issues every load of a row segment at once (about 210 VGPRs in
+// FP32, 240 in FP64: two wavefronts per SIMD), so its latency is covered
+// within the wavefront; forcing more wavefronts per SIMD only made the
+// compiler spill.
 
 
 // The hardware deals workgroups to the 8 XCDs round-robin (workgroup b runs
@@ -728,7 +720,7 @@ __device__ __forceinline__ void gs_wave(const LevelView &L, const T *__restrict_
 }
 
 template <typename T, bool Z = false>
-__global__ __launch_bounds__(256) PGMG_STREAM_ATTR void k_gs_pair(LevelView L, const T *__restrict__ A, T *__restrict__ x,
+__global__ __launch_bounds__(256) void k_gs_pair(LevelView L, const T *__restrict__ A, T *__restrict__ x,
 						 const T *__restrict__ r, int cj, int ck) {
 	const int w = __builtin_amdgcn_readfirstlane(xcd_block((int)blockIdx.x, (int)gridDim.x) * (int)blockDim.y + (int)threadIdx.y);
 	gs_wave<T, Z>(L, A, x, r, cj, ck, w);
@@ -742,11 +734,9 @@ __host__ __device__ inline int gs_pair_waves(const LevelView &L, int pair) {
 	return row_lanes(L.m, (int)sizeof(T)).lpr <= 64 ? row_waves(rows, L.m, (int)sizeof(T)) : rows;
 }
 
-// Wavefronts per workgroup of the row smoother.
-#ifndef PGMG_GS_ROWS
-#define PGMG_GS_ROWS 1
-#endif
-static const int kGsRows = PGMG_GS_ROWS;
+// Wavefronts per workgroup of the row smoother: one, so the XCD-contiguous
+// row mapping works at the finest grain.
+constexpr int kGsRows = 1;
 
 template <typename T>
 void launch_gs_pair(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int pair, bool from_zero) {
@@ -1273,7 +1263,7 @@ __device__ __forceinline__ double residual_wave(const LevelView &L, const T *__r
 }
 
 template <typename T, typename TR>
-__global__ __launch_bounds__(256) PGMG_STREAM_ATTR void k_residual(LevelView L, const T *__restrict__ A, const T *__restrict__ x,
+__global__ __launch_bounds__(256) void k_residual(LevelView L, const T *__restrict__ A, const T *__restrict__ x,
 						  const T *__restrict__ r, TR *__restrict__ res, double *__restrict__ partial,
 						  TR *__restrict__ zx) {
 	const int qb = xcd_block((int)blockIdx.x, (int)gridDim.x);
@@ -2286,7 +2276,7 @@ __device__ __forceinline__ bool cubes_unit(const LevelView &L, int lu, int &ic, 
 		const int lu = u - wl.prefix[q_];
 
 template <typename T, bool Z = false>
-__global__ __launch_bounds__(256) PGMG_STREAM_ATTR void kb_gs_pair(const SlotDev *__restrict__ slots, WorkList wl,
+__global__ __launch_bounds__(256) void kb_gs_pair(const SlotDev *__restrict__ slots, WorkList wl,
 								   int level, int cj, int ck) {
 	BATCH_LOOP
 		const LevelView L = S->lv[level];
@@ -2310,7 +2300,7 @@ __global__ __launch_bounds__(1024) void kb_gs_small(const SlotDev *__restrict__ 
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) PGMG_STREAM_ATTR void kb_residual(const SlotDev *__restrict__ slots, WorkList wl, int level) {
+__global__ __launch_bounds__(256) void kb_residual(const SlotDev *__restrict__ slots, WorkList wl, int level) {
 	BATCH_LOOP
 		const LevelView L = S->lv[level];
 		(void)residual_wave<T, T>(L, sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), sres<T>(S), (T *)nullptr,
@@ -2320,7 +2310,7 @@ __global__ __launch_bounds__(256) PGMG_STREAM_ATTR void kb_residual(const SlotDe
 
 // Top-level FP64 residual with per-slot block sums; optionally the residual
 // rounded to FP32 into the slot's level-0 r32 (the defect-correction source).
-__global__ __launch_bounds__(256) PGMG_STREAM_ATTR void kb_residual_top(const SlotDev *__restrict__ slots, WorkList wl, int to_f32) {
+__global__ __launch_bounds__(256) void kb_residual_top(const SlotDev *__restrict__ slots, WorkList wl, int to_f32) {
 	BATCH_LOOP
 		const LevelView L = S->lv[0];
 		const int w = __builtin_amdgcn_readfirstlane(lu * 4 + (int)threadIdx.y);
