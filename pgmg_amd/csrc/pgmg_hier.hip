@@ -174,13 +174,17 @@ template <> float *res_scratch<float>(pgmg_hier *h) { return h->res32; }
 
 // One workgroup smooths a small level when that beats per-class (or
 // per-hyperplane) launches: with x in LDS, for exact mode, and for forward
-// colour sweeps when each thread holds one point or one point pair.
+// colour sweeps when each thread holds one point with its coefficients in
+// registers.  (A level of up to 1024 point pairs per pass, 17^3, is faster on
+// the colour-pair launches alone: the single workgroup is bound by its one
+// CU's L2 bandwidth for the coefficients.  In a batch the small levels of
+// many slots run side by side, and the pair-staged form is kept there.)
 template <typename T>
 bool gs_use_small(const pgmg_hier *h, const Level &L, int mode) {
 	const bool exact = h->opts.smoother == PGMG_SMOOTH_EXACT;
 	const bool lds = small_lds_bytes(L.v, (int)sizeof(T)) <= kSmallLdsBytes;
 	if (exact) return lds || L.points() <= kSmallExact;
-	if (mode == 0) return lds && (L.points() <= 1024 || small_pairs_per_pass(L.v) <= 1024);
+	if (mode == 0) return lds && L.points() <= 1024;
 	return L.points() <= kSmallColour;
 }
 
