@@ -434,14 +434,18 @@ __device__ __forceinline__ void ld_vec_stream(const T *base, uint32_t off, T (&o
 #pragma unroll
 	for (int t = 0; t < N; t++) out[t] = pv[t];
 }
-// A level whose coefficients fit the 256 MB memory-side cache (MALL) keeps
-// them there from one sweep to the next with plain loads; a larger level
-// streams them with the non-temporal hint (measured on 257^3: levels 1-2 in
-// FP32 and level 2 in FP64 faster with plain loads, level 0 and FP64 level 1
-// with non-temporal ones).
+// Coefficient planes 0..split-1 of a level are read with plain loads, the
+// rest with the non-temporal hint.  A level whose coefficients fit the 256 MB
+// memory-side cache (MALL) keeps them there from one sweep to the next
+// (split 27); one up to 1 GB keeps a third of its planes there (a cyclic
+// sweep over more than the cache would otherwise evict every line before its
+// reuse); a larger level streams them all (split 0).  Measured on 257^3:
+// FP32 levels 1-2 and FP64 level 2 fastest with plain loads, FP64 level 1
+// (464 MB) 7 % faster with a third cached, level 0 fastest streamed.
 template <typename T>
-__device__ __forceinline__ bool coef_streamed(const LevelView &L) {
-	return 27LL * L.m * L.n * L.o * (long long)sizeof(T) > (256LL << 20);
+__device__ __forceinline__ int coef_split(const LevelView &L) {
+	const long long bytes = 27LL * L.m * L.n * L.o * (long long)sizeof(T);
+	return bytes <= (256LL << 20) ? 27 : (bytes <= (1024LL << 20) ? 9 : 0);
 }
 template <typename T, int N>
 __device__ __forceinline__ void ld_vec_coef(const T *base, uint32_t off, T (&out)[N], bool nt) {
@@ -480,7 +484,7 @@ __device__ __forceinline__ void gs_lanes(const LevelView &L, const T *__restrict
 	const int j0 = 2 * (q0 % nb) + cj, k0 = 2 * (q0 / nb) + ck;
 	const long long xb0 = idx3(L, 0, j0, k0), ab0 = arow0(L, j0, k0);
 	const long long px = L.px, pxy = L.pxy;
-	const bool nt = coef_streamed<T>(L);
+	const int split = coef_split<T>(L);
 	const T *Ab = A + ab0;
 	const T *rb = r + xb0;
 	T *xb = x + xb0;
@@ -510,7 +514,7 @@ __device__ __forceinline__ void gs_lanes(const LevelView &L, const T *__restrict
 			const T *xq = xb + dj * px + dk * pxy;
 			T a[3][PPL], xv[PPL];
 #pragma unroll
-			for (int d = 0; d < 3; d++) ld_vec_coef<T, PPL>(Ab + (3 * rq + d) * px, oa, a[d], nt);
+			for (int d = 0; d < 3; d++) ld_vec_coef<T, PPL>(Ab + (3 * rq + d) * px, oa, a[d], 3 * rq + d >= split);
 			T xm = (T)0, xp = (T)0;
 			if (Z && zero_row(cj, ck, dj, dk)) {
 #pragma unroll
@@ -614,7 +618,7 @@ __device__ __forceinline__ double residual_lanes(const LevelView &L, const T *__
 	const int j0 = q0 % L.n, k0 = q0 / L.n;
 	const long long xb0 = idx3(L, 0, j0, k0), ab0 = arow0(L, j0, k0);
 	const long long px = L.px, pxy = L.pxy;
-	const bool nt = coef_streamed<T>(L);
+	const int split = coef_split<T>(L);
 	double v2 = 0.0;
 	if (!live) return v2;
 	const int j = q % L.n, k = q / L.n;
@@ -638,7 +642,7 @@ __device__ __forceinline__ double residual_lanes(const LevelView &L, const T *__
 		const T *xq = xb + dj * px + dk * pxy;
 		T a[3][PPL], xv[PPL];
 #pragma unroll
-		for (int d = 0; d < 3; d++) ld_vec_coef<T, PPL>(Ab + (3 * rq + d) * px, oa, a[d], nt);
+		for (int d = 0; d < 3; d++) ld_vec_coef<T, PPL>(Ab + (3 * rq + d) * px, oa, a[d], 3 * rq + d >= split);
 		ld_vec<T, PPL>(xq, ox, xv);
 		const T xm = ld_row<T>(xq - 1, ox);
 		const T xp = needxp ? ld_row<T>(xq + PPL, ox) : (T)0;
