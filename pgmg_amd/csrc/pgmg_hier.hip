@@ -435,6 +435,7 @@ int pgmg_hier_create(const pgmg_dims *d, const pgmg_opts *opts, pgmg_hier **out)
 	for (int l = 0; l < nl; l++) {
 		Level &L = h->lev[l];
 		L.v = make_level_view(dims[3 * l], dims[3 * l + 1], dims[3 * l + 2]);
+		L.v.cache_coef = 1;  // one level runs at a time
 		const size_t np = (size_t)L.v.npad;
 		ALLOC(L.A, 27 * np * 8);
 		ALLOC(L.x, np * 8);

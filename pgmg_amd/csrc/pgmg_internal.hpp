@@ -24,6 +24,9 @@ struct LevelView {
 	long long pxy;     // plane pitch (elements)
 	long long base;    // index of point (0,0,0)
 	long long npad;    // elements per array, halo included
+	int cache_coef;    // 1: levels run one at a time (a lone hierarchy), so a
+	                   // level may keep coefficients in the memory-side cache;
+	                   // 0: stream them (batched slots run side by side)
 };
 
 inline LevelView make_level_view(int m, int n, int o) {
@@ -33,6 +36,7 @@ inline LevelView make_level_view(int m, int n, int o) {
 	v.pxy = (long long)v.px * (n + 2);
 	v.base = 4 + v.px + v.pxy;
 	v.npad = v.pxy * (o + 2);
+	v.cache_coef = 0;
 	return v;
 }
 

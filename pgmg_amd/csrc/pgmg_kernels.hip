@@ -435,15 +435,20 @@ __device__ __forceinline__ void ld_vec_stream(const T *base, uint32_t off, T (&o
 	for (int t = 0; t < N; t++) out[t] = pv[t];
 }
 // Coefficient planes 0..split-1 of a level are read with plain loads, the
-// rest with the non-temporal hint.  A level whose coefficients fit the 256 MB
-// memory-side cache (MALL) keeps them there from one sweep to the next
-// (split 27); one up to 1 GB keeps a third of its planes there (a cyclic
-// sweep over more than the cache would otherwise evict every line before its
-// reuse); a larger level streams them all (split 0).  Measured on 257^3:
-// FP32 levels 1-2 and FP64 level 2 fastest with plain loads, FP64 level 1
-// (464 MB) 7 % faster with a third cached, level 0 fastest streamed.
+// rest with the non-temporal hint.  In a lone hierarchy (L.cache_coef), a
+// level whose coefficients fit the 256 MB memory-side cache (MALL) keeps
+// them there from one sweep to the next (split 27); one up to 1 GB keeps a
+// third of its planes there (a cyclic sweep over more than the cache would
+// otherwise evict every line before its reuse); a larger level streams them
+// all (split 0).  Measured on 257^3: FP32 levels 1-2 and FP64 level 2
+// fastest with plain loads, FP64 level 1 (464 MB) 7 % faster with a third
+// cached, level 0 fastest streamed.  Batched slots run side by side, their
+// levels together far larger than the cache: they stream everything
+// (measured: 256-problem batch 16.3 s with the per-level rule, 13.6 s
+// streamed).
 template <typename T>
 __device__ __forceinline__ int coef_split(const LevelView &L) {
+	if (!L.cache_coef) return 0;
 	const long long bytes = 27LL * L.m * L.n * L.o * (long long)sizeof(T);
 	return bytes <= (256LL << 20) ? 27 : (bytes <= (1024LL << 20) ? 9 : 0);
 }
