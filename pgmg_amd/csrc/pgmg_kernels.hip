@@ -1681,15 +1681,30 @@ __device__ __forceinline__ double rat_entry(const LevelView &F, const LevelView 
 							else { if (sx) continue; wx = 0.5; }
 							const int gxl = max(plx, fi - 1), gxh = min(phx, min(fi + 2, F.m));
 							if (gxl >= gxh) continue;
+							// all 27 planes of f's row block are valid memory: load
+							// them at once, then sum the terms inside J's support
+							// in (k,j,i) order
 							const double *Afr = Af + aidx3(F, 0, fi, fj, fk);
+							double a[27];
+#pragma unroll
+							for (int ef = 0; ef < 27; ef++) a[ef] = Afr[ef * F.px];
 							double inner = 0.0;
-							for (int kk = gzl; kk < gzh; kk++) {
+#pragma unroll
+							for (int dk = 0; dk < 3; dk++) {
+								const int kk = fk - 1 + dk;
+								const bool okk = kk >= gzl && kk < gzh;
 								const double kfac = kk == pcz ? 1.0 : 0.5;
-								for (int jj = gyl; jj < gyh; jj++) {
+#pragma unroll
+								for (int dj = 0; dj < 3; dj++) {
+									const int jj = fj - 1 + dj;
+									const bool okj = okk && jj >= gyl && jj < gyh;
 									const double jkfac = jj == pcy ? kfac : 0.5 * kfac;
-									for (int ii = gxl; ii < gxh; ii++) {
-										const int ef = (kk - fk + 1) * 9 + (jj - fj + 1) * 3 + (ii - fi + 1);
-										inner = inner + (0.125 * Afr[ef * F.px]) * (ii == pcx ? jkfac : 0.5 * jkfac);
+#pragma unroll
+									for (int di = 0; di < 3; di++) {
+										const int ii = fi - 1 + di;
+										if (okj && ii >= gxl && ii < gxh)
+											inner = inner + (0.125 * a[dk * 9 + dj * 3 + di]) *
+													    (ii == pcx ? jkfac : 0.5 * jkfac);
 									}
 								}
 							}
@@ -1762,24 +1777,24 @@ __device__ __forceinline__ void fine_point(const LevelView &F, const double *__r
 // Cube q (cz = q>>2, cy, cx) and its local point l (lz = l>>2, ly, lx), in
 // the scan order cube first, then local; local 0 of a cube at offset 0 on
 // an axis exists only in the cube at offset 1 (restrict_point's loops).
-template <int Q, int L>
+// JS: the entries J formed (all 27, or one).
+template <int Q, int L, typename JS>
 __device__ __forceinline__ void cube_local(const LevelView &F, const double *__restrict__ Af, int fi, int fj, int fk,
 					   double (&acc)[27]) {
 	constexpr int cz = Q >> 2, cy = (Q >> 1) & 1, cx = Q & 1;
 	constexpr int lz = L >> 2, ly = (L >> 1) & 1, lx = L & 1;
 	if constexpr ((cz || lz) && (cy || ly) && (cx || lx))
-		fine_point<2 * (cx - 1) + lx, 2 * (cy - 1) + ly, 2 * (cz - 1) + lz>(F, Af, fi, fj, fk, acc,
-										      std::make_integer_sequence<int, 27>());
+		fine_point<2 * (cx - 1) + lx, 2 * (cy - 1) + ly, 2 * (cz - 1) + lz>(F, Af, fi, fj, fk, acc, JS{});
 }
-template <int Q, int... L>
+template <int Q, typename JS, int... L>
 __device__ __forceinline__ void cube(const LevelView &F, const double *__restrict__ Af, int fi, int fj, int fk,
 				     double (&acc)[27], std::integer_sequence<int, L...>) {
-	(cube_local<Q, L>(F, Af, fi, fj, fk, acc), ...);
+	(cube_local<Q, L, JS>(F, Af, fi, fj, fk, acc), ...);
 }
-template <int... Q>
+template <typename JS, int... Q>
 __device__ __forceinline__ void all(const LevelView &F, const double *__restrict__ Af, int fi, int fj, int fk,
 				    double (&acc)[27], std::integer_sequence<int, Q...>) {
-	(cube<Q>(F, Af, fi, fj, fk, acc, std::make_integer_sequence<int, 8>()), ...);
+	(cube<Q, JS>(F, Af, fi, fj, fk, acc, std::make_integer_sequence<int, 8>()), ...);
 }
 }  // namespace rat
 
@@ -1791,10 +1806,40 @@ __global__ __launch_bounds__(256) void k_rat_interior(LevelView F, LevelView C, 
 	double acc[27];
 #pragma unroll
 	for (int e = 0; e < 27; e++) acc[e] = 0.0;
-	rat::all(F, Af, 2 * I, 2 * Jy, 2 * K, acc, std::make_integer_sequence<int, 8>());
+	rat::all<std::make_integer_sequence<int, 27>>(F, Af, 2 * I, 2 * Jy, 2 * K, acc, std::make_integer_sequence<int, 8>());
 	double *Acr = Ac + aidx3(C, 0, I, Jy, K);
 #pragma unroll
 	for (int e = 0; e < 27; e++) Acr[e * C.px] = acc[e];
+}
+
+// The same interior points one entry per thread (blockIdx.y = entry J, so a
+// wavefront shares it): for levels with few interior points, where one
+// thread per point leaves most of the GPU idle and each thread long.  The
+// entry's sum takes the same fine points in the same order, so its bits are
+// k_rat_interior's.
+template <int J>
+__device__ __forceinline__ void rat_interior_entry(const LevelView &F, const LevelView &C, const double *__restrict__ Af,
+						   double *__restrict__ Ac, int I, int Jy, int K) {
+	double acc[27];
+#pragma unroll
+	for (int e = 0; e < 27; e++) acc[e] = 0.0;
+	rat::all<std::integer_sequence<int, J>>(F, Af, 2 * I, 2 * Jy, 2 * K, acc, std::make_integer_sequence<int, 8>());
+	Ac[aidx3(C, J, I, Jy, K)] = acc[J];
+}
+template <int... J>
+__device__ __forceinline__ void rat_interior_dispatch(int e, const LevelView &F, const LevelView &C,
+						      const double *__restrict__ Af, double *__restrict__ Ac, int I, int Jy,
+						      int K, std::integer_sequence<int, J...>) {
+	((e == J ? rat_interior_entry<J>(F, C, Af, Ac, I, Jy, K) : void()), ...);
+}
+__global__ __launch_bounds__(256) void k_rat_interior_entry(LevelView F, LevelView C, const double *__restrict__ Af,
+							    double *__restrict__ Ac) {
+	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
+	const int ix = lastx - 1, iy = lasty - 1, iz = lastz - 1;
+	const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= (long long)ix * iy * iz) return;
+	const int I = 1 + (int)(q % ix), Jy = 1 + (int)((q / ix) % iy), K = 1 + (int)(q / ((long long)ix * iy));
+	rat_interior_dispatch((int)blockIdx.y, F, C, Af, Ac, I, Jy, K, std::make_integer_sequence<int, 27>());
 }
 
 // The coarse points k_rat_interior leaves out, enumerated densely: those
@@ -1849,11 +1894,20 @@ __global__ __launch_bounds__(256) void k_rat_boundary(LevelView F, LevelView C, 
 	Ac[aidx3(C, eJ, I, Jy, K)] = rat_entry(F, C, Af, I, Jy, K, eJ);
 }
 
+// Interior coarse points below which a level's Galerkin entries are formed
+// one per thread.
+constexpr long long kRatEntrySplit = 1LL << 17;
+
 void launch_rat(hipStream_t s, const LevelView &F, const LevelView &C, const double *Af, double *Ac) {
 	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
 	if (lastx > 1 && lasty > 1 && lastz > 1) {
-		dim3 gi((lastx - 1 + 63) / 64, (lasty - 1 + 3) / 4, lastz - 1), block(64, 4, 1);
-		hipLaunchKernelGGL(k_rat_interior, gi, block, 0, s, F, C, Af, Ac);
+		const long long ni = (long long)(lastx - 1) * (lasty - 1) * (lastz - 1);
+		if (ni < kRatEntrySplit) {
+			hipLaunchKernelGGL(k_rat_interior_entry, dim3((unsigned)((ni + 255) / 256), 27), dim3(256), 0, s, F, C, Af, Ac);
+		} else {
+			dim3 gi((lastx - 1 + 63) / 64, (lasty - 1 + 3) / 4, lastz - 1), block(64, 4, 1);
+			hipLaunchKernelGGL(k_rat_interior, gi, block, 0, s, F, C, Af, Ac);
+		}
 		HIPCHECK_LAUNCH();
 	}
 	const RatEdges E = rat_edges(F, C);
@@ -2016,12 +2070,15 @@ void launch_lu_solve(hipStream_t s, const LevelView &L, int N, const double *LU,
 
 // ------------------------------------------------- generation, layout
 
-__global__ __launch_bounds__(256) void k_generate(LevelView L, int family, uint64_t seed, double *__restrict__ A,
+// Face-coefficient (7-point) families; the family is a template constant so
+// the row stays in registers.
+template <int FAM>
+__global__ __launch_bounds__(256) void k_generate(LevelView L, uint64_t seed, double *__restrict__ A,
 						  double *__restrict__ r) {
 	const int i = (int)(blockIdx.x * 64 + threadIdx.x), j = (int)(blockIdx.y * 4 + threadIdx.y), k = (int)blockIdx.z;
 	if (i >= L.m || j >= L.n) return;
 	double row[27];
-	pgmg_row27(family, seed, L.m, L.n, L.o, i, j, k, row);
+	pgmg_row27(FAM, seed, L.m, L.n, L.o, i, j, k, row);
 	const long long p = idx3(L, i, j, k);
 	double *Ap = A + aidx3(L, 0, i, j, k);
 #pragma unroll
@@ -2115,7 +2172,10 @@ void launch_generate(hipStream_t s, const LevelView &L, int family, uint64_t see
 		hipLaunchKernelGGL(k_generate_q1, grid, block, 0, s, L, seed, kz, A, r);
 	} else {
 		dim3 grid((L.m + 63) / 64, (L.n + 3) / 4, L.o);
-		hipLaunchKernelGGL(k_generate, grid, block, 0, s, L, family, seed, A, r);
+		if (family == PGMG_FAM_FACE7_LOGUNI)
+			hipLaunchKernelGGL(k_generate<PGMG_FAM_FACE7_LOGUNI>, grid, block, 0, s, L, seed, A, r);
+		else
+			hipLaunchKernelGGL(k_generate<PGMG_FAM_FACE7_UNIFORM>, grid, block, 0, s, L, seed, A, r);
 	}
 	HIPCHECK_LAUNCH();
 }
