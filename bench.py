@@ -198,6 +198,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 513^3, batch and serving legs")
+    ap.add_argument("--leg", choices=["q1_513", "batch_mixed", "batch_fp64", "serving"],
+                    help="run one extra leg alone and print its JSON (for profiling)")
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware counters: a 1 GiB elementwise copy (torch.mul, "
                          "calibration of known size) then two mixed 257^3 V-cycles")
@@ -209,6 +211,12 @@ def main():
 
     import pgmg_amd as P
     pts = N ** 3
+
+    if args.leg:
+        fn = {"q1_513": lambda: q1_513(P), "batch_mixed": lambda: batch_256(P, P.PREC_MIXED),
+              "batch_fp64": lambda: batch_256(P, P.PREC_FP64), "serving": lambda: serving(P)}[args.leg]
+        print(json.dumps({"leg": args.leg, "result": fn()}), flush=True)
+        return
 
     if args.probe:
         import torch

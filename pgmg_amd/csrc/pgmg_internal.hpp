@@ -153,10 +153,12 @@ const int kSmallLdsBytes = 64 * 1024;
 __host__ __device__ inline long long small_pairs_per_pass(const LevelView &L) {
 	return (long long)((L.m + 1) / 2) * ((L.n + 1) / 2) * ((L.o + 1) / 2);
 }
-// A batched level goes to the one-workgroup small-level kernel (as the lone
-// hierarchy's forward colour sweeps), else to the colour-pair kernels.
+// A batched level goes to the one-workgroup small-level kernel when each
+// thread holds one point (coefficients in registers), else to the colour-pair
+// kernels, which spread a level over every CU (the pair-staged one-workgroup
+// form is bound by its CU's L2 bandwidth).
 inline bool batch_level_small(const LevelView &L, int elem) {
-	return small_lds_bytes(L, elem) <= kSmallLdsBytes && ((long long)L.m * L.n * L.o <= 1024 || small_pairs_per_pass(L) <= 1024);
+	return small_lds_bytes(L, elem) <= kSmallLdsBytes && (long long)L.m * L.n * L.o <= 1024;
 }
 
 // Kernel classes for timing.
