@@ -439,18 +439,21 @@ __device__ __forceinline__ void ld_vec_stream(const T *base, uint32_t off, T (&o
 // level whose coefficients fit the 256 MB memory-side cache (MALL) keeps
 // them there from one sweep to the next (split 27); one up to 1 GB keeps a
 // third of its planes there (a cyclic sweep over more than the cache would
-// otherwise evict every line before its reuse); a larger level streams them
-// all (split 0).  Measured on 257^3: FP32 levels 1-2 and FP64 level 2
-// fastest with plain loads, FP64 level 1 (464 MB) 7 % faster with a third
-// cached, level 0 fastest streamed.  Batched slots run side by side, their
-// levels together far larger than the cache: they stream everything
+// otherwise evict every line before its reuse); a larger one keeps one plane
+// if that is at most 80 MB, and streams the rest.  Measured on 257^3: FP32
+// levels 1-2 and FP64 level 2 fastest with plain loads, FP64 level 1
+// (464 MB) 7 % faster with 9 planes cached, FP32 level 0 1.5 % faster with
+// one (two cost level 1 its cached planes).  Batched slots run side by side,
+// their levels together far larger than the cache: they stream everything
 // (measured: 256-problem batch 16.3 s with the per-level rule, 13.6 s
 // streamed).
 template <typename T>
 __device__ __forceinline__ int coef_split(const LevelView &L) {
 	if (!L.cache_coef) return 0;
-	const long long bytes = 27LL * L.m * L.n * L.o * (long long)sizeof(T);
-	return bytes <= (256LL << 20) ? 27 : (bytes <= (1024LL << 20) ? 9 : 0);
+	const long long plane = (long long)L.m * L.n * L.o * (long long)sizeof(T);
+	if (27 * plane <= (256LL << 20)) return 27;
+	if (27 * plane <= (1024LL << 20)) return 9;
+	return plane <= (80LL << 20) ? 1 : 0;
 }
 template <typename T, int N>
 __device__ __forceinline__ void ld_vec_coef(const T *base, uint32_t off, T (&out)[N], bool nt) {
