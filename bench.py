@@ -133,6 +133,38 @@ def q1_513(P):
     return out
 
 
+def pcie_inclusive(P):
+    """The drop-in path with host buffers: the 257^3 operator (27 doubles per
+    point, 3.7 GB) and source handed over from host memory, Galerkin set-up,
+    the mixed solve and the solution copied back, as a caller of
+    setup_matrices / solve / x0 would see it.  Not the headline value."""
+    h = P.Hierarchy(N, N, N, precision=P.PREC_MIXED, smoother=P.SMOOTH_COLOUR8)
+    h.generate_problem(P.FAM_FACE7_UNIFORM, SEED)
+    A = h.get_operator27(0)
+    b = h.get_rhs(0)
+    h.close()
+    h = P.Hierarchy(N, N, N, precision=P.PREC_MIXED, smoother=P.SMOOTH_COLOUR8)
+    h.synchronize()
+    t0 = time.perf_counter()
+    h.set_operator27(A)
+    h.set_rhs(b)
+    h.synchronize()
+    t1 = time.perf_counter()
+    h.compute_rats()
+    h.synchronize()
+    t2 = time.perf_counter()
+    c, hist = h.solve(TOL, 100)
+    h.synchronize()
+    t3 = time.perf_counter()
+    x = h.get_x(0)
+    t4 = time.perf_counter()
+    h.close()
+    del A, b, x
+    return {"total_s": round(t4 - t0, 4), "upload_s": round(t1 - t0, 4), "galerkin_setup_s": round(t2 - t1, 4),
+            "solve_s": round(t3 - t2, 4), "download_s": round(t4 - t3, 4), "cycles": c,
+            "bytes_uploaded": int(28 * 8 * N ** 3)}
+
+
 def batch_256(P, precision, tol=1e-8, count=256, max_cycles=1000):
     """256 independent problems, axes drawn from {33, 65, 129}, per-face
     a = 10^U[-2,2], solved together to tol."""
@@ -198,7 +230,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 513^3, batch and serving legs")
-    ap.add_argument("--leg", choices=["q1_513", "batch_mixed", "batch_fp64", "serving"],
+    ap.add_argument("--leg", choices=["pcie", "q1_513", "batch_mixed", "batch_fp64", "serving"],
                     help="run one extra leg alone and print its JSON (for profiling)")
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware counters: a 1 GiB elementwise copy (torch.mul, "
@@ -213,7 +245,7 @@ def main():
     pts = N ** 3
 
     if args.leg:
-        fn = {"q1_513": lambda: q1_513(P), "batch_mixed": lambda: batch_256(P, P.PREC_MIXED),
+        fn = {"pcie": lambda: pcie_inclusive(P), "q1_513": lambda: q1_513(P), "batch_mixed": lambda: batch_256(P, P.PREC_MIXED),
               "batch_fp64": lambda: batch_256(P, P.PREC_FP64), "serving": lambda: serving(P)}[args.leg]
         print(json.dumps({"leg": args.leg, "result": fn()}), flush=True)
         return
@@ -327,6 +359,8 @@ def main():
                              "avg_launch_us": round(1000.0 * ms_gs64 / max(n_gs64, 1), 2)},
     }
     if not args.no_extra:
+        out["pcie_inclusive_mixed"] = pcie_inclusive(P)
+        log("pcie-inclusive:", out["pcie_inclusive_mixed"])
         out["q1_513"] = q1_513(P)
         out["batch_256_mixed"] = batch_256(P, P.PREC_MIXED)
         log("batch mixed:", out["batch_256_mixed"])
