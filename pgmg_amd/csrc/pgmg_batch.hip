@@ -14,6 +14,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -521,19 +524,121 @@ int pgmg_batch_free_bytes(const pgmg_batch *b, uint64_t *bytes) {
 	return PGMG_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Result readout of the serving loop, off the cycling path: a worker thread
+// copies a finished slot's solution to the host on its own stream while the
+// main loop keeps stepping the batch.  A finished slot is in no plan, so no
+// queued kernel touches its arrays; its arena block is released only after
+// its copy has completed.
+struct Readouts {
+	struct Job {
+		int slot;
+		LevelView lv;
+		const double *x;
+		bool done = false;
+		int rc = PGMG_OK;
+		double t_done = 0;
+	};
+	std::mutex mu;
+	std::condition_variable cv;
+	std::deque<int> queue;  // indices into jobs
+	std::vector<Job> jobs;
+	bool stop = false;
+	hipStream_t stream = nullptr;
+	std::thread worker;
+	std::vector<double> host;  // the worker's destination buffer
+
+	template <typename Clock>
+	void run(Clock now) {
+		for (;;) {
+			int q;
+			Job jb;
+			{
+				std::unique_lock<std::mutex> lk(mu);
+				cv.wait(lk, [&] { return stop || !queue.empty(); });
+				if (queue.empty()) return;
+				q = queue.front();
+				queue.pop_front();
+				jb = jobs[q];
+			}
+			const long long N = (long long)jb.lv.m * jb.lv.n * jb.lv.o;
+			host.resize((size_t)N);
+			int rc = PGMG_OK;
+			double *stage = nullptr;
+			if (hipMallocAsync((void **)&stage, sizeof(double) * (size_t)N, stream) != hipSuccess) rc = PGMG_ERR_HIP;
+			if (!rc) {
+				launch_unpack_vec(stream, jb.lv, jb.x, stage, 0, N);
+				if (hipMemcpyAsync(host.data(), stage, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost, stream) != hipSuccess)
+					rc = PGMG_ERR_HIP;
+				hipFreeAsync(stage, stream);
+				if (hipStreamSynchronize(stream) != hipSuccess) rc = PGMG_ERR_HIP;
+			}
+			const double t = now();
+			std::lock_guard<std::mutex> lk(mu);
+			jobs[q].done = true;
+			jobs[q].rc = rc;
+			jobs[q].t_done = t;
+		}
+	}
+};
+
+}  // namespace
+
+extern "C" {
+
 int pgmg_serve_run(pgmg_batch *b, const pgmg_request *reqs, int n, pgmg_result *res, double *wall_s) {
 	if (!b || (n > 0 && (!reqs || !res))) return bfail(PGMG_ERR_ARG, "null argument");
 	typedef std::chrono::steady_clock clk;
 	const auto t0 = clk::now();
-	auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
+	auto now = [t0]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
 	std::vector<int> order(n);
 	for (int q = 0; q < n; q++) { order[q] = q; res[q].admit_s = res[q].done_s = -1; res[q].cycles = 0; res[q].status = -1; res[q].rel_residual = 0; }
 	std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return reqs[a].arrival_s < reqs[c].arrival_s; });
-	std::vector<int> owner(b->max_slots, -1);
-	std::vector<double> hostx;
+	std::vector<int> owner(b->max_slots, -1);     // request of each slot
+	std::vector<int> reading(b->max_slots, -1);   // readout job of each slot
+	Readouts ro;
+	BHIP(hipStreamCreateWithFlags(&ro.stream, hipStreamNonBlocking));
+	ro.jobs.reserve((size_t)n);
+	ro.worker = std::thread([&ro, now]() { ro.run(now); });
+	auto finish = [&](int rc) {
+		{
+			std::lock_guard<std::mutex> lk(ro.mu);
+			ro.stop = true;
+		}
+		ro.cv.notify_all();
+		ro.worker.join();
+		hipStreamSynchronize(ro.stream);
+		hipStreamDestroy(ro.stream);
+		return rc;
+	};
 	int next = 0, done = 0;
 	while (done < n) {
 		const double t = now();
+		// completed readouts: the result is on the host; the slot is free
+		for (int s = 0; s < b->max_slots; s++) {
+			if (reading[s] < 0) continue;
+			bool fin;
+			Readouts::Job jb;
+			{
+				std::lock_guard<std::mutex> lk(ro.mu);
+				jb = ro.jobs[reading[s]];
+				fin = jb.done;
+			}
+			if (!fin) continue;
+			if (jb.rc) return finish(bfail(jb.rc, "result readout of slot %d failed", s));
+			pgmg_result &R = res[owner[s]];
+			R.done_s = jb.t_done;
+			R.cycles = b->slots[s].cycles;
+			R.rel_residual = b->slots[s].rel;
+			R.status = b->slots[s].state == ST_DONE ? PGMG_OK : PGMG_ERR_NOT_CONVERGED;
+			b->arena.release(b->slots[s].off, b->slots[s].bytes);
+			b->slots[s].state = ST_FREE;
+			reading[s] = owner[s] = -1;
+			done++;
+		}
 		while (next < n && reqs[order[next]].arrival_s <= t) {
 			const pgmg_request &R = reqs[order[next]];
 			pgmg_dims d = {R.m, R.n, R.o, 0, 0, 0};
@@ -545,8 +650,12 @@ int pgmg_serve_run(pgmg_batch *b, const pgmg_request *reqs, int n, pgmg_result *
 			res[order[next]].admit_s = now();
 			next++;
 		}
-		bool busy = false;
-		for (int s = 0; s < b->max_slots; s++) if (owner[s] >= 0) { busy = true; break; }
+		bool busy = false, cycling = false;
+		for (int s = 0; s < b->max_slots; s++) {
+			if (owner[s] < 0) continue;
+			busy = true;
+			if (reading[s] < 0) cycling = true;
+		}
 		if (!busy) {
 			if (next < n) {
 				const double wait = reqs[order[next]].arrival_s - now();
@@ -555,29 +664,33 @@ int pgmg_serve_run(pgmg_batch *b, const pgmg_request *reqs, int n, pgmg_result *
 			}
 			break;
 		}
+		if (!cycling) {
+			// only readouts in flight
+			std::this_thread::sleep_for(std::chrono::microseconds(50));
+			continue;
+		}
 		int act = 0, fin = 0;
 		int rc = pgmg_batch_step(b, &act, &fin);
-		if (rc) return rc;
+		if (rc) return finish(rc);
 		for (int s = 0; s < b->max_slots; s++) {
-			if (owner[s] < 0) continue;
+			if (owner[s] < 0 || reading[s] >= 0) continue;
 			const int st = b->slots[s].state;
 			if (st != ST_DONE && st != ST_FAILED) continue;
-			const LevelView &L = b->slots[s].dev.lv[0];
-			hostx.resize((size_t)L.m * L.n * L.o);
-			rc = pgmg_batch_get_x(b, s, hostx.data());  // the result reaches the host
-			if (rc) return rc;
-			pgmg_result &R = res[owner[s]];
-			R.done_s = now();
-			R.cycles = b->slots[s].cycles;
-			R.rel_residual = b->slots[s].rel;
-			R.status = st == ST_DONE ? PGMG_OK : PGMG_ERR_NOT_CONVERGED;
-			pgmg_batch_release(b, s);
-			owner[s] = -1;
-			done++;
+			// the step synchronized after its residual: this slot's x is final
+			// and the cycle just queued does not touch it
+			Readouts::Job jb;
+			jb.slot = s;
+			jb.lv = b->slots[s].dev.lv[0];
+			jb.x = b->slots[s].dev.x[0];
+			std::lock_guard<std::mutex> lk(ro.mu);
+			reading[s] = (int)ro.jobs.size();
+			ro.jobs.push_back(jb);
+			ro.queue.push_back(reading[s]);
+			ro.cv.notify_one();
 		}
 	}
 	if (wall_s) *wall_s = now();
-	return PGMG_OK;
+	return finish(PGMG_OK);
 }
 
 }  // extern "C"
