@@ -1645,74 +1645,77 @@ __device__ __forceinline__ double rat_entry(const LevelView &F, const LevelView 
 	psupport(JK, F.o, plz, phz, pcz);
 	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
 	const bool evx = !(F.m & 1), evy = !(F.n & 1), evz = !(F.o & 1);
+	// The loops are unrolled with the fine point's validity as a predicate
+	// instead of a branch, and its row block is loaded from a clamped (always
+	// valid) address, so the loads of several fine points can be in flight
+	// at once; the terms are still added in cube-scan order.
+#pragma unroll
 	for (int cz = 0; cz < 2; cz++) {
 		const int kc = K - 1 + cz;
-		if (kc < 0 || kc > lastz) continue;
+		const bool vkc = kc >= 0 && kc <= lastz;
 		const bool sz = evz && kc == lastz;
+#pragma unroll
 		for (int cy = 0; cy < 2; cy++) {
 			const int jc = Jy - 1 + cy;
-			if (jc < 0 || jc > lasty) continue;
+			const bool vjc = vkc && jc >= 0 && jc <= lasty;
 			const bool sy = evy && jc == lasty;
+#pragma unroll
 			for (int cx = 0; cx < 2; cx++) {
 				const int ic = I - 1 + cx;
-				if (ic < 0 || ic > lastx) continue;
+				const bool vic = vjc && ic >= 0 && ic <= lastx;
 				const bool sx = evx && ic == lastx;
+#pragma unroll
 				for (int lz = cz ? 0 : 1; lz < 2; lz++) {
 					const int fk = 2 * kc + lz;
-					if (fk >= F.o) continue;
-					double wz;
-					if (lz == 0) wz = 1.0;
-					else if (cz == 0) wz = sz ? 1.0 : 0.5;
-					else { if (sz) continue; wz = 0.5; }
+					// weight of f in the cube's restriction; a second-cube local
+					// 1 past an even-size last point does not exist
+					const double wz = lz == 0 ? 1.0 : (cz == 0 ? (sz ? 1.0 : 0.5) : 0.5);
 					const int gzl = max(plz, fk - 1), gzh = min(phz, min(fk + 2, F.o));
-					if (gzl >= gzh) continue;
+					const bool vz = vic && fk < F.o && !(cz == 1 && lz == 1 && sz) && gzl < gzh;
+#pragma unroll
 					for (int ly = cy ? 0 : 1; ly < 2; ly++) {
 						const int fj = 2 * jc + ly;
-						if (fj >= F.n) continue;
-						double wy;
-						if (ly == 0) wy = 1.0;
-						else if (cy == 0) wy = sy ? 1.0 : 0.5;
-						else { if (sy) continue; wy = 0.5; }
+						const double wy = ly == 0 ? 1.0 : (cy == 0 ? (sy ? 1.0 : 0.5) : 0.5);
 						const int gyl = max(ply, fj - 1), gyh = min(phy, min(fj + 2, F.n));
-						if (gyl >= gyh) continue;
+						const bool vy = vz && fj < F.n && !(cy == 1 && ly == 1 && sy) && gyl < gyh;
+#pragma unroll
 						for (int lx = cx ? 0 : 1; lx < 2; lx++) {
 							const int fi = 2 * ic + lx;
-							if (fi >= F.m) continue;
-							double wx;
-							if (lx == 0) wx = 1.0;
-							else if (cx == 0) wx = sx ? 1.0 : 0.5;
-							else { if (sx) continue; wx = 0.5; }
+							const double wx = lx == 0 ? 1.0 : (cx == 0 ? (sx ? 1.0 : 0.5) : 0.5);
 							const int gxl = max(plx, fi - 1), gxh = min(phx, min(fi + 2, F.m));
-							if (gxl >= gxh) continue;
-							// all 27 planes of f's row block are valid memory: load
-							// them at once, then sum the terms inside J's support
-							// in (k,j,i) order
-							const double *Afr = Af + aidx3(F, 0, fi, fj, fk);
+							const bool valid = vy && fi < F.m && !(cx == 1 && lx == 1 && sx) && gxl < gxh;
+							// all 27 planes of a fine point's row block are valid
+							// memory; an invalid f reads a clamped in-grid one
+							const int ci = min(max(fi, 0), F.m - 1), cj2 = min(max(fj, 0), F.n - 1),
+								  ck2 = min(max(fk, 0), F.o - 1);
+							const double *Afr = Af + aidx3(F, 0, ci, cj2, ck2);
 							double a[27];
 #pragma unroll
 							for (int ef = 0; ef < 27; ef++) a[ef] = Afr[ef * F.px];
-							double inner = 0.0;
+							if (valid) {
+								double inner = 0.0;
 #pragma unroll
-							for (int dk = 0; dk < 3; dk++) {
-								const int kk = fk - 1 + dk;
-								const bool okk = kk >= gzl && kk < gzh;
-								const double kfac = kk == pcz ? 1.0 : 0.5;
+								for (int dk = 0; dk < 3; dk++) {
+									const int kk = fk - 1 + dk;
+									const bool okk = kk >= gzl && kk < gzh;
+									const double kfac = kk == pcz ? 1.0 : 0.5;
 #pragma unroll
-								for (int dj = 0; dj < 3; dj++) {
-									const int jj = fj - 1 + dj;
-									const bool okj = okk && jj >= gyl && jj < gyh;
-									const double jkfac = jj == pcy ? kfac : 0.5 * kfac;
+									for (int dj = 0; dj < 3; dj++) {
+										const int jj = fj - 1 + dj;
+										const bool okj = okk && jj >= gyl && jj < gyh;
+										const double jkfac = jj == pcy ? kfac : 0.5 * kfac;
 #pragma unroll
-									for (int di = 0; di < 3; di++) {
-										const int ii = fi - 1 + di;
-										if (okj && ii >= gxl && ii < gxh)
-											inner = inner + (0.125 * a[dk * 9 + dj * 3 + di]) *
-													    (ii == pcx ? jkfac : 0.5 * jkfac);
+										for (int di = 0; di < 3; di++) {
+											const int ii = fi - 1 + di;
+											if (okj && ii >= gxl && ii < gxh)
+												inner = inner + (0.125 * a[dk * 9 + dj * 3 + di]) *
+														    (ii == pcx ? jkfac : 0.5 * jkfac);
+										}
 									}
 								}
+								const double wr = wx * wy * wz;
+								acc = acc + inner * wr;
 							}
-							const double wr = wx * wy * wz;
-							acc = acc + inner * wr;
 						}
 					}
 				}
