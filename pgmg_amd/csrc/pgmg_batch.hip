@@ -644,7 +644,13 @@ int pgmg_serve_run(pgmg_batch *b, const pgmg_request *reqs, int n, pgmg_result *
 			pgmg_dims d = {R.m, R.n, R.o, 0, 0, 0};
 			int slot = -1;
 			int rc = pgmg_batch_admit(b, &d, R.family, R.seed, R.tol, R.max_cycles, &slot);
-			if (rc == PGMG_ERR_ADMISSION) break;  // wait for capacity (FIFO)
+			if (rc == PGMG_ERR_ADMISSION) {
+				bool any = false;
+				for (int q = 0; q < b->max_slots; q++) any = any || owner[q] >= 0;
+				if (any) break;  // wait for capacity (FIFO)
+				// the batch is empty and the request still does not fit: it
+				// never will (larger than the arena, or too many levels)
+			}
 			if (rc) { res[order[next]].status = rc; res[order[next]].done_s = now(); done++; next++; continue; }
 			owner[slot] = order[next];
 			res[order[next]].admit_s = now();

@@ -127,3 +127,17 @@ def test_serving_open_loop_small():
         assert r["admit_s"] >= q["arrival_s"] - 1e-9
         assert r["done_s"] > r["admit_s"]
     assert wall > 0
+
+
+def test_serving_request_that_never_fits():
+    """A request larger than the whole arena is reported (admission error)
+    instead of waiting forever; the others are served."""
+    fp = P.footprint(33, 33, 33, precision=P.PREC_MIXED)
+    reqs = [dict(arrival_s=0.0, m=33, n=33, o=33, family=P.FAM_Q1_ANISO, seed=1, tol=1e-8, max_cycles=60),
+            dict(arrival_s=0.0, m=129, n=129, o=129, family=P.FAM_Q1_ANISO, seed=2, tol=1e-8, max_cycles=60),
+            dict(arrival_s=0.001, m=33, n=33, o=33, family=P.FAM_Q1_ANISO, seed=3, tol=1e-8, max_cycles=60)]
+    with P.Batch(int(fp * 1.5), max_slots=4) as b:
+        res, wall = b.serve(reqs)
+        assert b.free_bytes() == int(fp * 1.5)
+    assert res[0]["status"] == 0 and res[2]["status"] == 0
+    assert res[1]["status"] == 10  # PGMG_ERR_ADMISSION
