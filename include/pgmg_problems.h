@@ -12,7 +12,10 @@
  * e = (dk+1)*9 + (dj+1)*3 + (di+1) couples point (i,j,k) to (i+di,j+dj,k+dk).
  * This is the box order of the variable-stencil format in region.hh:95-101
  * (k outer, i inner), with the box fixed to [-1,1]^3 and clipped entries set
- * to zero.  Grids are non-periodic with zero Dirichlet data outside.
+ * to zero.  An axis is either non-periodic, with zero Dirichlet data
+ * outside the grid (clipped entries), or periodic (bit 0 of prd for x, bit 1
+ * for y, bit 2 for z), where the row keeps every entry and an offset that
+ * leaves the grid couples to the point wrapped around the axis.
  *
  * Families:
  *  PGMG_FAM_FACE7_UNIFORM  -div(a grad u), 7-point, per-face a ~ U[1,10]
@@ -182,10 +185,15 @@ PGMG_HD static inline double pgmg_q1_entry(const double K[6], int a, int b) {
 	return s;
 }
 
-/* Row (i,j,k) of the operator, in 27-entry order, zero outside the grid. */
-PGMG_HD static inline void pgmg_row27(int family, uint64_t seed, int m, int n, int o,
+/* Row (i,j,k) of the operator, in 27-entry order, on a grid whose periodic
+ * axes are the set bits of prd.  On a non-periodic axis entries that leave
+ * the grid are zero.  On a periodic axis the face (7-point) or cell (Q1)
+ * index 0 is the same face or cell as index m, so the coefficients wrap
+ * consistently and the operator stays symmetric. */
+PGMG_HD static inline void pgmg_row27_prd(int family, uint64_t seed, int m, int n, int o, int prd,
 		int i, int j, int k, double out[27]) {
 	PGMG_NO_CONTRACT
+	const int px = prd & 1, py = (prd >> 1) & 1, pz = (prd >> 2) & 1;
 	int e;
 	for (e = 0; e < 27; e++) out[e] = 0.0;
 	if (family == PGMG_FAM_Q1_ANISO) {
@@ -196,31 +204,38 @@ PGMG_HD static inline void pgmg_row27(int family, uint64_t seed, int m, int n, i
 		for (dc2 = 0; dc2 < 2; dc2++) for (dc1 = 0; dc1 < 2; dc1++) for (dc0 = 0; dc0 < 2; dc0++) {
 			int ci = i + dc0, cj = j + dc1, ck = k + dc2;
 			double K[6];
-			pgmg_cell_tensor(seed, m, n, ci, cj, ck, K);
+			pgmg_cell_tensor(seed, m, n, (px && ci == 0) ? m : ci, (py && cj == 0) ? n : cj,
+					 (pz && ck == 0) ? o : ck, K);
 			int la = (1 - dc0) | ((1 - dc1) << 1) | ((1 - dc2) << 2);
 			int lb;
 			for (lb = 0; lb < 8; lb++) {
 				int bi = ci - 1 + (lb & 1), bj = cj - 1 + ((lb >> 1) & 1), bk = ck - 1 + ((lb >> 2) & 1);
-				if (bi < 0 || bi >= m || bj < 0 || bj >= n || bk < 0 || bk >= o) continue;
+				if ((!px && (bi < 0 || bi >= m)) || (!py && (bj < 0 || bj >= n)) || (!pz && (bk < 0 || bk >= o))) continue;
 				int idx = (bk - k + 1) * 9 + (bj - j + 1) * 3 + (bi - i + 1);
 				out[idx] = out[idx] + pgmg_q1_entry(K, la, lb);
 			}
 		}
 		return;
 	}
-	double axl = pgmg_face_coeff(family, seed, m, n, o, 0, i, j, k);
+	double axl = pgmg_face_coeff(family, seed, m, n, o, 0, (px && i == 0) ? m : i, j, k);
 	double axh = pgmg_face_coeff(family, seed, m, n, o, 0, i + 1, j, k);
-	double ayl = pgmg_face_coeff(family, seed, m, n, o, 1, i, j, k);
+	double ayl = pgmg_face_coeff(family, seed, m, n, o, 1, i, (py && j == 0) ? n : j, k);
 	double ayh = pgmg_face_coeff(family, seed, m, n, o, 1, i, j + 1, k);
-	double azl = pgmg_face_coeff(family, seed, m, n, o, 2, i, j, k);
+	double azl = pgmg_face_coeff(family, seed, m, n, o, 2, i, j, (pz && k == 0) ? o : k);
 	double azh = pgmg_face_coeff(family, seed, m, n, o, 2, i, j, k + 1);
 	out[13] = ((((axl + axh) + ayl) + ayh) + azl) + azh;
-	if (i > 0) out[12] = -axl;
-	if (i < m - 1) out[14] = -axh;
-	if (j > 0) out[10] = -ayl;
-	if (j < n - 1) out[16] = -ayh;
-	if (k > 0) out[4] = -azl;
-	if (k < o - 1) out[22] = -azh;
+	if (px || i > 0) out[12] = -axl;
+	if (px || i < m - 1) out[14] = -axh;
+	if (py || j > 0) out[10] = -ayl;
+	if (py || j < n - 1) out[16] = -ayh;
+	if (pz || k > 0) out[4] = -azl;
+	if (pz || k < o - 1) out[22] = -azh;
+}
+
+/* Row (i,j,k) of the operator on a non-periodic grid. */
+PGMG_HD static inline void pgmg_row27(int family, uint64_t seed, int m, int n, int o,
+		int i, int j, int k, double out[27]) {
+	pgmg_row27_prd(family, seed, m, n, o, 0, i, j, k, out);
 }
 
 #endif

@@ -48,6 +48,21 @@ def lib():
             "orc_lu_factor": (_i, [_i, _i, _i, _D, _D, _I]),
             "orc_lu_solve": (None, [_i, _D, _I, _D, _D]),
             "orc_mg_create": (_VP, [_i, _i, _i, _D]),
+            "orc_mg_create_p": (_VP, [_i, _i, _i, _i, _D]),
+            "orc_hierarchy_p": (_i, [_i, _i, _i, _i, _I]),
+            "orc_gen_operator_p": (None, [_i, _u64, _i, _i, _i, _i, _D]),
+            "orc_gs_lex_p": (None, [_i, _i, _i, _i, _D, _D, _D, _i]),
+            "orc_gs_colour8_p": (None, [_i, _i, _i, _i, _D, _D, _D, _i]),
+            "orc_gs_lex_p_f": (None, [_i, _i, _i, _i, _F, _F, _F, _i]),
+            "orc_gs_colour8_p_f": (None, [_i, _i, _i, _i, _F, _F, _F, _i]),
+            "orc_residual_p": (None, [_i, _i, _i, _i, _D, _D, _D, _D]),
+            "orc_l2_error_all_p": (ctypes.c_double, [_i, _i, _i, _i, _D, _D, _D]),
+            "orc_restrict_p": (None, [_i, _i, _i, _i, _D, _D, _D, _i, _i, _i, _D]),
+            "orc_restrict_p_f": (None, [_i, _i, _i, _i, _F, _F, _F, _i, _i, _i, _F]),
+            "orc_interp_p": (None, [_i, _i, _i, _i, _D, _i, _i, _i, _D]),
+            "orc_interp_p_f": (None, [_i, _i, _i, _i, _F, _i, _i, _i, _F]),
+            "orc_rat_p": (None, [_i, _i, _i, _i, _D, _i, _i, _i, _D]),
+            "orc_lu_factor_p": (_i, [_i, _i, _i, _i, _D, _D, _I]),
             "orc_mg_destroy": (None, [_VP]),
             "orc_mg_levels": (_i, [_VP]),
             "orc_mg_set_smoother": (None, [_VP, _i]),
@@ -84,15 +99,19 @@ def _p(a):
     return a.ctypes.data_as(_I)
 
 
-def hierarchy(m, n, o):
+def hierarchy(m, n, o, prd=0):
+    """Level sizes; prd holds the periodic axes (bit 0 x, bit 1 y, bit 2 z)."""
     d = np.zeros(48, dtype=np.int32)
-    nl = lib().orc_hierarchy(m, n, o, _p(d))
+    nl = lib().orc_hierarchy_p(m, n, o, prd, _p(d))
     return [tuple(int(v) for v in d[3 * l:3 * l + 3]) for l in range(nl)]
 
 
-def gen_operator(family, seed, m, n, o):
+def gen_operator(family, seed, m, n, o, prd=0):
     A = np.zeros((o, n, m, 27), dtype=np.float64)
-    lib().orc_gen_operator(family, seed, m, n, o, _p(A))
+    if prd:
+        lib().orc_gen_operator_p(family, seed, m, n, o, prd, _p(A))
+    else:
+        lib().orc_gen_operator(family, seed, m, n, o, _p(A))
     return A
 
 
@@ -137,65 +156,85 @@ def _arr(a, dt):
     return np.ascontiguousarray(a, dtype=dt)
 
 
-def gs(A, x, r, mode=0, colour8=False, fp32=False):
+def gs(A, x, r, mode=0, colour8=False, fp32=False, prd=0):
     o, n, m = x.shape
     dt = np.float32 if fp32 else np.float64
     A = _arr(A, dt); r = _arr(r, dt); x = _arr(x, dt).copy()
+    if prd:
+        f = {(False, False): lib().orc_gs_lex_p, (True, False): lib().orc_gs_colour8_p,
+             (False, True): lib().orc_gs_lex_p_f, (True, True): lib().orc_gs_colour8_p_f}[(colour8, fp32)]
+        f(m, n, o, prd, _p(A), _p(x), _p(r), mode)
+        return x
     f = {(False, False): lib().orc_gs_lex, (True, False): lib().orc_gs_colour8,
          (False, True): lib().orc_gs_lex_f, (True, True): lib().orc_gs_colour8_f}[(colour8, fp32)]
     f(m, n, o, _p(A), _p(x), _p(r), mode)
     return x
 
 
-def residual(A, x, r):
+def residual(A, x, r, prd=0):
     o, n, m = x.shape
     A = _arr(A, np.float64); x = _arr(x, np.float64); r = _arr(r, np.float64)
     out = np.zeros_like(x)
-    lib().orc_residual(m, n, o, _p(A), _p(x), _p(r), _p(out))
+    if prd:
+        lib().orc_residual_p(m, n, o, prd, _p(A), _p(x), _p(r), _p(out))
+    else:
+        lib().orc_residual(m, n, o, _p(A), _p(x), _p(r), _p(out))
     return out
 
 
-def l2_error_all(A, x, r):
+def l2_error_all(A, x, r, prd=0):
     o, n, m = x.shape
     A = _arr(A, np.float64); x = _arr(x, np.float64); r = _arr(r, np.float64)
+    if prd:
+        return lib().orc_l2_error_all_p(m, n, o, prd, _p(A), _p(x), _p(r))
     return lib().orc_l2_error_all(m, n, o, _p(A), _p(x), _p(r))
 
 
-def restrict(Af, xf, rf, coarse, fp32=False):
+def restrict(Af, xf, rf, coarse, fp32=False, prd=0):
     of, nf, mf = xf.shape
     mc, nc, oc = coarse
     dt = np.float32 if fp32 else np.float64
     Af = _arr(Af, dt); xf = _arr(xf, dt); rf = _arr(rf, dt)
     rc = np.zeros((oc, nc, mc), dtype=dt)
-    (lib().orc_restrict_f if fp32 else lib().orc_restrict)(mf, nf, of, _p(Af), _p(xf), _p(rf), mc, nc, oc, _p(rc))
+    if prd:
+        (lib().orc_restrict_p_f if fp32 else lib().orc_restrict_p)(mf, nf, of, prd, _p(Af), _p(xf), _p(rf),
+                                                                   mc, nc, oc, _p(rc))
+    else:
+        (lib().orc_restrict_f if fp32 else lib().orc_restrict)(mf, nf, of, _p(Af), _p(xf), _p(rf), mc, nc, oc, _p(rc))
     return rc
 
 
-def interp(xc, xf, fp32=False):
+def interp(xc, xf, fp32=False, prd=0):
     oc, nc, mc = xc.shape
     of, nf, mf = xf.shape
     dt = np.float32 if fp32 else np.float64
     xc = _arr(xc, dt); out = _arr(xf, dt).copy()
-    (lib().orc_interp_f if fp32 else lib().orc_interp)(mc, nc, oc, _p(xc), mf, nf, of, _p(out))
+    if prd:
+        (lib().orc_interp_p_f if fp32 else lib().orc_interp_p)(mc, nc, oc, prd, _p(xc), mf, nf, of, _p(out))
+    else:
+        (lib().orc_interp_f if fp32 else lib().orc_interp)(mc, nc, oc, _p(xc), mf, nf, of, _p(out))
     return out
 
 
-def rat(Af, coarse):
+def rat(Af, coarse, prd=0):
     of, nf, mf = Af.shape[:3]
     mc, nc, oc = coarse
     Af = _arr(Af, np.float64)
     Ac = np.zeros((oc, nc, mc, 27), dtype=np.float64)
-    lib().orc_rat(mf, nf, of, _p(Af), mc, nc, oc, _p(Ac))
+    if prd:
+        lib().orc_rat_p(mf, nf, of, prd, _p(Af), mc, nc, oc, _p(Ac))
+    else:
+        lib().orc_rat(mf, nf, of, _p(Af), mc, nc, oc, _p(Ac))
     return Ac
 
 
-def lu_factor(A):
+def lu_factor(A, prd=0):
     o, n, m = A.shape[:3]
     N = m * n * o
     A = _arr(A, np.float64)
     LU = np.zeros((N, N), dtype=np.float64)  # column-major: LU[col, row]
     ipiv = np.zeros(N, dtype=np.int32)
-    ok = lib().orc_lu_factor(m, n, o, _p(A), _p(LU), _p(ipiv))
+    ok = lib().orc_lu_factor_p(m, n, o, prd, _p(A), _p(LU), _p(ipiv))
     return ok, LU, ipiv
 
 
@@ -210,10 +249,10 @@ def lu_solve(LU, ipiv, b):
 class Multigrid:
     """The oracle's hierarchy: mirrors multigrid<p_class> at one rank."""
 
-    def __init__(self, A, colour8=False, lu=False, v_down=2, v_up=2):
-        self.dims = hierarchy(A.shape[2], A.shape[1], A.shape[0])
+    def __init__(self, A, colour8=False, lu=False, v_down=2, v_up=2, prd=0):
+        self.dims = hierarchy(A.shape[2], A.shape[1], A.shape[0], prd)
         A = _arr(A, np.float64)
-        self.h = lib().orc_mg_create(A.shape[2], A.shape[1], A.shape[0], _p(A))
+        self.h = lib().orc_mg_create_p(A.shape[2], A.shape[1], A.shape[0], prd, _p(A))
         lib().orc_mg_set_smoother(self.h, 1 if colour8 else 0)
         lib().orc_mg_set_sweeps(self.h, v_down, v_up)
         self.lu = bool(lib().orc_mg_enable_lu(self.h)) if lu else False

@@ -57,6 +57,36 @@ void orc_gen_rhs(uint64_t seed, int m, int n, int o, double *b) {
 		b[IDX(i, j, k, m, n)] = pgmg_rhs(seed, m, n, o, i, j, k);
 }
 
+/* Coarse size on a periodic axis: (m+1)>>1 (region.cc:14). */
+int orc_hierarchy_p(int m, int n, int o, int prd, int *dims) {
+	int l = 0, a;
+	dims[0] = m; dims[1] = n; dims[2] = o;
+	while ((long)dims[3 * l] * dims[3 * l + 1] * dims[3 * l + 2] > 1024) {
+		if (l + 1 == ORC_MAX_LEVELS) return -1;
+		for (a = 0; a < 3; a++) dims[3 * (l + 1) + a] = (dims[3 * l + a] + (((prd >> a) & 1) ? 1 : 2)) >> 1;
+		l++;
+	}
+	return l + 1;
+}
+
+void orc_gen_operator_p(int family, uint64_t seed, int m, int n, int o, int prd, double *A27) {
+	int i, j, k;
+	for (k = 0; k < o; k++) for (j = 0; j < n; j++) for (i = 0; i < m; i++)
+		pgmg_row27_prd(family, seed, m, n, o, prd, i, j, k, A27 + 27 * IDX(i, j, k, m, n));
+}
+
+/* Neighbour (i,j,k) of a grid with periodic axes prd: -1 if it lies outside
+ * a non-periodic axis, else its linear index in *q and 1 if it was reached
+ * across a periodic boundary (a ghost value), 0 if not. */
+static int orc_nb(int m, int n, int o, int prd, int i, int j, int k, size_t *q) {
+	int g = 0;
+	if (i < 0 || i >= m) { if (!(prd & 1)) return -1; i = (i + m) % m; g = 1; }
+	if (j < 0 || j >= n) { if (!(prd & 2)) return -1; j = (j + n) % n; g = 1; }
+	if (k < 0 || k >= o) { if (!(prd & 4)) return -1; k = (k + o) % o; g = 1; }
+	*q = IDX(i, j, k, m, n);
+	return g;
+}
+
 /* ------------------------------------------------- generic level kernels */
 
 #define REAL double
@@ -77,6 +107,16 @@ double orc_l2_error_all(int m, int n, int o, const double *A, const double *x, c
 	int i, j, k;
 	for (k = 0; k < o; k++) for (j = 0; j < n; j++) for (i = 0; i < m; i++) {
 		double rem = orc_residual_at(m, n, o, A, x, r, i, j, k);
+		err += rem * rem;
+	}
+	return err;
+}
+
+double orc_l2_error_all_p(int m, int n, int o, int prd, const double *A, const double *x, const double *r) {
+	double err = 0;
+	int i, j, k;
+	for (k = 0; k < o; k++) for (j = 0; j < n; j++) for (i = 0; i < m; i++) {
+		double rem = orc_residual_at_p(m, n, o, prd, A, x, r, i, j, k);
 		err += rem * rem;
 	}
 	return err;
@@ -195,6 +235,82 @@ void orc_rat(int mf, int nf, int of, const double *Af, int mc, int nc, int oc, d
 	}
 }
 
+/* One axis of the interpolation P on the unwrapped index line.  A periodic
+ * axis repeats with period M (fine) and Mc (coarse): coarse J sits on fine
+ * 2(J mod Mc) + M floor(J/Mc) (for odd M the last coarse point, fine M-1,
+ * and coarse 0 of the next period, fine M, are neighbours; igrid_map,
+ * region.cc:1233-1242).  On a non-periodic axis coarse J sits on fine 2J,
+ * except the last coarse point of an even M, on fine M-1 (region.cc:1243-1255).
+ * A fine point on a coarse position takes that coarse value alone; any
+ * other fine point half of each coarse neighbour. */
+static int p_fpos(int J, int M, int Mc, int prd) {
+	if (prd) {
+		int q = J >= 0 ? J / Mc : -((-J + Mc - 1) / Mc);
+		return 2 * (J - q * Mc) + M * q;
+	}
+	return (!(M & 1) && J == Mc - 1) ? M - 1 : 2 * J;
+}
+static double p_weight(int g, int J, int M, int Mc, int prd) {
+	int c, f = p_fpos(J, M, Mc, prd);
+	if (prd) c = ((g % M + M) % M & 1) == 0;
+	else c = (g & 1) == 0 || (!(M & 1) && g == M - 1);
+	if (c) return g == f ? 1.0 : 0.0;
+	return (g - f == 1 || f - g == 1) ? 0.5 : 0.0;
+}
+
+/* Galerkin coarse operator Ac = P^T Af P / 8 on a grid with periodic axes,
+ * on the unwrapped index line of every periodic axis, where each coarse box
+ * stays within [-1,1]^3 (the reference's periodic grid_map/igrid_map,
+ * region.cc:1201-1242).  Entry d of coarse row I is
+ *   sum over fine f with P(f,I) != 0, in (k,j,i) order, of
+ *   P(f,I) * [sum over f's box g in (k,j,i) order of 0.125*Af(f,g)*P(g,I+d)],
+ * the bracket formed as A_red forms At (region.cc:1117-1166) and scaled as
+ * A_mul does (region.hh:460-465).  The reference adds the fine points in
+ * cube-scan order and the wrapped ones last, through its RAT strips, so its
+ * coarse entries agree with these to rounding, not bitwise. */
+void orc_rat_p(int mf, int nf, int of, int prd, const double *Af, int mc, int nc, int oc, double *Ac) {
+	const int P[3] = {prd & 1, (prd >> 1) & 1, (prd >> 2) & 1};
+	const int Mf[3] = {mf, nf, of}, Mc[3] = {mc, nc, oc};
+	int I[3], d, a;
+	for (I[2] = 0; I[2] < oc; I[2]++) for (I[1] = 0; I[1] < nc; I[1]++) for (I[0] = 0; I[0] < mc; I[0]++)
+	for (d = 0; d < 27; d++) {
+		int J[3] = {I[0] + d % 3 - 1, I[1] + (d / 3) % 3 - 1, I[2] + d / 9 - 1}, ok = 1, fc[3];
+		double acc = 0;
+		for (a = 0; a < 3; a++) {
+			if (!P[a] && (J[a] < 0 || J[a] >= Mc[a])) ok = 0;
+			fc[a] = p_fpos(I[a], Mf[a], Mc[a], P[a]);
+		}
+		if (ok) {
+			int f[3];
+			for (f[2] = fc[2] - 1; f[2] <= fc[2] + 1; f[2]++)
+			for (f[1] = fc[1] - 1; f[1] <= fc[1] + 1; f[1]++)
+			for (f[0] = fc[0] - 1; f[0] <= fc[0] + 1; f[0]++) {
+				double w = 1;
+				int fw[3], e;
+				for (a = 0; a < 3; a++) {
+					if (!P[a] && (f[a] < 0 || f[a] >= Mf[a])) w = 0;
+					else w *= p_weight(f[a], I[a], Mf[a], Mc[a], P[a]);
+					fw[a] = ((f[a] % Mf[a]) + Mf[a]) % Mf[a];
+				}
+				if (w == 0) continue;
+				const double *Arow = Af + 27 * IDX(fw[0], fw[1], fw[2], mf, nf);
+				double sum = 0;
+				for (e = 0; e < 27; e++) {
+					int g[3] = {f[0] + e % 3 - 1, f[1] + (e / 3) % 3 - 1, f[2] + e / 9 - 1};
+					double pw = 1;
+					for (a = 0; a < 3; a++) {
+						if (!P[a] && (g[a] < 0 || g[a] >= Mf[a])) pw = 0;
+						else pw *= p_weight(g[a], J[a], Mf[a], Mc[a], P[a]);
+					}
+					if (pw != 0) sum += 0.125 * Arow[e] * pw;
+				}
+				acc += sum * w;
+			}
+		}
+		Ac[27 * IDX(I[0], I[1], I[2], mc, nc) + d] = acc;
+	}
+}
+
 /* ------------------------------------------------------------ bottom LU */
 
 /* Dense assembly as lu_decomposition, region_tr.cc:486-504 (column-major,
@@ -203,18 +319,8 @@ void orc_rat(int mf, int nf, int of, const double *Af, int mc, int nc, int oc, d
  * dgetrf's specification (region_tr.cc:518).  The usability test follows
  * region_tr.cc:526-533.  LAPACK's blocked dgetrf may round differently;
  * this factor is what the GPU kernel reproduces. */
-int orc_lu_factor(int m, int n, int o, const double *A27, double *LU, int *ipiv) {
-	int N = m * n * o, i, j, k, c, r, info = 0;
-	memset(LU, 0, sizeof(double) * (size_t)N * N);
-	for (k = 0; k < o; k++) for (j = 0; j < n; j++) for (i = 0; i < m; i++) {
-		int p = (int)IDX(i, j, k, m, n), e;
-		for (e = 0; e < 27; e++) {
-			int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
-			int qi = i + di, qj = j + dj, qk = k + dk;
-			if (qi < 0 || qi >= m || qj < 0 || qj >= n || qk < 0 || qk >= o) continue;
-			LU[p + (size_t)IDX(qi, qj, qk, m, n) * N] = A27[27 * (size_t)p + e];
-		}
-	}
+static int lu_factor_dense(int N, double *LU, int *ipiv) {
+	int c, r, j, info = 0;
 	for (c = 0; c < N; c++) {
 		int piv = c;
 		double best = fabs(LU[c + (size_t)c * N]);
@@ -239,6 +345,28 @@ int orc_lu_factor(int m, int n, int o, const double *A27, double *LU, int *ipiv)
 	return 1;
 }
 
+int orc_lu_factor(int m, int n, int o, const double *A27, double *LU, int *ipiv) {
+	return orc_lu_factor_p(m, n, o, 0, A27, LU, ipiv);
+}
+
+/* With periodic axes the column of a wrapped neighbour is its wrapped
+ * index (step_mod, region.hh:466; region_tr.cc:493-501).  Entries are
+ * stored, not added, in box order, so on an axis of size 1 or 2, where two
+ * offsets reach one column, the later one is kept, as in the reference. */
+int orc_lu_factor_p(int m, int n, int o, int prd, const double *A27, double *LU, int *ipiv) {
+	int N = m * n * o, i, j, k;
+	memset(LU, 0, sizeof(double) * (size_t)N * N);
+	for (k = 0; k < o; k++) for (j = 0; j < n; j++) for (i = 0; i < m; i++) {
+		int p = (int)IDX(i, j, k, m, n), e;
+		for (e = 0; e < 27; e++) {
+			size_t q;
+			if (orc_nb(m, n, o, prd, i + e % 3 - 1, j + (e / 3) % 3 - 1, k + e / 9 - 1, &q) < 0) continue;
+			LU[p + q * N] = A27[27 * (size_t)p + e];
+		}
+	}
+	return lu_factor_dense(N, LU, ipiv);
+}
+
 /* dgetrs('N') for one right-hand side (region_tr.cc:544-555): row
  * interchanges, then column-oriented unit-lower and upper triangular
  * substitutions. */
@@ -260,7 +388,7 @@ void orc_lu_solve(int N, const double *LU, const int *ipiv, const double *b, dou
 /* ------------------------------------------------------------ hierarchy */
 
 struct orc_mg {
-	int nl, smoother, v_down, v_up, lu_ok;
+	int nl, smoother, v_down, v_up, lu_ok, prd;
 	int dims[3 * ORC_MAX_LEVELS];
 	double *A[ORC_MAX_LEVELS], *x[ORC_MAX_LEVELS], *r[ORC_MAX_LEVELS];
 	float *Af[ORC_MAX_LEVELS], *xf[ORC_MAX_LEVELS], *rf[ORC_MAX_LEVELS];
@@ -279,9 +407,15 @@ struct orc_mg {
  * falls back); orc_mg_enable_lu mirrors re-running enable_exact() and
  * lu_decomposition() after the operators exist. */
 orc_mg *orc_mg_create(int m, int n, int o, const double *A27) {
+	return orc_mg_create_p(m, n, o, 0, A27);
+}
+
+/* The same with periodic axes prd (bit 0 x, bit 1 y, bit 2 z). */
+orc_mg *orc_mg_create_p(int m, int n, int o, int prd, const double *A27) {
 	orc_mg *mg = (orc_mg *)calloc(1, sizeof(orc_mg));
 	int l;
-	mg->nl = orc_hierarchy(m, n, o, mg->dims);
+	mg->prd = prd;
+	mg->nl = orc_hierarchy_p(m, n, o, prd, mg->dims);
 	mg->v_down = mg->v_up = 2;
 	for (l = 0; l < mg->nl; l++) {
 		size_t N = LPTS(mg, l);
@@ -290,8 +424,10 @@ orc_mg *orc_mg_create(int m, int n, int o, const double *A27) {
 		mg->r[l] = (double *)calloc(N, sizeof(double));
 	}
 	memcpy(mg->A[0], A27, sizeof(double) * 27 * LPTS(mg, 0));
-	for (l = 1; l < mg->nl; l++)
-		orc_rat(LM(mg, l - 1), LN(mg, l - 1), LO(mg, l - 1), mg->A[l - 1], LM(mg, l), LN(mg, l), LO(mg, l), mg->A[l]);
+	for (l = 1; l < mg->nl; l++) {
+		if (prd) orc_rat_p(LM(mg, l - 1), LN(mg, l - 1), LO(mg, l - 1), prd, mg->A[l - 1], LM(mg, l), LN(mg, l), LO(mg, l), mg->A[l]);
+		else orc_rat(LM(mg, l - 1), LN(mg, l - 1), LO(mg, l - 1), mg->A[l - 1], LM(mg, l), LN(mg, l), LO(mg, l), mg->A[l]);
+	}
 	return mg;
 }
 
@@ -316,7 +452,7 @@ int orc_mg_enable_lu(orc_mg *mg) {
 	free(mg->LU); free(mg->ipiv);
 	mg->LU = (double *)malloc(sizeof(double) * N * N);
 	mg->ipiv = (int *)malloc(sizeof(int) * N);
-	mg->lu_ok = orc_lu_factor(LM(mg, b), LN(mg, b), LO(mg, b), mg->A[b], mg->LU, mg->ipiv);
+	mg->lu_ok = orc_lu_factor_p(LM(mg, b), LN(mg, b), LO(mg, b), mg->prd, mg->A[b], mg->LU, mg->ipiv);
 	if (!mg->lu_ok) {free(mg->LU); free(mg->ipiv); mg->LU = NULL; mg->ipiv = NULL;}
 	return mg->lu_ok;
 }
@@ -329,7 +465,12 @@ void orc_mg_get_A(const orc_mg *mg, int l, double *A) {memcpy(A, mg->A[l], sizeo
 
 /* multigrid::gauss_seidel, multigrid.cc:180-182 -> region.cc:355-542. */
 void orc_mg_gauss_seidel(orc_mg *mg, int mode, int l) {
-	if (mg->smoother == ORC_SMOOTH_COLOUR8)
+	if (mg->prd) {
+		if (mg->smoother == ORC_SMOOTH_COLOUR8)
+			orc_gs_colour8_p(LM(mg, l), LN(mg, l), LO(mg, l), mg->prd, mg->A[l], mg->x[l], mg->r[l], mode);
+		else
+			orc_gs_lex_p(LM(mg, l), LN(mg, l), LO(mg, l), mg->prd, mg->A[l], mg->x[l], mg->r[l], mode);
+	} else if (mg->smoother == ORC_SMOOTH_COLOUR8)
 		orc_gs_colour8(LM(mg, l), LN(mg, l), LO(mg, l), mg->A[l], mg->x[l], mg->r[l], mode);
 	else
 		orc_gs_lex(LM(mg, l), LN(mg, l), LO(mg, l), mg->A[l], mg->x[l], mg->r[l], mode);
@@ -338,14 +479,17 @@ void orc_mg_gauss_seidel(orc_mg *mg, int mode, int l) {
 /* multigrid::restriction, multigrid.cc:211-224: coarse r from the fine
  * residual, then clear_x_field (region.cc:1679-1682) on the coarse level. */
 void orc_mg_restriction(orc_mg *mg, int l) {
-	orc_restrict(LM(mg, l - 1), LN(mg, l - 1), LO(mg, l - 1), mg->A[l - 1], mg->x[l - 1], mg->r[l - 1],
+	if (mg->prd) orc_restrict_p(LM(mg, l - 1), LN(mg, l - 1), LO(mg, l - 1), mg->prd, mg->A[l - 1], mg->x[l - 1], mg->r[l - 1],
+				    LM(mg, l), LN(mg, l), LO(mg, l), mg->r[l]);
+	else orc_restrict(LM(mg, l - 1), LN(mg, l - 1), LO(mg, l - 1), mg->A[l - 1], mg->x[l - 1], mg->r[l - 1],
 		     LM(mg, l), LN(mg, l), LO(mg, l), mg->r[l]);
 	memset(mg->x[l], 0, sizeof(double) * LPTS(mg, l));
 }
 
 /* multigrid::interpolation, multigrid.cc:196-206. */
 void orc_mg_interpolation(orc_mg *mg, int l) {
-	orc_interp(LM(mg, l), LN(mg, l), LO(mg, l), mg->x[l], LM(mg, l - 1), LN(mg, l - 1), LO(mg, l - 1), mg->x[l - 1]);
+	if (mg->prd) orc_interp_p(LM(mg, l), LN(mg, l), LO(mg, l), mg->prd, mg->x[l], LM(mg, l - 1), LN(mg, l - 1), LO(mg, l - 1), mg->x[l - 1]);
+	else orc_interp(LM(mg, l), LN(mg, l), LO(mg, l), mg->x[l], LM(mg, l - 1), LN(mg, l - 1), LO(mg, l - 1), mg->x[l - 1]);
 }
 
 /* region::solve_exact, region_tr.cc:540-558: LU solve if enabled, else
@@ -372,6 +516,7 @@ void orc_mg_v_cycle(orc_mg *mg, int mode) {
 }
 
 double orc_mg_l2_error_all(orc_mg *mg, int l) {
+	if (mg->prd) return orc_l2_error_all_p(LM(mg, l), LN(mg, l), LO(mg, l), mg->prd, mg->A[l], mg->x[l], mg->r[l]);
 	return orc_l2_error_all(LM(mg, l), LN(mg, l), LO(mg, l), mg->A[l], mg->x[l], mg->r[l]);
 }
 double orc_mg_rhs_l2norm(orc_mg *mg, int l) {return orc_rhs_l2norm(LM(mg, l), LN(mg, l), LO(mg, l), mg->r[l]);}
@@ -391,15 +536,22 @@ int orc_mg_solve(orc_mg *mg, double tol, int maxc, double *hist) {
 
 /* ----------------------------------------------------- mixed precision */
 
+static void f32_gs(orc_mg *mg, int l) {
+	if (mg->prd) {
+		if (mg->smoother == ORC_SMOOTH_COLOUR8) orc_gs_colour8_p_f(LM(mg, l), LN(mg, l), LO(mg, l), mg->prd, mg->Af[l], mg->xf[l], mg->rf[l], 0);
+		else orc_gs_lex_p_f(LM(mg, l), LN(mg, l), LO(mg, l), mg->prd, mg->Af[l], mg->xf[l], mg->rf[l], 0);
+	} else if (mg->smoother == ORC_SMOOTH_COLOUR8) orc_gs_colour8_f(LM(mg, l), LN(mg, l), LO(mg, l), mg->Af[l], mg->xf[l], mg->rf[l], 0);
+	else orc_gs_lex_f(LM(mg, l), LN(mg, l), LO(mg, l), mg->Af[l], mg->xf[l], mg->rf[l], 0);
+}
+
 static void fp32_cycle(orc_mg *mg) {
 	int i, q, L = mg->nl;
 	for (i = 0; i < L - 1; i++) {
-		for (q = 0; q < mg->v_down + i; q++) {
-			if (mg->smoother == ORC_SMOOTH_COLOUR8) orc_gs_colour8_f(LM(mg, i), LN(mg, i), LO(mg, i), mg->Af[i], mg->xf[i], mg->rf[i], 0);
-			else orc_gs_lex_f(LM(mg, i), LN(mg, i), LO(mg, i), mg->Af[i], mg->xf[i], mg->rf[i], 0);
-		}
-		orc_restrict_f(LM(mg, i), LN(mg, i), LO(mg, i), mg->Af[i], mg->xf[i], mg->rf[i],
-			       LM(mg, i + 1), LN(mg, i + 1), LO(mg, i + 1), mg->rf[i + 1]);
+		for (q = 0; q < mg->v_down + i; q++) f32_gs(mg, i);
+		if (mg->prd) orc_restrict_p_f(LM(mg, i), LN(mg, i), LO(mg, i), mg->prd, mg->Af[i], mg->xf[i], mg->rf[i],
+					      LM(mg, i + 1), LN(mg, i + 1), LO(mg, i + 1), mg->rf[i + 1]);
+		else orc_restrict_f(LM(mg, i), LN(mg, i), LO(mg, i), mg->Af[i], mg->xf[i], mg->rf[i],
+				    LM(mg, i + 1), LN(mg, i + 1), LO(mg, i + 1), mg->rf[i + 1]);
 		memset(mg->xf[i + 1], 0, sizeof(float) * LPTS(mg, i + 1));
 	}
 	{
@@ -411,17 +563,12 @@ static void fp32_cycle(orc_mg *mg) {
 			orc_lu_solve((int)N, mg->LU, mg->ipiv, rb, xb);
 			for (p = 0; p < N; p++) mg->xf[b][p] = (float)xb[p];
 			free(rb); free(xb);
-		} else for (q = 0; q < 24; q++) {
-			if (mg->smoother == ORC_SMOOTH_COLOUR8) orc_gs_colour8_f(LM(mg, b), LN(mg, b), LO(mg, b), mg->Af[b], mg->xf[b], mg->rf[b], 0);
-			else orc_gs_lex_f(LM(mg, b), LN(mg, b), LO(mg, b), mg->Af[b], mg->xf[b], mg->rf[b], 0);
-		}
+		} else for (q = 0; q < 24; q++) f32_gs(mg, b);
 	}
 	for (i = L - 1; i > 0; i--) {
-		orc_interp_f(LM(mg, i), LN(mg, i), LO(mg, i), mg->xf[i], LM(mg, i - 1), LN(mg, i - 1), LO(mg, i - 1), mg->xf[i - 1]);
-		for (q = 0; q < mg->v_up + i - 1; q++) {
-			if (mg->smoother == ORC_SMOOTH_COLOUR8) orc_gs_colour8_f(LM(mg, i - 1), LN(mg, i - 1), LO(mg, i - 1), mg->Af[i - 1], mg->xf[i - 1], mg->rf[i - 1], 0);
-			else orc_gs_lex_f(LM(mg, i - 1), LN(mg, i - 1), LO(mg, i - 1), mg->Af[i - 1], mg->xf[i - 1], mg->rf[i - 1], 0);
-		}
+		if (mg->prd) orc_interp_p_f(LM(mg, i), LN(mg, i), LO(mg, i), mg->prd, mg->xf[i], LM(mg, i - 1), LN(mg, i - 1), LO(mg, i - 1), mg->xf[i - 1]);
+		else orc_interp_f(LM(mg, i), LN(mg, i), LO(mg, i), mg->xf[i], LM(mg, i - 1), LN(mg, i - 1), LO(mg, i - 1), mg->xf[i - 1]);
+		for (q = 0; q < mg->v_up + i - 1; q++) f32_gs(mg, i - 1);
 	}
 }
 
@@ -444,7 +591,8 @@ int orc_mg_solve_mixed(orc_mg *mg, double tol, int maxc, double *hist) {
 	double bn = orc_mg_rhs_l2norm(mg, 0);
 	double *res = (double *)malloc(sizeof(double) * N0);
 	for (;;) {
-		orc_residual(LM(mg, 0), LN(mg, 0), LO(mg, 0), mg->A[0], mg->x[0], mg->r[0], res);
+		if (mg->prd) orc_residual_p(LM(mg, 0), LN(mg, 0), LO(mg, 0), mg->prd, mg->A[0], mg->x[0], mg->r[0], res);
+		else orc_residual(LM(mg, 0), LN(mg, 0), LO(mg, 0), mg->A[0], mg->x[0], mg->r[0], res);
 		double e = 0;
 		for (p = 0; p < N0; p++) e += res[p] * res[p];
 		if (hist) hist[c] = e;

@@ -6,7 +6,8 @@
  *
  * Layout: level arrays are point-major with p = i + m*(j + n*k); operators
  * hold 27 entries per point, e = (dk+1)*9 + (dj+1)*3 + (di+1), zero where the
- * neighbour lies outside the grid.  Grids are non-periodic (zero Dirichlet).
+ * neighbour lies outside the grid on a non-periodic axis (zero Dirichlet).
+ * The _p functions take the periodic axes prd (bit 0 x, bit 1 y, bit 2 z).
  * All arithmetic is done in the reference's order without contraction, so
  * results are bit-identical to the reference where it is deterministic
  * (see the per-function notes in pgmg_oracle.c). */
@@ -53,10 +54,28 @@ void orc_interp_f(int mc, int nc, int oc, const float *xc, int mf, int nf, int o
 int orc_lu_factor(int m, int n, int o, const double *A27, double *LU, int *ipiv);
 void orc_lu_solve(int N, const double *LU, const int *ipiv, const double *b, double *x);
 
+/* Periodic axes. */
+int orc_hierarchy_p(int m, int n, int o, int prd, int *dims);
+void orc_gen_operator_p(int family, uint64_t seed, int m, int n, int o, int prd, double *A27);
+void orc_gs_lex_p(int m, int n, int o, int prd, const double *A, double *x, const double *r, int mode);
+void orc_gs_colour8_p(int m, int n, int o, int prd, const double *A, double *x, const double *r, int mode);
+void orc_residual_p(int m, int n, int o, int prd, const double *A, const double *x, const double *r, double *res);
+double orc_l2_error_all_p(int m, int n, int o, int prd, const double *A, const double *x, const double *r);
+void orc_restrict_p(int mf, int nf, int of, int prd, const double *Af, const double *xf, const double *rf,
+		    int mc, int nc, int oc, double *rc);
+void orc_interp_p(int mc, int nc, int oc, int prd, const double *xc, int mf, int nf, int of, double *xf);
+void orc_rat_p(int mf, int nf, int of, int prd, const double *Af, int mc, int nc, int oc, double *Ac);
+int orc_lu_factor_p(int m, int n, int o, int prd, const double *A27, double *LU, int *ipiv);
+void orc_gs_lex_p_f(int m, int n, int o, int prd, const float *A, float *x, const float *r, int mode);
+void orc_gs_colour8_p_f(int m, int n, int o, int prd, const float *A, float *x, const float *r, int mode);
+void orc_restrict_p_f(int mf, int nf, int of, int prd, const float *Af, const float *xf, const float *rf,
+		      int mc, int nc, int oc, float *rc);
+void orc_interp_p_f(int mc, int nc, int oc, int prd, const float *xc, int mf, int nf, int of, float *xf);
 /* Whole hierarchy. */
 typedef struct orc_mg orc_mg;
 enum { ORC_SMOOTH_LEX = 0, ORC_SMOOTH_COLOUR8 = 1 };
 orc_mg *orc_mg_create(int m, int n, int o, const double *A27);
+orc_mg *orc_mg_create_p(int m, int n, int o, int prd, const double *A27);
 void orc_mg_destroy(orc_mg *mg);
 int orc_mg_levels(const orc_mg *mg);
 void orc_mg_set_smoother(orc_mg *mg, int smoother);

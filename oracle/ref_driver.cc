@@ -5,9 +5,12 @@
  *
  * The problem class below is a p_class in the sense of problem_simple.hh:14-73:
  * every grid point is an unknown ("internal"), its stencil box is the
- * [-1,1]^3 box clipped to the grid (range_* below), and fill_entries writes
- * the box entries in k,j,i order from a 27-entry row.  Rows come from
- * include/pgmg_problems.h.
+ * [-1,1]^3 box, clipped to the grid on non-periodic axes and whole on
+ * periodic ones (range_* below), and fill_entries writes the box entries in
+ * k,j,i order from a 27-entry row.  Rows come from include/pgmg_problems.h.
+ *
+ * <family> is a family number, optionally followed by ":<prd>", the periodic
+ * axes as a bit mask (1 = x, 2 = y, 4 = z), e.g. "1:5".
  *
  * Usage:
  *   pgmg_ref fixture <family> <seed> <m> <n> <o> <out.bin>
@@ -31,18 +34,18 @@ class array_problem {
 	public:
 		const bool x_prd, y_prd, z_prd;
 		const int m, n, o;
-		int family;
+		int family, prd;
 		uint64_t seed;
 		static double dummy[1];
 		array_problem(geometry &gm, int fam, uint64_t sd)
-			: x_prd(false), y_prd(false), z_prd(false), m(gm.m), n(gm.n), o(gm.o),
-			  family(fam), seed(sd) {}
-		inline int range_xd(int i, int j, int k) {return i > 0 ? -1 : 0;}
-		inline int range_xu(int i, int j, int k) {return i < m - 1 ? 2 : 1;}
-		inline int range_yd(int i, int j, int k) {return j > 0 ? -1 : 0;}
-		inline int range_yu(int i, int j, int k) {return j < n - 1 ? 2 : 1;}
-		inline int range_zd(int i, int j, int k) {return k > 0 ? -1 : 0;}
-		inline int range_zu(int i, int j, int k) {return k < o - 1 ? 2 : 1;}
+			: x_prd(gm.x_prd), y_prd(gm.y_prd), z_prd(gm.z_prd), m(gm.m), n(gm.n), o(gm.o),
+			  family(fam), prd((gm.x_prd ? 1 : 0) | (gm.y_prd ? 2 : 0) | (gm.z_prd ? 4 : 0)), seed(sd) {}
+		inline int range_xd(int i, int j, int k) {return x_prd || i > 0 ? -1 : 0;}
+		inline int range_xu(int i, int j, int k) {return x_prd || i < m - 1 ? 2 : 1;}
+		inline int range_yd(int i, int j, int k) {return y_prd || j > 0 ? -1 : 0;}
+		inline int range_yu(int i, int j, int k) {return y_prd || j < n - 1 ? 2 : 1;}
+		inline int range_zd(int i, int j, int k) {return z_prd || k > 0 ? -1 : 0;}
+		inline int range_zu(int i, int j, int k) {return z_prd || k < o - 1 ? 2 : 1;}
 		inline bool internal(int i, int j, int k) {return true;}
 		inline int mem_size(int i, int j, int k) {
 			return (range_xu(i, j, k) - range_xd(i, j, k)) * (range_yu(i, j, k) - range_yd(i, j, k))
@@ -50,7 +53,7 @@ class array_problem {
 		}
 		void fill_entries(int i, int j, int k, double *&en) {
 			double row[27];
-			pgmg_row27(family, seed, m, n, o, i, j, k, row);
+			pgmg_row27_prd(family, seed, m, n, o, prd, i, j, k, row);
 			for (int dk = range_zd(i, j, k); dk < range_zu(i, j, k); dk++)
 				for (int dj = range_yd(i, j, k); dj < range_yu(i, j, k); dj++)
 					for (int di = range_xd(i, j, k); di < range_xu(i, j, k); di++)
@@ -153,12 +156,14 @@ int main(int argc, char **argv) {
 		return 2;
 	}
 	std::string cmd = argv[1];
-	int family = atoi(argv[2]);
+	int family = atoi(argv[2]), prd = 0;
+	const char *colon = strchr(argv[2], ':');
+	if (colon) prd = atoi(colon + 1);
 	uint64_t seed = strtoull(argv[3], NULL, 10);
 	int m = atoi(argv[4]), n = atoi(argv[5]), o = atoi(argv[6]);
 
 	comm_buffer com;
-	geometry gm(m, n, o, false, false, false);
+	geometry gm(m, n, o, (prd & 1) != 0, (prd & 2) != 0, (prd & 4) != 0);
 	array_problem pr(gm, family, seed);
 	multigrid mg(pr, gm, com);
 	int aexact_after_ctor = mg.reg[mg.nlevels - 1]->Aexact != NULL;

@@ -6,7 +6,8 @@ are the pins for parity: coarse (Galerkin) operators at every level, one
 forward and one symmetric Gauss-Seidel sweep, a restriction, an
 interpolation, V-cycle residual histories and iterates with the default
 24-sweep bottom and with the bottom LU enabled, on odd, even and non-cubic
-grids of all three synthetic problem families.  Inputs are regenerated from
+grids of all three synthetic problem families, non-periodic and with
+periodic axes.  Inputs are regenerated from
 (family, seed, m, n, o) by include/pgmg_problems.h; the files keep SHA-256
 digests of the operator and source the reference actually received.
 
@@ -35,6 +36,15 @@ FIXTURES = [
     ("f7log_33x20x11", 2, 14, 33, 20, 11),
     ("q1_18x13x22", 3, 15, 18, 13, 22),
     ("q1_12x12x12", 3, 16, 12, 12, 12),
+]
+# Periodic grids: (name, family, seed, m, n, o, prd), prd the periodic axes
+# as a bit mask (1 = x, 2 = y, 4 = z).  The same records as FIXTURES.
+PFIXTURES = [
+    ("prd_f7u_13x10x11_p5", 1, 5, 13, 10, 11, 5),
+    ("prd_q1_12x18x9_p3", 3, 7, 12, 18, 9, 3),
+    ("prd_f7log_20x17x30_p6", 2, 9, 20, 17, 30, 6),
+    ("prd_q1_10x14x12_p7", 3, 4, 10, 14, 12, 7),
+    ("prd_f7u_16x16x16_p7", 1, 5, 16, 16, 16, 7),
 ]
 # (name, family, seed, m, n, o, bottom, tol)
 SOLVES = [
@@ -100,13 +110,14 @@ def main():
         return not only or name in only
     tmp = os.path.join(ROOT, "_work", "golden_tmp.bin")
     os.makedirs(os.path.dirname(tmp), exist_ok=True)
-    for name, fam, seed, m, n, o in FIXTURES:
+    for name, fam, seed, m, n, o, prd in [f + (0,) for f in FIXTURES] + PFIXTURES:
         if not want(name):
             continue
-        run(["fixture", fam, seed, m, n, o, tmp])
+        run(["fixture", "%d:%d" % (fam, prd) if prd else fam, seed, m, n, o, tmp])
         rec = read_records(tmp)
         dims = rec["dims"].reshape(-1, 3)
-        save = {"meta": np.array([fam, seed, m, n, o], dtype=np.int64), "dims": dims,
+        meta = [fam, seed, m, n, o] + ([prd] if prd else [])
+        save = {"meta": np.array(meta, dtype=np.int64), "dims": dims,
                 "aexact_after_ctor": rec["aexact_after_ctor"]}
         for k, v in rec.items():
             if k in ("dims", "aexact_after_ctor"):

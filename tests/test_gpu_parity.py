@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 FIXTURES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
-                  if not os.path.basename(p).startswith(("solve_", "slices_", "summary_")))
+                  if not os.path.basename(p).startswith(("solve_", "slices_", "summary_", "prd_")))
 NORM_RTOL = 1e-12
 
 

@@ -17,7 +17,7 @@ from oracle import oracle as O
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 FIXTURES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
-                  if not os.path.basename(p).startswith(("solve_", "slices_", "summary_")))
+                  if not os.path.basename(p).startswith(("solve_", "slices_", "summary_", "prd_")))
 SOLVES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "solve_*.npz")))
 
 
