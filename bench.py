@@ -105,6 +105,83 @@ def cpu_baseline():
             "sample": "oracle 1 V-cycle at 129^3, x8 for 257^3, x10 cycles"}
 
 
+def simple_periodic_operator(m, n, o):
+    """The reference's problem_simple on a grid periodic in every axis
+    (problem_simple.hh/.cc): every point interior, the 7-point Laplacian
+    (1 on the six faces, -6 at the centre); source +1 at (3,3,3) and -1 at
+    (12,12,12), x = 0."""
+    A = np.zeros((o, n, m, 27))
+    for e in (4, 10, 12, 14, 16, 22):
+        A[..., e] = 1.0
+    A[..., 13] = -6.0
+    b = np.zeros((o, n, m))
+    b[3, 3, 3] = 1.0
+    b[12, 12, 12] = -1.0
+    return A, b
+
+
+def periodic_256(P, tests=5, iters=6):
+    """The reference's own benchmark, mg_parallel.cc: a 256^3 grid periodic in
+    every axis, problem_simple, `tests` runs of `iters` V-cycles from x = 0,
+    digits gained = log10(err_start) - log10(err_end).  Here in both smoothers
+    (exact: the reference's lexicographic order), FP64, and mixed with the
+    colour smoother; beside it, the reference binary (oracle/_ref/mg_parallel,
+    built unchanged from its sources) on this host's cores when present."""
+    n = 256
+    A, b = simple_periodic_operator(n, n, n)
+    out = {"workload": "mg_parallel.cc: 256^3 periodic problem_simple, %d tests x %d V-cycles" % (tests, iters)}
+    for name, kw in [("exact_fp64", dict(smoother=P.SMOOTH_EXACT, precision=P.PREC_FP64)),
+                     ("colour8_fp64", dict(smoother=P.SMOOTH_COLOUR8, precision=P.PREC_FP64)),
+                     ("colour8_mixed", dict(smoother=P.SMOOTH_COLOUR8, precision=P.PREC_MIXED))]:
+        h = P.Hierarchy(n, n, n, prd=7, use_graph=True, **kw)
+        h.set_operator27(A)
+        h.set_rhs(b)
+        h.compute_rats()
+        ts = []
+        for t in range(tests + 1):
+            h.clear_x(0)
+            e0 = h.l2_error_all()
+            h.synchronize()
+            t0 = time.perf_counter()
+            if kw["precision"] == P.PREC_MIXED:
+                h.solve(1e-300, iters, check=False)
+            else:
+                for _ in range(iters):
+                    h.v_cycle()
+            h.synchronize()
+            if t:  # the first run is a warm-up
+                ts.append(time.perf_counter() - t0)
+        e1 = h.l2_error_all()
+        digs = float(np.log10(e0) - np.log10(e1))
+        out[name] = {"s_per_test": round(float(np.mean(ts)), 4), "err_start": e0, "err_end": e1,
+                     "digits_gained": round(digs, 3), "digits_per_s": round(digs / float(np.mean(ts)), 2),
+                     "levels": h.nlevels}
+        h.close()
+        log("periodic 256 %s: %.3f s per %d cycles, %.2f digits" % (name, np.mean(ts), iters, digs))
+    ref = os.path.join(ROOT, "oracle", "_ref", "mg_parallel")
+    mpiexec = "/opt/conda/bin/mpiexec"
+    if os.path.exists(ref) and os.path.exists(mpiexec):
+        ranks = max(1, min(16, os.cpu_count() or 1))
+        try:
+            p = subprocess.run([mpiexec, "-n", str(ranks), ref], capture_output=True, text=True, timeout=400,
+                               env=dict(os.environ, LD_LIBRARY_PATH="/opt/conda/lib"), stdin=subprocess.DEVNULL)
+            vals = {}
+            for line in p.stdout.splitlines():
+                if line.startswith("Duration:"):
+                    vals["s_per_test"] = float(line.split()[1])
+                elif line.startswith("Digits gained:"):
+                    vals["digits_gained"] = float(line.split()[2])
+                elif "iters," in line and "->" in line:
+                    parts = line.replace(",", " ").split()
+                    vals["err_start"], vals["err_end"] = float(parts[2]), float(parts[4])
+            vals["ranks"] = ranks
+            out["reference_cpu"] = vals
+            log("periodic 256 reference on %d ranks:" % ranks, vals)
+        except Exception as e:  # pragma: no cover - reported, not fatal
+            log("mg_parallel failed:", e)
+    return out
+
+
 def q1_513(P):
     """513^3 Q1 finite elements with per-cell anisotropic tensors: mixed
     precision to 1e-10 and its error against the FP64 solve."""
@@ -230,7 +307,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 513^3, batch and serving legs")
-    ap.add_argument("--leg", choices=["pcie", "q1_513", "batch_mixed", "batch_fp64", "serving"],
+    ap.add_argument("--leg", choices=["pcie", "q1_513", "batch_mixed", "batch_fp64", "serving", "periodic_256"],
                     help="run one extra leg alone and print its JSON (for profiling)")
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware counters: a 1 GiB elementwise copy (torch.mul, "
@@ -246,7 +323,8 @@ def main():
 
     if args.leg:
         fn = {"pcie": lambda: pcie_inclusive(P), "q1_513": lambda: q1_513(P), "batch_mixed": lambda: batch_256(P, P.PREC_MIXED),
-              "batch_fp64": lambda: batch_256(P, P.PREC_FP64), "serving": lambda: serving(P)}[args.leg]
+              "batch_fp64": lambda: batch_256(P, P.PREC_FP64), "serving": lambda: serving(P),
+              "periodic_256": lambda: periodic_256(P)}[args.leg]
         print(json.dumps({"leg": args.leg, "result": fn()}), flush=True)
         return
 
@@ -367,6 +445,7 @@ def main():
         out["batch_256_fp64"] = batch_256(P, P.PREC_FP64)
         log("batch fp64:", out["batch_256_fp64"])
         out["serving"] = serving(P)
+        out["periodic_256"] = periodic_256(P)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -87,14 +87,13 @@ class multigrid {
 		multigrid(p_class &pr, geometry &gm, comm_buffer &com_, const pgmg_opts *opts_ = NULL)
 			: v_down(2), v_bottom(20), v_up(2), rank(0), nlevels(0), mlevels(0), geom(&gm), hier(NULL), com(com_) {
 			(void)pr;
-			if (gm.x_prd || gm.y_prd || gm.z_prd) p_fatal_error("Periodic problems are not supported", PGMG_SETUP_ERROR);
 			if (opts_) opts = *opts_;
 			else {
 				pgmg_default_opts(&opts);
 				opts.smoother = PGMG_SMOOTH_EXACT;
 			}
 			v_down = opts.v_down; v_up = opts.v_up; v_bottom = opts.v_bottom;
-			pgmg_dims d = {gm.m, gm.n, gm.o, 0, 0, 0};
+			pgmg_dims d = {gm.m, gm.n, gm.o, gm.x_prd ? 1 : 0, gm.y_prd ? 1 : 0, gm.z_prd ? 1 : 0};
 			pgmg_check(pgmg_hier_create(&d, &opts, &hier), "multigrid");
 			pgmg_check(pgmg_num_levels(hier, &nlevels), "multigrid");
 			mlevels = nlevels;
@@ -103,7 +102,7 @@ class multigrid {
 			for (int l = 0; l < nlevels; l++) {
 				int dl[3];
 				pgmg_check(pgmg_level_dims(hier, l, dl), "multigrid");
-				reg[l] = new region(l ? reg[l - 1] : NULL, hier, l, dl[0], dl[1], dl[2]);
+				reg[l] = new region(l ? reg[l - 1] : NULL, hier, l, dl[0], dl[1], dl[2], gm.x_prd, gm.y_prd, gm.z_prd);
 				geo[l] = NULL;
 			}
 		}

@@ -24,12 +24,12 @@ class synthetic_problem {
 		synthetic_problem(geometry &gm, int family_, uint64_t seed_)
 			: x_prd(gm.x_prd), y_prd(gm.y_prd), z_prd(gm.z_prd), m(gm.m), n(gm.n), o(gm.o),
 			  family(family_), seed(seed_) {}
-		inline int range_xd(int i, int, int) {return i > 0 ? -1 : 0;}
-		inline int range_xu(int i, int, int) {return i < m - 1 ? 2 : 1;}
-		inline int range_yd(int, int j, int) {return j > 0 ? -1 : 0;}
-		inline int range_yu(int, int j, int) {return j < n - 1 ? 2 : 1;}
-		inline int range_zd(int, int, int k) {return k > 0 ? -1 : 0;}
-		inline int range_zu(int, int, int k) {return k < o - 1 ? 2 : 1;}
+		inline int range_xd(int i, int, int) {return x_prd || i > 0 ? -1 : 0;}
+		inline int range_xu(int i, int, int) {return x_prd || i < m - 1 ? 2 : 1;}
+		inline int range_yd(int, int j, int) {return y_prd || j > 0 ? -1 : 0;}
+		inline int range_yu(int, int j, int) {return y_prd || j < n - 1 ? 2 : 1;}
+		inline int range_zd(int, int, int k) {return z_prd || k > 0 ? -1 : 0;}
+		inline int range_zu(int, int, int k) {return z_prd || k < o - 1 ? 2 : 1;}
 		inline bool internal(int, int, int) {return true;}
 		inline int mem_size(int i, int j, int k) {
 			return (range_xu(i, j, k) - range_xd(i, j, k)) * (range_yu(i, j, k) - range_yd(i, j, k))
@@ -37,7 +37,7 @@ class synthetic_problem {
 		}
 		void fill_entries(int i, int j, int k, double *&en) {
 			double row[27];
-			pgmg_row27(family, seed, m, n, o, i, j, k, row);
+			pgmg_row27_prd(family, seed, m, n, o, (x_prd ? 1 : 0) | (y_prd ? 2 : 0) | (z_prd ? 4 : 0), i, j, k, row);
 			for (int dk = range_zd(i, j, k); dk < range_zu(i, j, k); dk++)
 				for (int dj = range_yd(i, j, k); dj < range_yu(i, j, k); dj++)
 					for (int di = range_xd(i, j, k); di < range_xu(i, j, k); di++)

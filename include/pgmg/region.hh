@@ -31,11 +31,14 @@ template <int D> int pgmg_region_counter<D>::gs_num = 0;
 namespace pgmg_detail {
 
 /** Ingests a p_class operator (region.hh:474-529, 95-101) into 27-entry rows.
- * Each point's box [range_xd, range_xu) x ... must lie inside [-1,1]^3; the
- * entries of internal points come from fill_entries in k,j,i order, those of
- * the others from external_ptr, in the box's k,j,i order. */
+ * Each point's box [range_xd, range_xu) x ... must lie inside [-1,1]^3, and
+ * inside the grid along the non-periodic axes (prd: bit 0 x, bit 1 y, bit 2
+ * z); along a periodic axis an offset that leaves the grid couples to the
+ * wrapped point.  The entries of internal points come from fill_entries in
+ * k,j,i order, those of the others from external_ptr, in the box's k,j,i
+ * order. */
 template <class p_class>
-void ingest_operator(p_class &pr, int m, int n, int o, std::vector<double> &A27) {
+void ingest_operator(p_class &pr, int m, int n, int o, int prd, std::vector<double> &A27) {
 	A27.assign((size_t)m * n * o * 27, 0.0);
 	std::vector<double> tmp(27);
 	for (int k = 0; k < o; k++) for (int j = 0; j < n; j++) for (int i = 0; i < m; i++) {
@@ -44,8 +47,9 @@ void ingest_operator(p_class &pr, int m, int n, int o, std::vector<double> &A27)
 		int zd = pr.range_zd(i, j, k), zu = pr.range_zu(i, j, k);
 		if (xd < -1 || xu > 2 || yd < -1 || yu > 2 || zd < -1 || zu > 2 || xd >= xu || yd >= yu || zd >= zu)
 			p_fatal_error("Stencil box outside [-1,1]^3 is not supported", PGMG_SETUP_ERROR);
-		if (i + xd < 0 || i + xu > m || j + yd < 0 || j + yu > n || k + zd < 0 || k + zu > o)
-			p_fatal_error("Stencil box reaches outside the grid (periodic problems are not supported)", PGMG_SETUP_ERROR);
+		if ((!(prd & 1) && (i + xd < 0 || i + xu > m)) || (!(prd & 2) && (j + yd < 0 || j + yu > n)) ||
+		    (!(prd & 4) && (k + zd < 0 || k + zu > o)))
+			p_fatal_error("Stencil box reaches outside the grid along a non-periodic axis", PGMG_SETUP_ERROR);
 		const double *src;
 		if (pr.internal(i, j, k)) {
 			int sz = pr.mem_size(i, j, k);
@@ -88,8 +92,7 @@ class region : public pgmg_region_counter<0> {
 			: p(NULL), rank(0), x_prd(gm.x_prd), y_prd(gm.y_prd), z_prd(gm.z_prd), gs_enable(true),
 			  m(gm.m), n(gm.n), o(gm.o), hier(NULL), level(0), owns(true) {
 			(void)com_;
-			if (x_prd || y_prd || z_prd) p_fatal_error("Periodic problems are not supported", PGMG_SETUP_ERROR);
-			pgmg_dims d = {m, n, o, 0, 0, 0};
+			pgmg_dims d = {m, n, o, x_prd ? 1 : 0, y_prd ? 1 : 0, z_prd ? 1 : 0};
 			pgmg_opts opt;
 			pgmg_default_opts(&opt);
 			opt.smoother = PGMG_SMOOTH_EXACT;
@@ -98,8 +101,9 @@ class region : public pgmg_region_counter<0> {
 			init_fields();
 		}
 		/** A view of level l of an existing hierarchy (used by multigrid). */
-		region(region *parent, pgmg_hier *h, int l, int m_, int n_, int o_)
-			: p(parent), rank(0), x_prd(false), y_prd(false), z_prd(false), gs_enable(true),
+		region(region *parent, pgmg_hier *h, int l, int m_, int n_, int o_, bool xp = false, bool yp = false,
+		       bool zp = false)
+			: p(parent), rank(0), x_prd(xp), y_prd(yp), z_prd(zp), gs_enable(true),
 			  m(m_), n(n_), o(o_), hier(h), level(l), owns(false) {init_fields();}
 		~region() {
 			delete[] x;
@@ -113,7 +117,7 @@ class region : public pgmg_region_counter<0> {
 		template <class p_class>
 		void setup_matrices(p_class &pr) {
 			std::vector<double> A27;
-			pgmg_detail::ingest_operator(pr, m, n, o, A27);
+			pgmg_detail::ingest_operator(pr, m, n, o, (x_prd ? 1 : 0) | (y_prd ? 2 : 0) | (z_prd ? 4 : 0), A27);
 			pgmg_check(pgmg_set_operator27(hier, &A27[0]), "setup_matrices");
 			if (owns) pgmg_check(pgmg_compute_rats(hier), "setup_matrices");
 		}

@@ -16,8 +16,11 @@
  * Layout of host arrays: point-major, p = i + m*(j + n*k).  Operators carry
  * 27 entries per point, e = (dk+1)*9 + (dj+1)*3 + (di+1): the reference's box
  * order (region.hh:95-101) with the box fixed to [-1,1]^3 and zero entries
- * where a neighbour lies outside the grid.  Grids are non-periodic with zero
- * Dirichlet data outside. */
+ * where a neighbour lies outside the grid on a non-periodic axis (zero
+ * Dirichlet data outside).  On a periodic axis every entry is kept, and an
+ * offset that leaves the grid couples to the wrapped point; such a grid
+ * coarsens as (m+1)>>1 on that axis (region.cc:13-14).  Batches and serving
+ * take non-periodic grids only. */
 #ifndef PGMG_CAPI_H
 #define PGMG_CAPI_H
 
@@ -40,7 +43,7 @@ enum {
 	PGMG_ERR_SETUP = 6,
 	PGMG_ERR_HIP = 7,          /* a HIP runtime call failed */
 	PGMG_ERR_ARG = 8,          /* invalid argument */
-	PGMG_ERR_UNSUPPORTED = 9,  /* e.g. periodic axes */
+	PGMG_ERR_UNSUPPORTED = 9,  /* e.g. periodic axes in a batch */
 	PGMG_ERR_ADMISSION = 10,   /* serving: request cannot be admitted */
 	PGMG_ERR_NOT_CONVERGED = 11,
 	PGMG_ERR_NAN = 12
@@ -58,7 +61,7 @@ enum { PGMG_BOTTOM_GS24 = 0, PGMG_BOTTOM_LU = 1 };
 
 typedef struct {
 	int m, n, o;              /* global grid size, geometry.hh:12-17 */
-	int x_prd, y_prd, z_prd;  /* periodicity; must be 0 */
+	int x_prd, y_prd, z_prd;  /* periodic axes (0 or 1; 0 for batches) */
 } pgmg_dims;
 
 typedef struct {

@@ -122,16 +122,21 @@ def default_opts(**kw):
     return o
 
 
-def hierarchy_shape(m, n, o):
-    d = Dims(m, n, o, 0, 0, 0)
+def _dims(m, n, o, prd):
+    """prd: the periodic axes as a bit mask (1 = x, 2 = y, 4 = z)."""
+    return Dims(m, n, o, prd & 1, (prd >> 1) & 1, (prd >> 2) & 1)
+
+
+def hierarchy_shape(m, n, o, prd=0):
+    d = _dims(m, n, o, prd)
     nl = ctypes.c_int()
     out = (ctypes.c_int * 48)()
     _check(lib().pgmg_hierarchy_shape(ctypes.byref(d), ctypes.byref(nl), out))
     return [(out[3 * l], out[3 * l + 1], out[3 * l + 2]) for l in range(nl.value)]
 
 
-def footprint(m, n, o, **kw):
-    d = Dims(m, n, o, 0, 0, 0)
+def footprint(m, n, o, prd=0, **kw):
+    d = _dims(m, n, o, prd)
     b = ctypes.c_uint64()
     _check(lib().pgmg_hier_footprint(ctypes.byref(d), ctypes.byref(default_opts(**kw)), ctypes.byref(b)))
     return b.value
@@ -143,14 +148,15 @@ class Hierarchy:
     Method names follow the reference's multigrid class (multigrid.hh)."""
 
     def __init__(self, m, n, o, precision=PREC_FP64, smoother=SMOOTH_COLOUR8, bottom=BOTTOM_GS24,
-                 v_down=2, v_up=2, use_graph=True, max_levels=0):
+                 v_down=2, v_up=2, use_graph=True, max_levels=0, prd=0):
         self.opts = default_opts(precision=precision, smoother=smoother, bottom=bottom,
                                  v_down=v_down, v_up=v_up, use_graph=use_graph, max_levels=max_levels)
-        d = Dims(m, n, o, 0, 0, 0)
+        self.prd = prd
+        d = _dims(m, n, o, prd)
         h = _VP()
         _check(lib().pgmg_hier_create(ctypes.byref(d), ctypes.byref(self.opts), ctypes.byref(h)))
         self.h = h
-        self.dims = hierarchy_shape(m, n, o)
+        self.dims = hierarchy_shape(m, n, o, prd)
         if max_levels > 0:
             self.dims = self.dims[:max_levels]
 

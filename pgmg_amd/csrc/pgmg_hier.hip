@@ -94,21 +94,25 @@ long long level_bytes(const LevelView &v, bool mixed) {
 	return b;
 }
 
-int shape(int m, int n, int o, int *dims) {
+// Level sizes: coarse m' = (m+2)>>1 on a non-periodic axis, (m+1)>>1 on a
+// periodic one (region.cc:13-14), until m*n*o <= 1024 (multigrid.cc:26).
+int shape(int m, int n, int o, int prd, int *dims) {
 	int l = 0;
 	dims[0] = m; dims[1] = n; dims[2] = o;
 	while ((long long)dims[3 * l] * dims[3 * l + 1] * dims[3 * l + 2] > 1024) {
 		if (l + 1 == kMaxLevels) return -1;
-		for (int a = 0; a < 3; a++) dims[3 * (l + 1) + a] = (dims[3 * l + a] + 2) >> 1;
+		for (int a = 0; a < 3; a++) dims[3 * (l + 1) + a] = (dims[3 * l + a] + (((prd >> a) & 1) ? 1 : 2)) >> 1;
 		l++;
 	}
 	return l + 1;
 }
 
+int dims_prd(const pgmg_dims *d) {
+	return (d->x_prd ? 1 : 0) | (d->y_prd ? 2 : 0) | (d->z_prd ? 4 : 0);
+}
+
 int check_dims(const pgmg_dims *d) {
 	if (!d) return fail(PGMG_ERR_ARG, "null dims");
-	if (d->x_prd || d->y_prd || d->z_prd)
-		return fail(PGMG_ERR_UNSUPPORTED, "periodic axes are not supported");
 	if (d->m < 1 || d->n < 1 || d->o < 1) return fail(PGMG_ERR_ARG, "grid sizes must be positive");
 	if ((long long)d->m * d->n * d->o > (1ll << 33)) return fail(PGMG_ERR_ARG, "grid too large");
 	return PGMG_OK;
@@ -191,6 +195,7 @@ bool gs_use_small(const pgmg_hier *h, const Level &L, int mode) {
 // Whether op_gs(..., from_zero) starts a forward FP32 colour sweep without
 // reading x (then x need not be cleared before it).
 bool gs_from_zero_ok(const pgmg_hier *h, int level, int mode) {
+	if (h->lev[level].v.prd) return false;
 	return h->opts.smoother != PGMG_SMOOTH_EXACT && mode == 0 && !gs_use_small<float>(h, h->lev[level], mode);
 }
 
@@ -202,6 +207,25 @@ void op_gs(pgmg_hier *h, int level, int mode, int nsweeps, bool from_zero = fals
 	const bool exact = h->opts.smoother == PGMG_SMOOTH_EXACT;
 	h->gs_count += nsweeps;
 	if (nsweeps <= 0) return;
+	if (L.v.prd) {
+		// Periodic axes: the ghost copies are refreshed before each pass
+		// (region.cc:361, 457), then the pass runs on the per-point kernels,
+		// which read the halo like any neighbour.
+		const int tmax = (L.v.m - 1) + 2 * (L.v.n - 1) + 4 * (L.v.o - 1);
+		for (int s = 0; s < nsweeps; s++)
+			for (int pass = 0; pass < (mode ? 2 : 1); pass++) {
+				ProfScope ps(h, KC_GS, level, L.points());
+				launch_halo_wrap<T>(h->stream, L.v, lx<T>(L));
+				if (exact) {
+					for (int tt = 0; tt <= tmax; tt++)
+						launch_gs_plane<T>(h->stream, L.v, lA<T>(L), lx<T>(L), lr<T>(L), pass ? tmax - tt : tt);
+				} else {
+					for (int c = 0; c < 8; c++)
+						launch_gs_colour<T>(h->stream, L.v, lA<T>(L), lx<T>(L), lr<T>(L), pass ? 7 - c : c);
+				}
+			}
+		return;
+	}
 	const bool use_small = gs_use_small<T>(h, L, mode);
 	if (use_small) {
 		ProfScope ps(h, KC_GS, level, L.points() * nsweeps * (mode ? 2 : 1));
@@ -370,7 +394,7 @@ int pgmg_hierarchy_shape(const pgmg_dims *d, int *nlevels, int *dims_out) {
 	int rc = check_dims(d);
 	if (rc) return rc;
 	int tmp[3 * kMaxLevels];
-	int nl = shape(d->m, d->n, d->o, tmp);
+	int nl = shape(d->m, d->n, d->o, dims_prd(d), tmp);
 	if (nl < 0) return fail(PGMG_ERR_MULTIGRID, "Maximum number of multigrid levels reached");
 	if (nlevels) *nlevels = nl;
 	if (dims_out) memcpy(dims_out, tmp, sizeof(int) * 3 * nl);
@@ -381,13 +405,14 @@ int pgmg_hier_footprint(const pgmg_dims *d, const pgmg_opts *opts, uint64_t *byt
 	int rc = check_dims(d);
 	if (rc) return rc;
 	int dims[3 * kMaxLevels];
-	int nl = shape(d->m, d->n, d->o, dims);
+	int nl = shape(d->m, d->n, d->o, dims_prd(d), dims);
 	if (nl < 0) return fail(PGMG_ERR_MULTIGRID, "Maximum number of multigrid levels reached");
 	if (opts && opts->max_levels > 0 && opts->max_levels < nl) nl = opts->max_levels;
 	const bool mixed = opts && opts->precision == PGMG_PREC_MIXED;
 	uint64_t b = 0;
 	for (int l = 0; l < nl; l++) b += (uint64_t)level_bytes(make_level_view(dims[3 * l], dims[3 * l + 1], dims[3 * l + 2]), mixed);
 	LevelView top = make_level_view(d->m, d->n, d->o);
+	top.prd = dims_prd(d);
 	b += (uint64_t)top.npad * (8 + (mixed ? 4 : 0));
 	b += (uint64_t)residual_partial_count(top) * 8;
 	if (opts && opts->bottom == PGMG_BOTTOM_LU) {
@@ -412,7 +437,7 @@ int pgmg_hier_create(const pgmg_dims *d, const pgmg_opts *opts, pgmg_hier **out)
 	if (o.v_down < 0 || o.v_up < 0) return fail(PGMG_ERR_ARG, "negative sweep counts");
 	if (o.max_levels < 0) return fail(PGMG_ERR_ARG, "negative max_levels");
 	int dims[3 * kMaxLevels];
-	int nl = shape(d->m, d->n, d->o, dims);
+	int nl = shape(d->m, d->n, d->o, dims_prd(d), dims);
 	if (nl < 0) return fail(PGMG_ERR_MULTIGRID, "Maximum number of multigrid levels reached");
 	if (o.max_levels > 0 && o.max_levels < nl) nl = o.max_levels;
 	int ndev = 0;
@@ -436,6 +461,7 @@ int pgmg_hier_create(const pgmg_dims *d, const pgmg_opts *opts, pgmg_hier **out)
 		Level &L = h->lev[l];
 		L.v = make_level_view(dims[3 * l], dims[3 * l + 1], dims[3 * l + 2]);
 		L.v.cache_coef = 1;  // one level runs at a time
+		L.v.prd = dims_prd(d);
 		const size_t np = (size_t)L.v.npad;
 		ALLOC(L.A, 27 * np * 8);
 		ALLOC(L.x, np * 8);
@@ -790,6 +816,7 @@ int pgmg_apply_operator(pgmg_hier *h, int level, const double *in, double *out) 
 	rc = vec_io(h, level, din, in, nullptr);
 	if (rc) return rc;
 	Level &L = h->lev[level];
+	launch_halo_wrap<double>(h->stream, L.v, din);
 	launch_apply(h->stream, L.v, L.A, din, dout);
 	return vec_io(h, level, dout, nullptr, out);
 }

@@ -10,7 +10,9 @@
 // i even, moves with one 16-byte (FP64) or 8-byte (FP32) vector access, and
 // a quad (i .. i+3), i a multiple of 4, with one 16-byte FP32 access.
 // Halo entries are zero and stay zero, so a stencil at the grid edge reads
-// zeros instead of branching.  Coefficients: see arow0.
+// zeros instead of branching; on a periodic axis the halo instead holds
+// ghost copies of the wrapped points, refreshed (launch_halo_wrap) before
+// each sweep and residual.  Coefficients: see arow0.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -27,6 +29,7 @@ struct LevelView {
 	int cache_coef;    // 1: levels run one at a time (a lone hierarchy), so a
 	                   // level may keep coefficients in the memory-side cache;
 	                   // 0: stream them (batched slots run side by side)
+	int prd;           // periodic axes: bit 0 x, bit 1 y, bit 2 z
 };
 
 inline LevelView make_level_view(int m, int n, int o) {
@@ -37,6 +40,7 @@ inline LevelView make_level_view(int m, int n, int o) {
 	v.base = 4 + v.px + v.pxy;
 	v.npad = v.pxy * (o + 2);
 	v.cache_coef = 0;
+	v.prd = 0;
 	return v;
 }
 
@@ -173,6 +177,9 @@ template <typename T> void launch_gs_colour(hipStream_t s, const LevelView &L, c
 // x need not be cleared beforehand.
 template <typename T> void launch_gs_pair(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int pair,
 					  bool from_zero = false);
+// Ghost copies of the wrapped points into the halo of the periodic axes of
+// L (no-op without periodic axes).
+template <typename T> void launch_halo_wrap(hipStream_t s, const LevelView &L, T *x);
 // Gauss-Seidel on the hyperplane i+2j+4k = t.
 template <typename T> void launch_gs_plane(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int t);
 // nsweeps Gauss-Seidel sweeps of a small level by one workgroup, with x

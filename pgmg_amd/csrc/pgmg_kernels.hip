@@ -47,6 +47,51 @@ __device__ __forceinline__ void gs_point(const LevelView &L, const T *__restrict
 	x[p] = (r[p] - ax) * ((T)1 / Ap[13 * px]);
 }
 
+// ------------------------------------------------------- periodic halo
+
+// Ghost copies for the periodic axes of L.  At one rank the reference's
+// periodic neighbour is the rank itself, and communicate()
+// (region.cc:1338-1365) copies the grid's wrapped values into its ghost
+// layers before each sweep and each residual (region.cc:361, 457, 549,
+// 1478); a sweep then reads, but never updates, those copies.  One thread
+// per halo point of the six faces (edges and corners come up more than once
+// and get the same value); a halo point outside a non-periodic axis keeps
+// its zero.  Every source is a grid point, never a halo point, so the
+// copies do not depend on one another.
+template <typename T>
+__global__ __launch_bounds__(256) void k_halo_wrap(LevelView L, T *__restrict__ x) {
+	const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+	const long long ex = 2LL * (L.n + 2) * (L.o + 2), ey = 2LL * (L.m + 2) * (L.o + 2);
+	const long long ez = 2LL * (L.m + 2) * (L.n + 2);
+	int i, j, k;
+	if (q < ex) {
+		const long long t = q >> 1;
+		i = (q & 1) ? L.m : -1; j = (int)(t % (L.n + 2)) - 1; k = (int)(t / (L.n + 2)) - 1;
+	} else if (q < ex + ey) {
+		const long long t = (q - ex) >> 1;
+		j = ((q - ex) & 1) ? L.n : -1; i = (int)(t % (L.m + 2)) - 1; k = (int)(t / (L.m + 2)) - 1;
+	} else if (q < ex + ey + ez) {
+		const long long t = (q - ex - ey) >> 1;
+		k = ((q - ex - ey) & 1) ? L.o : -1; i = (int)(t % (L.m + 2)) - 1; j = (int)(t / (L.m + 2)) - 1;
+	} else {
+		return;
+	}
+	int si = i, sj = j, sk = k;
+	if (si < 0 || si >= L.m) { if (!(L.prd & 1)) return; si = si < 0 ? si + L.m : si - L.m; }
+	if (sj < 0 || sj >= L.n) { if (!(L.prd & 2)) return; sj = sj < 0 ? sj + L.n : sj - L.n; }
+	if (sk < 0 || sk >= L.o) { if (!(L.prd & 4)) return; sk = sk < 0 ? sk + L.o : sk - L.o; }
+	x[idx3(L, i, j, k)] = x[idx3(L, si, sj, sk)];
+}
+
+template <typename T>
+void launch_halo_wrap(hipStream_t s, const LevelView &L, T *x) {
+	if (!L.prd) return;
+	const long long tot = 2LL * ((long long)(L.n + 2) * (L.o + 2) + (long long)(L.m + 2) * (L.o + 2) +
+				     (long long)(L.m + 2) * (L.n + 2));
+	hipLaunchKernelGGL(k_halo_wrap<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, L, x);
+	HIPCHECK_LAUNCH();
+}
+
 // ------------------------------------------------------------ smoothers
 
 // One colour class: points (2a+ci, 2b+cj, 2d+ck).  Points of one class never
@@ -1288,13 +1333,53 @@ __global__ __launch_bounds__(256) void k_residual(LevelView L, const T *__restri
 	}
 }
 
+// Residual of a level with periodic axes, one point per thread, after the
+// halo holds the ghost copies: the 27-entry sum in box order from zero,
+// centre included (region.cc:1833-1842), as the row kernels form it.
+template <typename T, typename TR>
+__global__ __launch_bounds__(256) void k_residual_pt(LevelView L, const T *__restrict__ A, const T *__restrict__ x,
+						     const T *__restrict__ r, TR *__restrict__ res,
+						     double *__restrict__ partial, TR *__restrict__ zx) {
+	const int i = (int)(blockIdx.x * 64 + threadIdx.x), j = (int)(blockIdx.y * 4 + threadIdx.y), k = (int)blockIdx.z;
+	double v2 = 0.0;
+	if (i < L.m && j < L.n) {
+		const long long p = idx3(L, i, j, k), px = L.px, pxy = L.pxy;
+		const T *Ap = A + aidx3(L, 0, i, j, k);
+		T ax = (T)0;
+#pragma unroll
+		for (int e = 0; e < 27; e++) {
+			const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
+			ax = ax + Ap[e * px] * x[p + di + dj * px + dk * pxy];
+		}
+		const T v = r[p] - ax;
+		if (res) res[p] = (TR)v;
+		if (zx) zx[p] = (TR)0;
+		v2 = (double)v * (double)v;
+	}
+	if (partial) {
+		const double sblk = block_sum_256(v2);
+		if (threadIdx.x == 0 && threadIdx.y == 0)
+			partial[blockIdx.x + gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z)] = sblk;
+	}
+}
+
 int residual_partial_count(const LevelView &L) {
+	if (L.prd) return ((L.m + 63) / 64) * ((L.n + 3) / 4) * L.o;
 	return residual_blocks(L, 8);
 }
 
 template <typename T, typename TR>
 int launch_residual(hipStream_t s, const LevelView &L, const T *A, const T *x, const T *r, TR *res, double *partial,
 		    TR *zero_out) {
+	if (L.prd) {
+		// the ghost copies are not part of the vector: refreshing them
+		// leaves its value unchanged
+		launch_halo_wrap<T>(s, L, const_cast<T *>(x));
+		dim3 grid((L.m + 63) / 64, (L.n + 3) / 4, L.o), block(64, 4, 1);
+		hipLaunchKernelGGL((k_residual_pt<T, TR>), grid, block, 0, s, L, A, x, r, res, partial, zero_out);
+		HIPCHECK_LAUNCH();
+		return residual_partial_count(L);
+	}
 	const int G = residual_blocks(L, (int)sizeof(T));
 	hipLaunchKernelGGL((k_residual<T, TR>), dim3(G), dim3(64, 4, 1), 0, s, L, A, x, r, res, partial, zero_out);
 	HIPCHECK_LAUNCH();
@@ -1324,7 +1409,8 @@ int launch_sumsq(hipStream_t s, const LevelView &L, const T *r, double *partial)
 
 // out = A in at the interior points of L (region::mul_A with an outside
 // array, region.cc:1861-1869, used by region::Ax): the 27-entry sum in box
-// order from zero.  in must have a zero halo.
+// order from zero.  in must have a zero halo on non-periodic axes and the
+// ghost copies on periodic ones.
 __global__ __launch_bounds__(256) void k_apply(LevelView L, const double *__restrict__ A, const double *__restrict__ in,
 					       double *__restrict__ out) {
 	const int i = (int)(blockIdx.x * 64 + threadIdx.x), j = (int)(blockIdx.y * 4 + threadIdx.y), k = (int)blockIdx.z;
@@ -1404,22 +1490,32 @@ void launch_reduce(hipStream_t s, const double *partial, int n, double *out) {
 // injection flag for the y split of local (0,1,1) in boundary cubes
 // (region.cc:888-895); that is reproduced.  Also clears coarse x
 // (multigrid.cc:219, region.cc:1679-1682).
+// Periodic axes: no injection flag (erow is false on a periodic axis,
+// region.cc:666-690); on an even periodic axis the last cube's local 1
+// (fine M-1) also belongs to coarse 0, which the reference reaches through
+// its ghost strip.  In cube-scan order that cube comes after cube 0, so for
+// coarse 0 the axis visits cube 0 first and the wrapped last cube second.
 template <typename T>
 __device__ __forceinline__ void restrict_point(const LevelView &F, const LevelView &C, const T *__restrict__ res,
 					       T *__restrict__ rc, T *__restrict__ xc, int I, int J, int K) {
 	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
-	const bool evx = !(F.m & 1), evy = !(F.n & 1), evz = !(F.o & 1);
+	const bool px = F.prd & 1, py = (F.prd >> 1) & 1, pz = (F.prd >> 2) & 1;
+	const bool evx = !(F.m & 1) && !px, evy = !(F.n & 1) && !py, evz = !(F.o & 1) && !pz;
+	const bool wx = px && !(F.m & 1) && I == 0, wy = py && !(F.n & 1) && J == 0, wz = pz && !(F.o & 1) && K == 0;
 	T acc = (T)0;
-	for (int cz = 0; cz < 2; cz++) {
-		const int kc = K - 1 + cz;
+	for (int oz = 0; oz < 2; oz++) {
+		const int cz = wz ? 1 - oz : oz;
+		const int kc = (wz && cz == 0) ? lastz : K - 1 + cz;
 		if (kc < 0 || kc > lastz) continue;
 		const bool sz = evz && kc == lastz;
-		for (int cy = 0; cy < 2; cy++) {
-			const int jc = J - 1 + cy;
+		for (int oy = 0; oy < 2; oy++) {
+			const int cy = wy ? 1 - oy : oy;
+			const int jc = (wy && cy == 0) ? lasty : J - 1 + cy;
 			if (jc < 0 || jc > lasty) continue;
 			const bool sy = evy && jc == lasty;
-			for (int cx = 0; cx < 2; cx++) {
-				const int ic = I - 1 + cx;
+			for (int ox = 0; ox < 2; ox++) {
+				const int cx = wx ? 1 - ox : ox;
+				const int ic = (wx && cx == 0) ? lastx : I - 1 + cx;
 				if (ic < 0 || ic > lastx) continue;
 				const bool sx = evx && ic == lastx;
 				const bool partial = ic == lastx || jc == lasty || kc == lastz;
@@ -1570,13 +1666,20 @@ __device__ __forceinline__ void interp_cube(const LevelView &C, const LevelView 
 					    T *__restrict__ xf, int ic, int jc, int kc) {
 	typedef typename Vec2<T>::type V2;
 	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
-	const int i1 = ic + 1 < C.m ? ic + 1 : ic, j1 = jc + 1 < C.n ? jc + 1 : jc, k1 = kc + 1 < C.o ? kc + 1 : kc;
+	// on a periodic axis the upper neighbour of the last cube is coarse 0
+	// (the reference's ghost strip, region.cc:1369-1388), and no injection
+	// flag is set
+	const bool px = F.prd & 1, py = (F.prd >> 1) & 1, pz = (F.prd >> 2) & 1;
+	const int i1 = ic + 1 < C.m ? ic + 1 : (px ? 0 : ic);
+	const int j1 = jc + 1 < C.n ? jc + 1 : (py ? 0 : jc);
+	const int k1 = kc + 1 < C.o ? kc + 1 : (pz ? 0 : kc);
 	const T v0 = xc[idx3(C, ic, jc, kc)], v1 = xc[idx3(C, i1, jc, kc)];
 	const T v2 = xc[idx3(C, ic, j1, kc)], v3 = xc[idx3(C, i1, j1, kc)];
 	const T v4 = xc[idx3(C, ic, jc, k1)], v5 = xc[idx3(C, i1, jc, k1)];
 	const T v6 = xc[idx3(C, ic, j1, k1)], v7 = xc[idx3(C, i1, j1, k1)];
 	const bool full = ic < lastx && jc < lasty && kc < lastz;
-	const bool xs = (ic == lastx) && !(F.m & 1), ys = (jc == lasty) && !(F.n & 1), zs = (kc == lastz) && !(F.o & 1);
+	const bool xs = !px && (ic == lastx) && !(F.m & 1), ys = !py && (jc == lasty) && !(F.n & 1);
+	const bool zs = !pz && (kc == lastz) && !(F.o & 1);
 	const bool two = 2 * ic + 1 < F.m;
 #pragma unroll
 	for (int lk = 0; lk < 2; lk++) {
@@ -1904,7 +2007,82 @@ __global__ __launch_bounds__(256) void k_rat_boundary(LevelView F, LevelView C, 
 // one per thread.
 constexpr long long kRatEntrySplit = 1LL << 17;
 
+// Galerkin entries with periodic axes, one (coarse point, entry) per thread,
+// on the unwrapped index line of each periodic axis (the reference's
+// periodic grid_map/igrid_map, region.cc:1201-1242).  One axis of P: coarse
+// J sits on fine pfpos(J); a fine point on a coarse position takes that
+// value alone, any other half of each coarse neighbour.  Entry d of coarse
+// row I sums, over fine f with P(f,I) != 0 in (k,j,i) order,
+// P(f,I) * [sum over f's box g in (k,j,i) order of 0.125*Af(f,g)*P(g,I+d)],
+// the bracket formed as A_red forms At (region.cc:1117-1166).
+__device__ __forceinline__ int pfpos(int J, int M, int Mc, bool prd) {
+	if (prd) {
+		const int q = J >= 0 ? J / Mc : -((-J + Mc - 1) / Mc);
+		return 2 * (J - q * Mc) + M * q;
+	}
+	return (!(M & 1) && J == Mc - 1) ? M - 1 : 2 * J;
+}
+__device__ __forceinline__ double pweight(int g, int J, int M, int Mc, bool prd) {
+	const int f = pfpos(J, M, Mc, prd);
+	const bool c = prd ? ((((g % M) + M) % M) & 1) == 0 : ((g & 1) == 0 || (!(M & 1) && g == M - 1));
+	if (c) return g == f ? 1.0 : 0.0;
+	return (g - f == 1 || f - g == 1) ? 0.5 : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_rat_prd(LevelView F, LevelView C, const double *__restrict__ Af,
+						 double *__restrict__ Ac) {
+	const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+	const long long npts = (long long)C.m * C.n * C.o;
+	if (t >= npts * 27) return;
+	const int d = (int)(t % 27);
+	const long long pc = t / 27;
+	const int I[3] = {(int)(pc % C.m), (int)((pc / C.m) % C.n), (int)(pc / ((long long)C.m * C.n))};
+	const bool P[3] = {(F.prd & 1) != 0, (F.prd & 2) != 0, (F.prd & 4) != 0};
+	const int Mf[3] = {F.m, F.n, F.o}, Mc[3] = {C.m, C.n, C.o};
+	const int J[3] = {I[0] + d % 3 - 1, I[1] + (d / 3) % 3 - 1, I[2] + d / 9 - 1};
+	double acc = 0.0;
+	bool ok = true;
+	int fc[3];
+	for (int a = 0; a < 3; a++) {
+		if (!P[a] && (J[a] < 0 || J[a] >= Mc[a])) ok = false;
+		fc[a] = pfpos(I[a], Mf[a], Mc[a], P[a]);
+	}
+	if (ok) {
+		for (int fz = fc[2] - 1; fz <= fc[2] + 1; fz++)
+		for (int fy = fc[1] - 1; fy <= fc[1] + 1; fy++)
+		for (int fx = fc[0] - 1; fx <= fc[0] + 1; fx++) {
+			const int f[3] = {fx, fy, fz};
+			double w = 1.0;
+			int fw[3];
+			for (int a = 0; a < 3; a++) {
+				if (!P[a] && (f[a] < 0 || f[a] >= Mf[a])) w = 0.0;
+				else w = w * pweight(f[a], I[a], Mf[a], Mc[a], P[a]);
+				fw[a] = ((f[a] % Mf[a]) + Mf[a]) % Mf[a];
+			}
+			if (w == 0.0) continue;
+			double sum = 0.0;
+			for (int e = 0; e < 27; e++) {
+				const int g[3] = {f[0] + e % 3 - 1, f[1] + (e / 3) % 3 - 1, f[2] + e / 9 - 1};
+				double pw = 1.0;
+				for (int a = 0; a < 3; a++) {
+					if (!P[a] && (g[a] < 0 || g[a] >= Mf[a])) pw = 0.0;
+					else pw = pw * pweight(g[a], J[a], Mf[a], Mc[a], P[a]);
+				}
+				if (pw != 0.0) sum = sum + 0.125 * Af[aidx3(F, e, fw[0], fw[1], fw[2])] * pw;
+			}
+			acc = acc + sum * w;
+		}
+	}
+	Ac[aidx3(C, d, I[0], I[1], I[2])] = acc;
+}
+
 void launch_rat(hipStream_t s, const LevelView &F, const LevelView &C, const double *Af, double *Ac) {
+	if (F.prd) {
+		const long long tot = (long long)C.m * C.n * C.o * 27;
+		hipLaunchKernelGGL(k_rat_prd, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, F, C, Af, Ac);
+		HIPCHECK_LAUNCH();
+		return;
+	}
 	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
 	if (lastx > 1 && lasty > 1 && lastz > 1) {
 		const long long ni = (long long)(lastx - 1) * (lasty - 1) * (lastz - 1);
@@ -1938,7 +2116,32 @@ __global__ void k_lu_assemble(LevelView L, const double *__restrict__ A, double 
 	LU[p + (long long)q * N] = A[aidx3(L, e, i, j, k)];
 }
 
+// With periodic axes, one thread per row: a wrapped neighbour's column is its
+// wrapped index (step_mod, region.hh:466; region_tr.cc:493-501), and entries
+// are stored in box order, so where two offsets reach one column (an axis
+// of size 1 or 2) the later one is kept, as in the reference.
+__global__ void k_lu_assemble_prd(LevelView L, const double *__restrict__ A, double *__restrict__ LU) {
+	const int N = L.m * L.n * L.o;
+	const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (p >= N) return;
+	const int i = p % L.m, j = (p / L.m) % L.n, k = p / (L.m * L.n);
+	for (int e = 0; e < 27; e++) {
+		int qi = i + e % 3 - 1, qj = j + (e / 3) % 3 - 1, qk = k + e / 9 - 1;
+		if (qi < 0 || qi >= L.m) { if (!(L.prd & 1)) continue; qi = (qi + L.m) % L.m; }
+		if (qj < 0 || qj >= L.n) { if (!(L.prd & 2)) continue; qj = (qj + L.n) % L.n; }
+		if (qk < 0 || qk >= L.o) { if (!(L.prd & 4)) continue; qk = (qk + L.o) % L.o; }
+		const int q = qi + L.m * (qj + L.n * qk);
+		LU[p + (long long)q * N] = A[aidx3(L, e, i, j, k)];
+	}
+}
+
 void launch_lu_assemble(hipStream_t s, const LevelView &L, const double *A, double *LU) {
+	if (L.prd) {
+		const int N = L.m * L.n * L.o;
+		hipLaunchKernelGGL(k_lu_assemble_prd, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, L, A, LU);
+		HIPCHECK_LAUNCH();
+		return;
+	}
 	const long long total = (long long)L.m * L.n * L.o * 27;
 	hipLaunchKernelGGL(k_lu_assemble, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, L, A, LU);
 	HIPCHECK_LAUNCH();
@@ -2076,15 +2279,15 @@ void launch_lu_solve(hipStream_t s, const LevelView &L, int N, const double *LU,
 
 // ------------------------------------------------- generation, layout
 
-// Face-coefficient (7-point) families; the family is a template constant so
-// the row stays in registers.
+// One point per thread (the 7-point families, and Q1 on a grid with periodic
+// axes); the family is a template constant so the row stays in registers.
 template <int FAM>
 __global__ __launch_bounds__(256) void k_generate(LevelView L, uint64_t seed, double *__restrict__ A,
 						  double *__restrict__ r) {
 	const int i = (int)(blockIdx.x * 64 + threadIdx.x), j = (int)(blockIdx.y * 4 + threadIdx.y), k = (int)blockIdx.z;
 	if (i >= L.m || j >= L.n) return;
 	double row[27];
-	pgmg_row27(FAM, seed, L.m, L.n, L.o, i, j, k, row);
+	pgmg_row27_prd(FAM, seed, L.m, L.n, L.o, L.prd, i, j, k, row);
 	const long long p = idx3(L, i, j, k);
 	double *Ap = A + aidx3(L, 0, i, j, k);
 #pragma unroll
@@ -2169,7 +2372,11 @@ __global__ __launch_bounds__(256) void k_generate_q1(LevelView L, uint64_t seed,
 
 void launch_generate(hipStream_t s, const LevelView &L, int family, uint64_t seed, double *A, double *r) {
 	dim3 block(64, 4, 1);
-	if (family == PGMG_FAM_Q1_ANISO) {
+	if (family == PGMG_FAM_Q1_ANISO && L.prd) {
+		// periodic cells: the per-point form, which wraps the cell indices
+		dim3 grid((L.m + 63) / 64, (L.n + 3) / 4, L.o);
+		hipLaunchKernelGGL(k_generate<PGMG_FAM_Q1_ANISO>, grid, block, 0, s, L, seed, A, r);
+	} else if (family == PGMG_FAM_Q1_ANISO) {
 		// planes per block: as many as keep >= 2048 blocks, at most 8
 		const long long cols = (long long)((L.m + 63) / 64) * ((L.n + 3) / 4);
 		int kz = 8;
@@ -2541,6 +2748,8 @@ void launch_b_add_x(hipStream_t s, const SlotDev *slots, WorkList wl) {
 	template void launch_b_lu_solve<T>(hipStream_t, const SlotDev *, WorkList, int);
 INST(double)
 INST(float)
+template void launch_halo_wrap<double>(hipStream_t, const LevelView &, double *);
+template void launch_halo_wrap<float>(hipStream_t, const LevelView &, float *);
 template int launch_residual<double, double>(hipStream_t, const LevelView &, const double *, const double *, const double *, double *, double *, double *);
 template int launch_residual<double, float>(hipStream_t, const LevelView &, const double *, const double *, const double *, float *, double *, float *);
 template int launch_residual<float, float>(hipStream_t, const LevelView &, const float *, const float *, const float *, float *, double *, float *);

@@ -17,6 +17,7 @@
 //    outside-array helpers (copy_rhs_to_soln, soln_diff, overwrite_r_field,
 //    update_r_field) on the result.
 //   mg_drivers <cycles|guide|fmg|slices|pcg> <family> <seed> <m> <n> <o> <x_out> [smoother|k] [bottom]
+// <family> may carry the periodic axes as ":<mask>" (1 = x, 2 = y, 4 = z).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -28,7 +29,9 @@
 int main(int argc, char **argv) {
 	if (argc < 8) {fprintf(stderr, "usage: mg_drivers cycles|guide|fmg family seed m n o x_out [smoother] [bottom]\n"); return 2;}
 	const char *mode = argv[1];
-	geometry gm(atoi(argv[4]), atoi(argv[5]), atoi(argv[6]), false, false, false);
+	const char *colon = strchr(argv[2], ':');
+	const int prd = colon ? atoi(colon + 1) : 0;
+	geometry gm(atoi(argv[4]), atoi(argv[5]), atoi(argv[6]), (prd & 1) != 0, (prd & 2) != 0, (prd & 4) != 0);
 	synthetic_problem p(gm, atoi(argv[2]), strtoull(argv[3], NULL, 10));
 	comm_buffer com;
 	pgmg_opts opts;

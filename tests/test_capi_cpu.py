@@ -46,13 +46,10 @@ def test_hierarchy_shape_matches_oracle(dims):
     assert pgmg_amd.hierarchy_shape(*dims) == O.hierarchy(*dims)
 
 
-def test_periodic_is_rejected():
-    d = pgmg_amd.Dims(9, 9, 9, 1, 0, 0)
-    nl = ctypes.c_int()
-    out = (ctypes.c_int * 48)()
-    rc = pgmg_amd.lib().pgmg_hierarchy_shape(ctypes.byref(d), ctypes.byref(nl), out)
-    assert rc == 9  # PGMG_ERR_UNSUPPORTED
-    assert b"periodic" in pgmg_amd.lib().pgmg_last_error()
+@pytest.mark.parametrize("dims,prd", [((64, 64, 64), 7), ((33, 33, 33), 1), ((20, 17, 30), 6), ((13, 10, 11), 5),
+                                      ((256, 256, 256), 7), ((100, 2, 300), 3)])
+def test_periodic_hierarchy_shape_matches_oracle(dims, prd):
+    assert pgmg_amd.hierarchy_shape(*dims, prd=prd) == O.hierarchy(*dims, prd)
 
 
 def test_footprint_scales():

@@ -154,3 +154,21 @@ def test_pcg_with_vcycle_preconditioner():
     assert bb2 == bb and true_res < 1e-18 * bb
     assert vals["diff_self"][0] == 0.0 and vals["diff_rhs"][0] == 0.0
     assert abs(vals["rhs_ratio"][0] - 2.25) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fam,seed,dims,prd", [(1, 5, (13, 10, 11), 5), (3, 7, (12, 18, 9), 3)])
+def test_multigrid_cycles_periodic(tmp_path, fam, seed, dims, prd):
+    # a p_class with periodic axes through the drop-in API, against the oracle
+    # (itself pinned to the reference's periodic runs)
+    m, n, o = dims
+    xf = tmp_path / "x.bin"
+    out = run(["mg_drivers", "cycles", "%d:%d" % (fam, prd), seed, m, n, o, xf]).splitlines()
+    assert out[0] == "levels %d " % len(O.hierarchy(m, n, o, prd)) + " ".join(
+        "%dx%dx%d" % d for d in O.hierarchy(m, n, o, prd))
+    res = [float(l.split()[2]) for l in out if l.startswith("res ")]
+    mg = O.Multigrid(O.gen_operator(fam, seed, m, n, o, prd), prd=prd)
+    mg.set_rhs(O.gen_rhs(seed, m, n, o))
+    c, hist = mg.solve(1e-10, 100)
+    np.testing.assert_allclose(res, hist, rtol=1e-12)
+    assert np.array_equal(np.fromfile(xf).reshape(o, n, m), mg.get_x(0))
