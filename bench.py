@@ -120,7 +120,7 @@ def simple_periodic_operator(m, n, o):
     return A, b
 
 
-def periodic_256(P, tests=5, iters=6):
+def periodic_256(P, tests=5, iters=6, modes=("exact_fp64", "colour8_fp64", "colour8_mixed"), ref_cpu=True):
     """The reference's own benchmark, mg_parallel.cc: a 256^3 grid periodic in
     every axis, problem_simple, `tests` runs of `iters` V-cycles from x = 0,
     digits gained = log10(err_start) - log10(err_end).  Here in both smoothers
@@ -133,6 +133,8 @@ def periodic_256(P, tests=5, iters=6):
     for name, kw in [("exact_fp64", dict(smoother=P.SMOOTH_EXACT, precision=P.PREC_FP64)),
                      ("colour8_fp64", dict(smoother=P.SMOOTH_COLOUR8, precision=P.PREC_FP64)),
                      ("colour8_mixed", dict(smoother=P.SMOOTH_COLOUR8, precision=P.PREC_MIXED))]:
+        if name not in modes:
+            continue
         h = P.Hierarchy(n, n, n, prd=7, use_graph=True, **kw)
         h.set_operator27(A)
         h.set_rhs(b)
@@ -160,7 +162,7 @@ def periodic_256(P, tests=5, iters=6):
         log("periodic 256 %s: %.3f s per %d cycles, %.2f digits" % (name, np.mean(ts), iters, digs))
     ref = os.path.join(ROOT, "oracle", "_ref", "mg_parallel")
     mpiexec = "/opt/conda/bin/mpiexec"
-    if os.path.exists(ref) and os.path.exists(mpiexec):
+    if ref_cpu and os.path.exists(ref) and os.path.exists(mpiexec):
         ranks = max(1, min(16, os.cpu_count() or 1))
         try:
             p = subprocess.run([mpiexec, "-n", str(ranks), ref], capture_output=True, text=True, timeout=400,
@@ -309,6 +311,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the 513^3, batch and serving legs")
     ap.add_argument("--leg", choices=["pcie", "q1_513", "batch_mixed", "batch_fp64", "serving", "periodic_256"],
                     help="run one extra leg alone and print its JSON (for profiling)")
+    ap.add_argument("--periodic-modes", default="exact_fp64,colour8_fp64,colour8_mixed",
+                    help="with --leg periodic_256: the smoother/precision runs to time (comma-separated)")
+    ap.add_argument("--periodic-tests", type=int, default=5, help="with --leg periodic_256: timed runs per mode")
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware counters: a 1 GiB elementwise copy (torch.mul, "
                          "calibration of known size) then two mixed 257^3 V-cycles")
@@ -324,7 +329,9 @@ def main():
     if args.leg:
         fn = {"pcie": lambda: pcie_inclusive(P), "q1_513": lambda: q1_513(P), "batch_mixed": lambda: batch_256(P, P.PREC_MIXED),
               "batch_fp64": lambda: batch_256(P, P.PREC_FP64), "serving": lambda: serving(P),
-              "periodic_256": lambda: periodic_256(P)}[args.leg]
+              "periodic_256": lambda: periodic_256(P, tests=args.periodic_tests,
+                                                   modes=tuple(args.periodic_modes.split(",")),
+                                                   ref_cpu=not args.no_cpu)}[args.leg]
         print(json.dumps({"leg": args.leg, "result": fn()}), flush=True)
         return
 
