@@ -207,10 +207,12 @@ void op_gs(pgmg_hier *h, int level, int mode, int nsweeps, bool from_zero = fals
 	const bool exact = h->opts.smoother == PGMG_SMOOTH_EXACT;
 	h->gs_count += nsweeps;
 	if (nsweeps <= 0) return;
-	if (L.v.prd) {
+	const bool use_small = gs_use_small<T>(h, L, mode);
+	if (L.v.prd && !use_small) {
 		// Periodic axes: the ghost copies are refreshed before each pass
 		// (region.cc:361, 457), then the pass runs on the per-point kernels,
-		// which read the halo like any neighbour.
+		// which read the halo like any neighbour.  (The one-workgroup kernel
+		// refreshes them itself.)
 		const int tmax = (L.v.m - 1) + 2 * (L.v.n - 1) + 4 * (L.v.o - 1);
 		for (int s = 0; s < nsweeps; s++)
 			for (int pass = 0; pass < (mode ? 2 : 1); pass++) {
@@ -226,7 +228,6 @@ void op_gs(pgmg_hier *h, int level, int mode, int nsweeps, bool from_zero = fals
 			}
 		return;
 	}
-	const bool use_small = gs_use_small<T>(h, L, mode);
 	if (use_small) {
 		ProfScope ps(h, KC_GS, level, L.points() * nsweeps * (mode ? 2 : 1));
 		launch_gs_small<T>(h->stream, L.v, lA<T>(L), lx<T>(L), lr<T>(L), nsweeps, mode, exact ? 1 : 0);

@@ -58,36 +58,60 @@ __device__ __forceinline__ void gs_point(const LevelView &L, const T *__restrict
 // and get the same value); a halo point outside a non-periodic axis keeps
 // its zero.  Every source is a grid point, never a halo point, so the
 // copies do not depend on one another.
-template <typename T>
-__global__ __launch_bounds__(256) void k_halo_wrap(LevelView L, T *__restrict__ x) {
-	const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+// Halo point q of the six faces of L: its indices, and those of the grid
+// point it copies; false when it lies outside a non-periodic axis.
+__device__ __forceinline__ bool halo_src(const LevelView &L, long long q, int &i, int &j, int &k, int &si, int &sj,
+					 int &sk) {
 	const long long ex = 2LL * (L.n + 2) * (L.o + 2), ey = 2LL * (L.m + 2) * (L.o + 2);
-	const long long ez = 2LL * (L.m + 2) * (L.n + 2);
-	int i, j, k;
 	if (q < ex) {
 		const long long t = q >> 1;
 		i = (q & 1) ? L.m : -1; j = (int)(t % (L.n + 2)) - 1; k = (int)(t / (L.n + 2)) - 1;
 	} else if (q < ex + ey) {
 		const long long t = (q - ex) >> 1;
 		j = ((q - ex) & 1) ? L.n : -1; i = (int)(t % (L.m + 2)) - 1; k = (int)(t / (L.m + 2)) - 1;
-	} else if (q < ex + ey + ez) {
+	} else {
 		const long long t = (q - ex - ey) >> 1;
 		k = ((q - ex - ey) & 1) ? L.o : -1; i = (int)(t % (L.m + 2)) - 1; j = (int)(t / (L.m + 2)) - 1;
-	} else {
-		return;
 	}
-	int si = i, sj = j, sk = k;
-	if (si < 0 || si >= L.m) { if (!(L.prd & 1)) return; si = si < 0 ? si + L.m : si - L.m; }
-	if (sj < 0 || sj >= L.n) { if (!(L.prd & 2)) return; sj = sj < 0 ? sj + L.n : sj - L.n; }
-	if (sk < 0 || sk >= L.o) { if (!(L.prd & 4)) return; sk = sk < 0 ? sk + L.o : sk - L.o; }
+	si = i; sj = j; sk = k;
+	if (si < 0 || si >= L.m) { if (!(L.prd & 1)) return false; si = si < 0 ? si + L.m : si - L.m; }
+	if (sj < 0 || sj >= L.n) { if (!(L.prd & 2)) return false; sj = sj < 0 ? sj + L.n : sj - L.n; }
+	if (sk < 0 || sk >= L.o) { if (!(L.prd & 4)) return false; sk = sk < 0 ? sk + L.o : sk - L.o; }
+	return true;
+}
+
+__host__ __device__ inline long long halo_points(const LevelView &L) {
+	return 2LL * ((long long)(L.n + 2) * (L.o + 2) + (long long)(L.m + 2) * (L.o + 2) + (long long)(L.m + 2) * (L.n + 2));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_halo_wrap(LevelView L, T *__restrict__ x) {
+	const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+	int i, j, k, si, sj, sk;
+	if (q >= halo_points(L) || !halo_src(L, q, i, j, k, si, sj, sk)) return;
 	x[idx3(L, i, j, k)] = x[idx3(L, si, sj, sk)];
+}
+
+// The same copies by one workgroup, between barriers, on an array whose
+// point (i,j,k) is at org + i + hx*j + hxy*k (a level in memory, or one staged
+// in LDS).  No-op without periodic axes.
+template <typename T>
+__device__ __forceinline__ void halo_wrap_block(const LevelView &L, T *__restrict__ x, long long org, long long hx,
+						long long hxy) {
+	if (!L.prd) return;
+	__syncthreads();
+	const long long tot = halo_points(L);
+	for (long long q = threadIdx.x; q < tot; q += blockDim.x) {
+		int i, j, k, si, sj, sk;
+		if (halo_src(L, q, i, j, k, si, sj, sk)) x[org + i + hx * j + hxy * k] = x[org + si + hx * sj + hxy * sk];
+	}
+	__syncthreads();
 }
 
 template <typename T>
 void launch_halo_wrap(hipStream_t s, const LevelView &L, T *x) {
 	if (!L.prd) return;
-	const long long tot = 2LL * ((long long)(L.n + 2) * (L.o + 2) + (long long)(L.m + 2) * (L.o + 2) +
-				     (long long)(L.m + 2) * (L.n + 2));
+	const long long tot = halo_points(L);
 	hipLaunchKernelGGL(k_halo_wrap<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, L, x);
 	HIPCHECK_LAUNCH();
 }
@@ -857,6 +881,8 @@ __device__ __forceinline__ void gs_small_body(const LevelView &L, const T *__res
 	const int npair = L.n * L.o;
 	for (int sw = 0; sw < nsweeps; sw++) {
 		for (int pass = 0; pass < (mode ? 2 : 1); pass++) {
+			// periodic axes: ghost copies before each pass (region.cc:361, 457)
+			halo_wrap_block<T>(L, x, L.base, L.px, L.pxy);
 			if (exact) {
 				for (int tt = 0; tt <= tmax; tt++) {
 					const int t = pass ? tmax - tt : tt;
@@ -1010,6 +1036,7 @@ __device__ __forceinline__ void gs_lds_run(const LevelView &L, const T *__restri
 	if (staged) {
 #pragma unroll 1
 		for (int sw = 0; sw < nsweeps; sw++) {
+			halo_wrap_block<T>(L, xs, 1 + hx + hxy, hx, hxy);
 #pragma unroll 1
 			for (int pp = 0; pp < 4; pp++) {
 				gs_lds_pair_pass<T>(L, A, r, xs, pp, hx, hxy);
@@ -1021,6 +1048,7 @@ __device__ __forceinline__ void gs_lds_run(const LevelView &L, const T *__restri
 	for (int sw = 0; sw < nsweeps; sw++) {
 #pragma unroll 1
 		for (int pass = 0; pass < (mode ? 2 : 1); pass++) {
+			halo_wrap_block<T>(L, xs, 1 + hx + hxy, hx, hxy);
 			if (exact) {
 #pragma unroll 1
 				for (int tt = 0; tt <= tmax; tt++) {
