@@ -214,6 +214,11 @@ void op_gs(pgmg_hier *h, int level, int mode, int nsweeps, bool from_zero = fals
 		// which read the halo like any neighbour.  (The one-workgroup kernel
 		// refreshes them itself.)
 		const int tmax = (L.v.m - 1) + 2 * (L.v.n - 1) + 4 * (L.v.o - 1);
+		// The colour-pair row kernels read the halo point m from memory where
+		// a row ends, except in the point out of a row of ppl*q+1 points,
+		// which takes it as zero: with a periodic x axis such rows use the
+		// per-class kernel.
+		const bool pair_ok = !(L.v.prd & 1) || !row_lanes(L.v.m, (int)sizeof(T)).ext;
 		for (int s = 0; s < nsweeps; s++)
 			for (int pass = 0; pass < (mode ? 2 : 1); pass++) {
 				ProfScope ps(h, KC_GS, level, L.points());
@@ -221,6 +226,9 @@ void op_gs(pgmg_hier *h, int level, int mode, int nsweeps, bool from_zero = fals
 				if (exact) {
 					for (int tt = 0; tt <= tmax; tt++)
 						launch_gs_plane<T>(h->stream, L.v, lA<T>(L), lx<T>(L), lr<T>(L), pass ? tmax - tt : tt);
+				} else if (pass == 0 && pair_ok) {
+					for (int q = 0; q < 4; q++)
+						launch_gs_pair<T>(h->stream, L.v, lA<T>(L), lx<T>(L), lr<T>(L), q, false);
 				} else {
 					for (int c = 0; c < 8; c++)
 						launch_gs_colour<T>(h->stream, L.v, lA<T>(L), lx<T>(L), lr<T>(L), pass ? 7 - c : c);

@@ -255,7 +255,7 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 		// the new even values exist.  Products are the same roundings the
 		// in-order sum would form.
 		T ax = (T)0, ay = (T)0, az = (T)0, ac_e = (T)1, ac_o = (T)1, ac_z = (T)1, a12 = (T)0, a14 = (T)0;
-		T rx = (T)0, ry = (T)0, rz = (T)0;
+		T rx = (T)0, ry = (T)0, rz = (T)0, xh = (T)0;
 		T prod[12];
 		if (live) {
 			const V2 rr = ld_stream<V2>(rr_, ob);
@@ -275,6 +275,7 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 					ax = ax + am.x * xm;
 					ax = ax + ap.x * xv.y;
 					ac_e = a0.x; ac_o = a0.y; a12 = am.y; a14 = ap.y;
+					xh = xp;  // i0+2: the halo point m when the row ends there
 				} else {
 					ax = ax + am.x * xm;
 					ax = ax + a0.x * xv.x;
@@ -310,10 +311,12 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 		if (doE) xe = (rx - ax) * ((T)1 / ac_e);
 		else if (live) xe = carry;
 		if (ext) xz = (rz - az) * ((T)1 / ac_z);
-		// Shuffles run with every lane active; lanes past the row end hold 0,
-		// as does lane 63's point out when it lies in the halo.
+		// Shuffles run with every lane active.  The odd point's right
+		// neighbour is the next lane's new even point, lane 63's point out,
+		// or, where the row ends (i0+2 = m), the halo value: zero on a
+		// non-periodic axis, the ghost copy on a periodic one.
 		const T xs = __shfl_down(xe, 1, 64);
-		const T xe_next = lane == 63 ? xz : xs;
+		const T xe_next = i0 + 2 >= L.m ? xh : (lane == 63 ? xz : xs);
 		if (doO) {
 			// odd point i0+1: its even neighbours in the own row are new
 			ay = ay + a12 * xe;
@@ -375,7 +378,7 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 		// sums' roundings)
 		float e0 = 0.0f, e2 = 0.0f, ex = 0.0f, o1 = 0.0f, o3 = 0.0f;
 		float c0 = 1.0f, c1 = 1.0f, c2 = 1.0f, c3 = 1.0f, cx = 1.0f;
-		float w1m = 0.0f, w1p = 0.0f, w3m = 0.0f, w3p = 0.0f;
+		float w1m = 0.0f, w1p = 0.0f, w3m = 0.0f, w3p = 0.0f, xh2 = 0.0f, xh4 = 0.0f;
 		float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f, rx = 0.0f;
 		float p1[12], p3[12];
 		if (live) {
@@ -413,6 +416,7 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 					ex = ex + bp * x5;
 					c0 = a0.x; c1 = a0.y; c2 = a0.z; c3 = a0.w; cx = b0;
 					w1m = am.y; w1p = ap.y; w3m = am.w; w3p = ap.w;
+					xh2 = xv.z; xh4 = xp;  // own-row i0+2, i0+4: the halo where the row ends
 				} else {
 					e0 = e0 + am.x * xm;
 					e0 = e0 + a0.x * xv.x;
@@ -456,14 +460,14 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 		const float x4 = lane == 63 ? xx : xn;  // lane 63: the point out, or the halo
 		if (doO1) {
 			o1 = o1 + w1m * x0;
-			o1 = o1 + w1p * x2;
+			o1 = o1 + w1p * (i0 + 2 >= L.m ? xh2 : x2);
 #pragma unroll
 			for (int q2 = 0; q2 < 12; q2++) o1 = o1 + p1[q2];
 			y1 = (r1 - o1) * (1.0f / c1);
 		}
 		if (doO3) {
 			o3 = o3 + w3m * x2;
-			o3 = o3 + w3p * x4;
+			o3 = o3 + w3p * (i0 + 4 >= L.m ? xh4 : x4);
 #pragma unroll
 			for (int q2 = 0; q2 < 12; q2++) o3 = o3 + p3[q2];
 			y3 = (r3 - o3) * (1.0f / c3);
@@ -579,7 +583,7 @@ __device__ __forceinline__ void gs_lanes(const LevelView &L, const T *__restrict
 	const uint32_t oe = (uint32_t)(PPL * sizeof(T));
 	T s[PPL], cc[PPL], wm[PPL], wp[PPL], rv[PPL];
 	T prod[PPL / 2][12];
-	T sx = (T)0, cx = (T)1, rx = (T)0;
+	T sx = (T)0, cx = (T)1, rx = (T)0, xhalo = (T)0;
 #pragma unroll
 	for (int t = 0; t < PPL; t++) { s[t] = (T)0; cc[t] = (T)1; wm[t] = (T)0; wp[t] = (T)0; rv[t] = (T)0; }
 	if (live) {
@@ -605,6 +609,13 @@ __device__ __forceinline__ void gs_lanes(const LevelView &L, const T *__restrict
 			if (ext) {
 #pragma unroll
 				for (int d = 0; d < 3; d++) b[d] = ld_row<T>(Ab + (3 * rq + d) * px, oa + oe);
+			}
+			if (rq == 4) {
+				// own-row value at m, where the row ends inside this lane
+#pragma unroll
+				for (int t = 1; t < PPL; t++)
+					if (i0 + t == L.m) xhalo = xv[t];
+				if (i0 + PPL == L.m) xhalo = xp;
 			}
 #pragma unroll
 			for (int t = 0; t < PPL; t++) {
@@ -659,7 +670,10 @@ __device__ __forceinline__ void gs_lanes(const LevelView &L, const T *__restrict
 #pragma unroll
 	for (int t = 1; t < PPL; t += 2) {
 		if (live && i0 + t < L.m) {
-			const T rt = t + 1 < PPL ? xn[t + 1 < PPL ? t + 1 : 0] : right;
+			// right neighbour: a new even point, or the halo value (zero on a
+			// non-periodic axis, the ghost copy on a periodic one) where the
+			// row ends
+			const T rt = i0 + t + 1 >= L.m ? xhalo : (t + 1 < PPL ? xn[t + 1 < PPL ? t + 1 : 0] : right);
 			T acc = s[t];
 			acc = acc + wm[t] * xn[t - 1];
 			acc = acc + wp[t] * rt;

@@ -195,3 +195,27 @@ def test_periodic_64_cube_properties(prd):
     e6 = h.l2_error_all()
     assert e6 < 1e-6 * e0
     h.close()
+
+
+@pytest.mark.parametrize("dims,prd", [((262, 6, 5), 1), ((256, 4, 4), 7), ((130, 7, 6), 3), ((18, 22, 14), 7)])
+def test_periodic_row_kernels_colour8(dims, prd):
+    # rows long enough for the segmented row kernels (FP64 beyond 128 points,
+    # FP32 beyond 256), rows ending inside a lane, and a periodic x axis,
+    # whose halo point m the row kernels read from the ghost copy
+    say("case", dims, "prd", prd)
+    m, n, o = dims
+    meta = (1, 41, m, n, o, prd)
+    for prec in (P.PREC_FP64, P.PREC_MIXED):
+        h, A, b = setup(meta, smoother=P.SMOOTH_COLOUR8, precision=prec)
+        mg = O.Multigrid(A, colour8=True, prd=prd)
+        mg.set_rhs(b)
+        if prec == P.PREC_FP64:
+            for c in range(3):
+                h.v_cycle()
+                mg.v_cycle()
+        else:
+            c_o, hist_o = mg.solve(1e-30, 3, mixed=True)
+            c_g, hist_g = h.solve(1e-30, 3, check=False)
+            assert c_g == c_o == 3
+        assert np.array_equal(h.get_x(0), mg.get_x(0)), prec
+        h.close()
