@@ -2071,46 +2071,63 @@ __device__ __forceinline__ double pweight(int g, int J, int M, int Mc, bool prd)
 	return (g - f == 1 || f - g == 1) ? 0.5 : 0.0;
 }
 
+// The one-axis factors of an entry take few values: for each axis a thread
+// tabulates P(f,I) over its three fine positions f and P(g,J) over their
+// three box neighbours g, so the 27 x 27 terms need no integer division.
+// The products are formed in the same order (axis x, y, z, from 1) and are
+// exact (powers of two or zero), so the sums are those of the untabulated
+// form term by term.
 __global__ __launch_bounds__(256) void k_rat_prd(LevelView F, LevelView C, const double *__restrict__ Af,
 						 double *__restrict__ Ac) {
 	const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
 	const long long npts = (long long)C.m * C.n * C.o;
 	if (t >= npts * 27) return;
-	const int d = (int)(t % 27);
-	const long long pc = t / 27;
+	// entry-major: the lanes of a wavefront take one entry d at neighbouring
+	// coarse points, so their zero factors (and skipped loads) agree
+	const int d = (int)(t / npts);
+	const long long pc = t - (long long)d * npts;
 	const int I[3] = {(int)(pc % C.m), (int)((pc / C.m) % C.n), (int)(pc / ((long long)C.m * C.n))};
 	const bool P[3] = {(F.prd & 1) != 0, (F.prd & 2) != 0, (F.prd & 4) != 0};
 	const int Mf[3] = {F.m, F.n, F.o}, Mc[3] = {C.m, C.n, C.o};
 	const int J[3] = {I[0] + d % 3 - 1, I[1] + (d / 3) % 3 - 1, I[2] + d / 9 - 1};
-	double acc = 0.0;
 	bool ok = true;
-	int fc[3];
-	for (int a = 0; a < 3; a++) {
+	for (int a = 0; a < 3; a++)
 		if (!P[a] && (J[a] < 0 || J[a] >= Mc[a])) ok = false;
-		fc[a] = pfpos(I[a], Mf[a], Mc[a], P[a]);
-	}
+	double acc = 0.0;
 	if (ok) {
-		for (int fz = fc[2] - 1; fz <= fc[2] + 1; fz++)
-		for (int fy = fc[1] - 1; fy <= fc[1] + 1; fy++)
-		for (int fx = fc[0] - 1; fx <= fc[0] + 1; fx++) {
-			const int f[3] = {fx, fy, fz};
-			double w = 1.0;
-			int fw[3];
-			for (int a = 0; a < 3; a++) {
-				if (!P[a] && (f[a] < 0 || f[a] >= Mf[a])) w = 0.0;
-				else w = w * pweight(f[a], I[a], Mf[a], Mc[a], P[a]);
-				fw[a] = ((f[a] % Mf[a]) + Mf[a]) % Mf[a];
-			}
-			if (w == 0.0) continue;
-			double sum = 0.0;
-			for (int e = 0; e < 27; e++) {
-				const int g[3] = {f[0] + e % 3 - 1, f[1] + (e / 3) % 3 - 1, f[2] + e / 9 - 1};
-				double pw = 1.0;
-				for (int a = 0; a < 3; a++) {
-					if (!P[a] && (g[a] < 0 || g[a] >= Mf[a])) pw = 0.0;
-					else pw = pw * pweight(g[a], J[a], Mf[a], Mc[a], P[a]);
+		double wf[3][3], pg[3][3][3];
+		int fw[3][3];
+#pragma unroll
+		for (int a = 0; a < 3; a++) {
+			const int fc = pfpos(I[a], Mf[a], Mc[a], P[a]);
+#pragma unroll
+			for (int df = 0; df < 3; df++) {
+				const int f = fc + df - 1;
+				const bool vf = P[a] || (f >= 0 && f < Mf[a]);
+				wf[a][df] = vf ? pweight(f, I[a], Mf[a], Mc[a], P[a]) : 0.0;
+				fw[a][df] = ((f % Mf[a]) + Mf[a]) % Mf[a];
+#pragma unroll
+				for (int de = 0; de < 3; de++) {
+					const int g = f + de - 1;
+					const bool vg = P[a] || (g >= 0 && g < Mf[a]);
+					pg[a][df][de] = (vf && vg) ? pweight(g, J[a], Mf[a], Mc[a], P[a]) : 0.0;
 				}
-				if (pw != 0.0) sum = sum + 0.125 * Af[aidx3(F, e, fw[0], fw[1], fw[2])] * pw;
+			}
+		}
+#pragma unroll
+		for (int fz = 0; fz < 3; fz++)
+#pragma unroll
+		for (int fy = 0; fy < 3; fy++)
+#pragma unroll
+		for (int fx = 0; fx < 3; fx++) {
+			const double w = ((1.0 * wf[0][fx]) * wf[1][fy]) * wf[2][fz];
+			if (w == 0.0) continue;
+			const double *Arow = Af + aidx3(F, 0, fw[0][fx], fw[1][fy], fw[2][fz]);
+			double sum = 0.0;
+#pragma unroll
+			for (int e = 0; e < 27; e++) {
+				const double pw = ((1.0 * pg[0][fx][e % 3]) * pg[1][fy][(e / 3) % 3]) * pg[2][fz][e / 9];
+				if (pw != 0.0) sum = sum + 0.125 * Arow[(long long)e * F.px] * pw;
 			}
 			acc = acc + sum * w;
 		}
