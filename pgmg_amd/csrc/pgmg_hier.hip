@@ -242,6 +242,12 @@ void op_gs(pgmg_hier *h, int level, int mode, int nsweeps, bool from_zero = fals
 		return;
 	}
 	const int tmax = (L.v.m - 1) + 2 * (L.v.n - 1) + 4 * (L.v.o - 1);
+	if (exact && !mode && nsweeps > 1) {
+		// forward sweeps of one call, pipelined eight hyperplanes apart
+		ProfScope ps(h, KC_GS, level, L.points() * nsweeps);
+		launch_gs_planes<T>(h->stream, L.v, lA<T>(L), lx<T>(L), lr<T>(L), nsweeps);
+		return;
+	}
 	for (int s = 0; s < nsweeps; s++) {
 		for (int pass = 0; pass < (mode ? 2 : 1); pass++) {
 			if (exact) {

@@ -182,6 +182,9 @@ template <typename T> void launch_gs_pair(hipStream_t s, const LevelView &L, con
 template <typename T> void launch_halo_wrap(hipStream_t s, const LevelView &L, T *x);
 // Gauss-Seidel on the hyperplane i+2j+4k = t.
 template <typename T> void launch_gs_plane(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int t);
+// nsweeps forward lexicographic sweeps with the sweeps pipelined eight
+// hyperplanes apart (bit-identical to nsweeps separate sweeps).
+template <typename T> void launch_gs_planes(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int nsweeps);
 // nsweeps Gauss-Seidel sweeps of a small level by one workgroup, with x
 // staged in LDS when small_lds_bytes(L, sizeof(T)) <= kSmallLdsBytes.
 template <typename T> void launch_gs_small(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r,

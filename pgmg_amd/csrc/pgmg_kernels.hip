@@ -884,6 +884,41 @@ void launch_gs_plane(hipStream_t s, const LevelView &L, const T *A, T *x, const 
 	HIPCHECK_LAUNCH();
 }
 
+// Several forward sweeps in flight: launch tau updates hyperplane tau - 8q
+// of sweep q (blockIdx.z = q).  A stencil offset moves t by at most 7.  When
+// sweep q updates plane t, its own planes below t are done, and sweep q-1
+// has finished every plane up to t+7, whose values are the ones sweep q
+// must read there.  Sweep q-1 works on plane t+8 in the same launch, which
+// shares no stencil with plane t.  Every point therefore sees exactly the
+// neighbour values of the sweeps run one after another: the result is
+// bit-identical, with tmax+1+8(S-1) launches for S sweeps instead of
+// S(tmax+1).
+template <typename T>
+__global__ __launch_bounds__(128) void k_gs_planes(LevelView L, const T *__restrict__ A, T *__restrict__ x,
+						   const T *__restrict__ r, int tau, int tmax) {
+	const int t = tau - 8 * (int)blockIdx.z;
+	if (t < 0 || t > tmax) return;
+	const int k = (int)blockIdx.y;
+	const int sres = t - 4 * k;  // i + 2j
+	if (sres < 0) return;
+	const int jlo = max(0, (sres - (L.m - 1) + 1) >> 1);
+	const int jhi = min(L.n - 1, sres >> 1);
+	const int j = jlo + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (j > jhi) return;
+	gs_point<T>(L, A, x, r, sres - 2 * j, j, k);
+}
+
+template <typename T>
+void launch_gs_planes(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int nsweeps) {
+	const int tmax = (L.m - 1) + 2 * (L.n - 1) + 4 * (L.o - 1);
+	const int jspan = min(L.n, (L.m + 1) / 2 + 1);
+	const dim3 grid((jspan + 127) / 128, L.o, nsweeps), block(128, 1, 1);
+	for (int tau = 0; tau <= tmax + 8 * (nsweeps - 1); tau++) {
+		hipLaunchKernelGGL(k_gs_planes<T>, grid, block, 0, s, L, A, x, r, tau, tmax);
+		HIPCHECK_LAUNCH();
+	}
+}
+
 // A small level smoothed by one workgroup: all sweeps of one call in one
 // launch, ordered by workgroup barriers.  Exact mode walks the hyperplanes;
 // colour mode the eight classes.
@@ -2793,6 +2828,7 @@ void launch_b_add_x(hipStream_t s, const SlotDev *slots, WorkList wl) {
 	template void launch_gs_colour<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int); \
 	template void launch_gs_pair<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int, bool); \
 	template void launch_gs_plane<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int); \
+	template void launch_gs_planes<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int); \
 	template void launch_gs_small<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int, int, int); \
 	template int launch_sumsq<T>(hipStream_t, const LevelView &, const T *, double *); \
 	template void launch_restrict<T>(hipStream_t, const LevelView &, const LevelView &, const T *, T *, T *); \
