@@ -36,6 +36,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def conv_factor(hist, cycles):
+    """Geometric mean of the per-cycle residual reduction |res_k|/|res_k-1|
+    over cycles k >= 2 (hist holds sum res^2 before and after each cycle)."""
+    if cycles < 2:
+        return None
+    return round(float((hist[cycles] / hist[1]) ** (1.0 / (2 * (cycles - 1)))), 4)
+
+
 def solve_timed(P, h, steps, warmup, prof_level=0):
     for _ in range(warmup):
         h.clear_x(0)
@@ -389,8 +397,7 @@ def main():
         ce, hist_e = he.solve(TOL, 100)
         he.synchronize()
         te = time.perf_counter() - t0
-        fe = (hist_e[-1] / hist_e[1]) ** (1.0 / (2 * max(ce - 1, 1)))
-        exact = {"solve_s": round(te, 4), "cycles": ce, "conv_factor": round(fe, 4)}
+        exact = {"solve_s": round(te, 4), "cycles": ce, "conv_factor": conv_factor(hist_e, ce)}
         he.close()
         log("exact: %.3f s, %d cycles" % (te, ce))
 
@@ -430,9 +437,11 @@ def main():
                                "mixed = FP64 defect correction around FP32 V-cycles",
                    "grid": [N, N, N], "tol": TOL, "levels": 6},
         "mixed": {"solve_s": round(avg_mixed, 5), "cycles": c_mixed,
-                  "rel_residual": float((hist_m[-1] / bn) ** 0.5), "err_vs_fp64": err_mixed},
+                  "rel_residual": float((hist_m[-1] / bn) ** 0.5), "err_vs_fp64": err_mixed,
+                  "conv_factor": conv_factor(hist_m, c_mixed)},
         "fp64": {"solve_s": round(avg_fp64, 5), "cycles": c_fp64,
-                 "rel_residual": float((hist_d[-1] / bn) ** 0.5), "err_vs_fp64": err_fp64},
+                 "rel_residual": float((hist_d[-1] / bn) ** 0.5), "err_vs_fp64": err_fp64,
+                 "conv_factor": conv_factor(hist_d, c_fp64)},
         "speedup_mixed_vs_fp64": round(avg_fp64 / avg_mixed, 3),
         "dof_per_s_per_vcycle_mixed": round(pts * c_mixed / avg_mixed, 1),
         "exact_mode": exact,
