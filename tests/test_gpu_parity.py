@@ -314,3 +314,26 @@ def test_outside_array_operations(level):
     h.copy_rhs_to_x(level)
     assert np.array_equal(h.get_x(level), r)
     h.close()
+
+
+@pytest.mark.parametrize("dims,nsweeps", [((70, 45, 40), 5), ((41, 60, 33), 3), ((129, 20, 17), 2)])
+def test_exact_pipelined_sweeps_bitwise_vs_oracle(dims, nsweeps):
+    # levels above the one-workgroup size run several forward sweeps in
+    # flight, eight hyperplanes apart: the iterate must equal that many
+    # separate lexicographic sweeps
+    say("case", dims, "sweeps", nsweeps)
+    m, n, o = dims
+    A = O.gen_operator(3, 5, m, n, o)
+    b = O.gen_rhs(5, m, n, o)
+    x0 = O.test_field(m, n, o, 6)
+    h = P.Hierarchy(m, n, o, smoother=P.SMOOTH_EXACT, max_levels=1)
+    h.set_operator27(A)
+    h.compute_rats()
+    h.set_rhs(b)
+    h.set_x(x0)
+    h.gauss_seidel(0, 0, nsweeps)
+    x = x0
+    for _ in range(nsweeps):
+        x = O.gs(A, x, b, 0)
+    assert np.array_equal(h.get_x(0), x)
+    h.close()
