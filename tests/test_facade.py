@@ -92,15 +92,18 @@ def test_solve_v_cycles_guide():
 
 
 @pytest.mark.gpu
-def test_fmg_cycle_against_oracle_composition():
+@pytest.mark.parametrize("prd", [0, 5])
+def test_fmg_cycle_against_oracle_composition(prd):
     """fmg_cycle (multigrid.cc:109-134) through the facade against the same
-    schedule composed from the oracle's level operations."""
+    schedule composed from the oracle's level operations (prd: periodic
+    axes)."""
     m, n, o = 20, 17, 30
-    out = run(["mg_drivers", "fmg", 1, 9, m, n, o, "/dev/null"]).splitlines()
+    fam = "1:%d" % prd if prd else 1
+    out = run(["mg_drivers", "fmg", fam, 9, m, n, o, "/dev/null"]).splitlines()
     l2 = float([l for l in out if l.startswith("fmg ")][0].split()[1])
-    A = O.gen_operator(1, 9, m, n, o)
+    A = O.gen_operator(1, 9, m, n, o, prd)
     b = O.gen_rhs(9, m, n, o)
-    mg = O.Multigrid(A)
+    mg = O.Multigrid(A, prd=prd)
     mg.set_rhs(b)
     L = mg.nlevels
     for i in range(L - 1):
