@@ -47,6 +47,10 @@ struct Level {
 	LevelView v;
 	double *A = nullptr, *x = nullptr, *r = nullptr;
 	float *A32 = nullptr, *x32 = nullptr, *r32 = nullptr;
+	// plane-skewed FP64 copies for the exact smoother (levels that do not
+	// take the one-workgroup kernel), made by pgmg_compute_rats
+	SkewView sv;
+	double *As = nullptr, *xs = nullptr, *rs = nullptr;
 	long long points() const { return (long long)v.m * v.n * v.o; }
 };
 
@@ -242,9 +246,19 @@ void op_gs(pgmg_hier *h, int level, int mode, int nsweeps, bool from_zero = fals
 		return;
 	}
 	const int tmax = (L.v.m - 1) + 2 * (L.v.n - 1) + 4 * (L.v.o - 1);
-	if (exact && !mode && nsweeps > 1) {
+	if (exact && !mode) {
 		// forward sweeps of one call, pipelined eight hyperplanes apart
 		ProfScope ps(h, KC_GS, level, L.points() * nsweeps);
+		if constexpr (sizeof(T) == 8) {
+			if (L.As) {
+				// on the plane-skewed copies: x and r in, x out
+				launch_skew_vec(h->stream, L.v, L.sv, L.x, L.xs, true);
+				launch_skew_vec(h->stream, L.v, L.sv, L.r, L.rs, true);
+				launch_gs_planes_skew(h->stream, L.sv, L.As, L.xs, L.rs, nsweeps);
+				launch_skew_vec(h->stream, L.v, L.sv, L.xs, L.x, false);
+				return;
+			}
+		}
 		launch_gs_planes<T>(h->stream, L.v, lA<T>(L), lx<T>(L), lr<T>(L), nsweeps);
 		return;
 	}
@@ -518,6 +532,7 @@ int pgmg_hier_destroy(pgmg_hier *h) {
 		Level &L = h->lev[l];
 		hipFree(L.A); hipFree(L.x); hipFree(L.r);
 		hipFree(L.A32); hipFree(L.x32); hipFree(L.r32);
+		hipFree(L.As); hipFree(L.xs); hipFree(L.rs);
 	}
 	hipFree(h->res64); hipFree(h->res32); hipFree(h->partial); hipFree(h->dscalar); hipFree(h->aux64);
 	if (h->hscalar) hipHostFree(h->hscalar);
@@ -604,6 +619,33 @@ int pgmg_compute_rats(pgmg_hier *h) {
 	}
 	if (h->opts.precision == PGMG_PREC_MIXED) {
 		for (int l = 0; l < h->nl; l++) launch_round_f32(h->stream, h->lev[l].A, h->lev[l].A32, 27 * h->lev[l].v.npad);
+	}
+	if (h->opts.smoother == PGMG_SMOOTH_EXACT) {
+		// plane-skewed copies for the FP64 exact sweeps of large non-periodic
+		// levels (about three times a level's size each: skipped, and the
+		// level layout swept instead, when they would not fit)
+		for (int l = 0; l < h->nl; l++) {
+			Level &L = h->lev[l];
+			if (L.v.prd || gs_use_small<double>(h, L, 0)) continue;
+			if (!L.As) {
+				const SkewView sv = make_skew_view(L.v);
+				const size_t n = (size_t)sv.size;
+				size_t fr = 0, tot = 0;
+				if (hipMemGetInfo(&fr, &tot) != hipSuccess || 29 * n * 8 > fr / 2) continue;
+				if (hipMalloc(&L.As, 27 * n * 8) != hipSuccess || hipMalloc(&L.xs, n * 8) != hipSuccess ||
+				    hipMalloc(&L.rs, n * 8) != hipSuccess) {
+					hipFree(L.As); hipFree(L.xs); hipFree(L.rs);
+					L.As = L.xs = L.rs = nullptr;
+					(void)hipGetLastError();
+					continue;
+				}
+				L.sv = sv;
+				launch_fill<double>(h->stream, L.As, 27 * (long long)n, 0.0);
+				launch_fill<double>(h->stream, L.xs, (long long)n, 0.0);
+				launch_fill<double>(h->stream, L.rs, (long long)n, 0.0);
+			}
+			launch_skew_pack_op(h->stream, L.v, L.sv, L.A, L.As);
+		}
 	}
 	HIP_TRY(hipStreamSynchronize(h->stream));
 	h->ops_ready = true;

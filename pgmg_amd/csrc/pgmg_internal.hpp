@@ -44,6 +44,30 @@ inline LevelView make_level_view(int m, int n, int o) {
 	return v;
 }
 
+// Plane-skewed copy of a level for the exact smoother: point (i,j,k) at
+// base + j + jp*(i + 2j) + pk*k, so the points of one hyperplane
+// i + 2j + 4k = t with the same k lie side by side in j, and so do each of
+// their neighbours.  Positions that are not grid points hold zero.
+struct SkewView {
+	int m, n, o;
+	int jp;            // j pitch: n + 2 (j = -1 .. n), rounded up to 16
+	long long pk;      // k pitch: jp * (m + 2n + 4) (u = i + 2j = -3 .. m + 2n)
+	long long base;    // offset of point (0,0,0)
+	long long size;    // elements per array, halo included
+};
+inline SkewView make_skew_view(const LevelView &L) {
+	SkewView v;
+	v.m = L.m; v.n = L.n; v.o = L.o;
+	v.jp = ((L.n + 2 + 15) / 16) * 16;
+	v.pk = (long long)v.jp * (L.m + 2 * L.n + 4);
+	v.base = 1 + 3LL * v.jp + v.pk;
+	v.size = v.pk * (L.o + 2);
+	return v;
+}
+__host__ __device__ inline long long sidx3(const SkewView &S, int i, int j, int k) {
+	return S.base + j + (long long)S.jp * (i + 2 * j) + S.pk * k;
+}
+
 // Point (i,j,k) of a per-point array.
 __host__ __device__ inline long long idx3(const LevelView &L, int i, int j, int k) {
 	return L.base + i + (long long)L.px * j + L.pxy * k;
@@ -182,6 +206,14 @@ template <typename T> void launch_gs_pair(hipStream_t s, const LevelView &L, con
 template <typename T> void launch_halo_wrap(hipStream_t s, const LevelView &L, T *x);
 // Gauss-Seidel on the hyperplane i+2j+4k = t.
 template <typename T> void launch_gs_plane(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int t);
+// Plane-skewed copies for the exact smoother: the 27 coefficient planes
+// (As[e*S.size + ...]) and one vector, both ways.
+void launch_skew_pack_op(hipStream_t s, const LevelView &L, const SkewView &S, const double *A, double *As);
+void launch_skew_vec(hipStream_t s, const LevelView &L, const SkewView &S, const double *v, double *vs, bool to_skew);
+// nsweeps forward lexicographic sweeps on skewed copies, pipelined eight
+// hyperplanes apart (bit-identical to the sweeps on the level layout).
+void launch_gs_planes_skew(hipStream_t s, const SkewView &S, const double *As, double *xs, const double *rs,
+			   int nsweeps);
 // nsweeps forward lexicographic sweeps with the sweeps pipelined eight
 // hyperplanes apart (bit-identical to nsweeps separate sweeps).
 template <typename T> void launch_gs_planes(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int nsweeps);

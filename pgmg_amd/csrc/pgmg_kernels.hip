@@ -919,6 +919,77 @@ void launch_gs_planes(hipStream_t s, const LevelView &L, const T *A, T *x, const
 	}
 }
 
+// The same pipelined sweeps on the plane-skewed copies (SkewView): the
+// threads of a block take consecutive j of one (t, k), whose points, their
+// coefficients and each of their neighbours are consecutive in memory, so
+// every load of a hyperplane is a coalesced row segment.  Per-point
+// arithmetic and order are gs_point's.
+__global__ __launch_bounds__(128) void k_gs_planes_skew(SkewView S, const double *__restrict__ As,
+							double *__restrict__ xs, const double *__restrict__ rs, int tau,
+							int tmax) {
+	const int t = tau - 8 * (int)blockIdx.z;
+	if (t < 0 || t > tmax) return;
+	const int k = (int)blockIdx.y;
+	const int sres = t - 4 * k;  // i + 2j
+	if (sres < 0) return;
+	const int jlo = max(0, (sres - (S.m - 1) + 1) >> 1);
+	const int jhi = min(S.n - 1, sres >> 1);
+	const int j = jlo + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+	if (j > jhi) return;
+	const long long p = S.base + j + (long long)S.jp * sres + S.pk * k;
+	double ax = 0.0;
+#pragma unroll
+	for (int e = 0; e < 27; e++) {
+		if (e == 13) continue;
+		const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
+		ax = ax + As[e * S.size + p] * xs[p + dj + (long long)S.jp * (di + 2 * dj) + S.pk * dk];
+	}
+	xs[p] = (rs[p] - ax) * (1.0 / As[13 * S.size + p]);
+}
+
+void launch_gs_planes_skew(hipStream_t s, const SkewView &S, const double *As, double *xs, const double *rs,
+			   int nsweeps) {
+	const int tmax = (S.m - 1) + 2 * (S.n - 1) + 4 * (S.o - 1);
+	const int jspan = min(S.n, (S.m + 1) / 2 + 1);
+	const dim3 grid((jspan + 127) / 128, S.o, nsweeps), block(128, 1, 1);
+	for (int tau = 0; tau <= tmax + 8 * (nsweeps - 1); tau++) {
+		hipLaunchKernelGGL(k_gs_planes_skew, grid, block, 0, s, S, As, xs, rs, tau, tmax);
+		HIPCHECK_LAUNCH();
+	}
+}
+
+__global__ __launch_bounds__(256) void k_skew_pack_op(LevelView L, SkewView S, const double *__restrict__ A,
+						      double *__restrict__ As) {
+	const long long N = (long long)L.m * L.n * L.o;
+	for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < N * 27; q += (long long)gridDim.x * 256) {
+		const int e = (int)(q / N);
+		const long long p = q - (long long)e * N;
+		const int i = (int)(p % L.m), j = (int)((p / L.m) % L.n), k = (int)(p / ((long long)L.m * L.n));
+		As[e * S.size + sidx3(S, i, j, k)] = A[aidx3(L, e, i, j, k)];
+	}
+}
+
+void launch_skew_pack_op(hipStream_t s, const LevelView &L, const SkewView &S, const double *A, double *As) {
+	hipLaunchKernelGGL(k_skew_pack_op, dim3(4096), dim3(256), 0, s, L, S, A, As);
+	HIPCHECK_LAUNCH();
+}
+
+__global__ __launch_bounds__(256) void k_skew_vec(LevelView L, SkewView S, const double *__restrict__ v,
+						  double *__restrict__ vs, int to_skew) {
+	const long long N = (long long)L.m * L.n * L.o;
+	for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < N; p += (long long)gridDim.x * 256) {
+		const int i = (int)(p % L.m), j = (int)((p / L.m) % L.n), k = (int)(p / ((long long)L.m * L.n));
+		if (to_skew) vs[sidx3(S, i, j, k)] = v[idx3(L, i, j, k)];
+		else vs[idx3(L, i, j, k)] = v[sidx3(S, i, j, k)];
+	}
+}
+
+// to_skew: vs (skewed) = v (level layout); else vs (level layout) = v (skewed).
+void launch_skew_vec(hipStream_t s, const LevelView &L, const SkewView &S, const double *v, double *vs, bool to_skew) {
+	hipLaunchKernelGGL(k_skew_vec, dim3(2048), dim3(256), 0, s, L, S, v, vs, to_skew ? 1 : 0);
+	HIPCHECK_LAUNCH();
+}
+
 // A small level smoothed by one workgroup: all sweeps of one call in one
 // launch, ordered by workgroup barriers.  Exact mode walks the hyperplanes;
 // colour mode the eight classes.
