@@ -223,6 +223,20 @@ void op_gs(pgmg_hier *h, int level, int mode, int nsweeps, bool from_zero = fals
 		// which takes it as zero: with a periodic x axis such rows use the
 		// per-class kernel.
 		const bool pair_ok = !(L.v.prd & 1) || !row_lanes(L.v.m, (int)sizeof(T)).ext;
+		if constexpr (sizeof(T) == 8) {
+			if (exact && !mode && L.As) {
+				// plane-skewed copies, ghosts refreshed on them before each sweep
+				ProfScope ps(h, KC_GS, level, L.points() * nsweeps);
+				launch_skew_vec(h->stream, L.v, L.sv, L.x, L.xs, true);
+				launch_skew_vec(h->stream, L.v, L.sv, L.r, L.rs, true);
+				for (int s = 0; s < nsweeps; s++) {
+					launch_skew_halo_wrap(h->stream, L.v, L.sv, L.xs);
+					launch_gs_planes_skew(h->stream, L.sv, L.As, L.xs, L.rs, 1);
+				}
+				launch_skew_vec(h->stream, L.v, L.sv, L.xs, L.x, false);
+				return;
+			}
+		}
 		for (int s = 0; s < nsweeps; s++)
 			for (int pass = 0; pass < (mode ? 2 : 1); pass++) {
 				ProfScope ps(h, KC_GS, level, L.points());
@@ -621,12 +635,12 @@ int pgmg_compute_rats(pgmg_hier *h) {
 		for (int l = 0; l < h->nl; l++) launch_round_f32(h->stream, h->lev[l].A, h->lev[l].A32, 27 * h->lev[l].v.npad);
 	}
 	if (h->opts.smoother == PGMG_SMOOTH_EXACT) {
-		// plane-skewed copies for the FP64 exact sweeps of large non-periodic
-		// levels (about three times a level's size each: skipped, and the
-		// level layout swept instead, when they would not fit)
+		// plane-skewed copies for the FP64 exact sweeps of large levels
+		// (about three times a level's size each: skipped, and the level
+		// layout swept instead, when they would not fit)
 		for (int l = 0; l < h->nl; l++) {
 			Level &L = h->lev[l];
-			if (L.v.prd || gs_use_small<double>(h, L, 0)) continue;
+			if (gs_use_small<double>(h, L, 0)) continue;
 			if (!L.As) {
 				const SkewView sv = make_skew_view(L.v);
 				const size_t n = (size_t)sv.size;

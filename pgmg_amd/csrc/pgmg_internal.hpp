@@ -210,6 +210,8 @@ template <typename T> void launch_gs_plane(hipStream_t s, const LevelView &L, co
 // (As[e*S.size + ...]) and one vector, both ways.
 void launch_skew_pack_op(hipStream_t s, const LevelView &L, const SkewView &S, const double *A, double *As);
 void launch_skew_vec(hipStream_t s, const LevelView &L, const SkewView &S, const double *v, double *vs, bool to_skew);
+// Ghost copies of the periodic axes of L on a skewed vector.
+void launch_skew_halo_wrap(hipStream_t s, const LevelView &L, const SkewView &S, double *xs);
 // nsweeps forward lexicographic sweeps on skewed copies, pipelined eight
 // hyperplanes apart (bit-identical to the sweeps on the level layout).
 void launch_gs_planes_skew(hipStream_t s, const SkewView &S, const double *As, double *xs, const double *rs,

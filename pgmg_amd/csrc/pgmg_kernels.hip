@@ -984,6 +984,21 @@ __global__ __launch_bounds__(256) void k_skew_vec(LevelView L, SkewView S, const
 	}
 }
 
+// Ghost copies of a periodic level's halo on its skewed copy (as
+// k_halo_wrap on the level layout).
+__global__ __launch_bounds__(256) void k_skew_halo_wrap(LevelView L, SkewView S, double *__restrict__ xs) {
+	const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+	int i, j, k, si, sj, sk;
+	if (q >= halo_points(L) || !halo_src(L, q, i, j, k, si, sj, sk)) return;
+	xs[sidx3(S, i, j, k)] = xs[sidx3(S, si, sj, sk)];
+}
+
+void launch_skew_halo_wrap(hipStream_t s, const LevelView &L, const SkewView &S, double *xs) {
+	if (!L.prd) return;
+	hipLaunchKernelGGL(k_skew_halo_wrap, dim3((unsigned)((halo_points(L) + 255) / 256)), dim3(256), 0, s, L, S, xs);
+	HIPCHECK_LAUNCH();
+}
+
 // to_skew: vs (skewed) = v (level layout); else vs (level layout) = v (skewed).
 void launch_skew_vec(hipStream_t s, const LevelView &L, const SkewView &S, const double *v, double *vs, bool to_skew) {
 	hipLaunchKernelGGL(k_skew_vec, dim3(2048), dim3(256), 0, s, L, S, v, vs, to_skew ? 1 : 0);

@@ -262,3 +262,19 @@ def test_periodic_random_shapes(dims, prd, fam):
     c_g, _ = h.solve(1e-30, 2, check=False)
     assert np.array_equal(h.get_x(0), mg.get_x(0))
     h.close()
+
+
+@pytest.mark.parametrize("dims,prd", [((48, 40, 30), 5), ((36, 50, 33), 7)])
+def test_periodic_exact_large_level(dims, prd):
+    # a level above the one-workgroup size: exact sweeps on the plane-skewed
+    # copies, with the ghosts refreshed on them before each sweep
+    say("case", dims, "prd", prd)
+    m, n, o = dims
+    h, A, b = setup((2, 13, m, n, o, prd), smoother=P.SMOOTH_EXACT)
+    mg = O.Multigrid(A, prd=prd)
+    mg.set_rhs(b)
+    for c in range(2):
+        h.v_cycle()
+        mg.v_cycle()
+    assert np.array_equal(h.get_x(0), mg.get_x(0))
+    h.close()
