@@ -83,10 +83,12 @@ int pgmg_version(void);
 /* Default options: FP64, COLOUR8, GS24, V(2,2), graphs on. */
 void pgmg_default_opts(pgmg_opts *opts);
 /* Levels of the hierarchy for a grid, without allocating: multigrid.cc:17-85
- * (coarse size (m+2)>>1, region.cc:14; stop at m*n*o <= 1024). dims_out holds
- * 3*16 ints. */
+ * (coarse size (m+2)>>1, or (m+1)>>1 on a periodic axis, region.cc:13-14;
+ * stop at m*n*o <= 1024). dims_out holds 3*16 ints. */
 int pgmg_hierarchy_shape(const pgmg_dims *dims, int *nlevels, int *dims_out);
-/* Device bytes a hierarchy with these options would hold. */
+/* Device bytes a hierarchy with these options would hold (with the exact
+ * smoother, including the plane-skewed copies that pgmg_compute_rats makes
+ * of its large levels when they fit). */
 int pgmg_hier_footprint(const pgmg_dims *dims, const pgmg_opts *opts, uint64_t *bytes);
 
 /* multigrid<p_class>::multigrid(pr, gm, com), multigrid.hh:282-295, and

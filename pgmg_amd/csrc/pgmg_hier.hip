@@ -458,6 +458,15 @@ int pgmg_hier_footprint(const pgmg_dims *d, const pgmg_opts *opts, uint64_t *byt
 	top.prd = dims_prd(d);
 	b += (uint64_t)top.npad * (8 + (mixed ? 4 : 0));
 	b += (uint64_t)residual_partial_count(top) * 8;
+	if (opts && opts->smoother == PGMG_SMOOTH_EXACT) {
+		// plane-skewed copies of the levels the one-workgroup kernel does not take
+		for (int l = 0; l < nl; l++) {
+			const LevelView v = make_level_view(dims[3 * l], dims[3 * l + 1], dims[3 * l + 2]);
+			const long long pts = (long long)v.m * v.n * v.o;
+			if (small_lds_bytes(v, 8) <= kSmallLdsBytes || pts <= kSmallExact) continue;
+			b += (uint64_t)make_skew_view(v).size * 29 * 8;
+		}
+	}
 	if (opts && opts->bottom == PGMG_BOTTOM_LU) {
 		LevelView bot = make_level_view(dims[3 * (nl - 1)], dims[3 * (nl - 1) + 1], dims[3 * (nl - 1) + 2]);
 		long long N = (long long)bot.m * bot.n * bot.o;
