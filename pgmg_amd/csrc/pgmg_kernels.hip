@@ -924,7 +924,7 @@ void launch_gs_planes(hipStream_t s, const LevelView &L, const T *A, T *x, const
 // coefficients and each of their neighbours are consecutive in memory, so
 // every load of a hyperplane is a coalesced row segment.  Per-point
 // arithmetic and order are gs_point's.
-__global__ __launch_bounds__(128) void k_gs_planes_skew(SkewView S, const double *__restrict__ As,
+__global__ __launch_bounds__(256) void k_gs_planes_skew(SkewView S, const double *__restrict__ As,
 							double *__restrict__ xs, const double *__restrict__ rs, int tau,
 							int tmax) {
 	const int t = tau - 8 * (int)blockIdx.z;
@@ -951,7 +951,9 @@ void launch_gs_planes_skew(hipStream_t s, const SkewView &S, const double *As, d
 			   int nsweeps) {
 	const int tmax = (S.m - 1) + 2 * (S.n - 1) + 4 * (S.o - 1);
 	const int jspan = min(S.n, (S.m + 1) / 2 + 1);
-	const dim3 grid((jspan + 127) / 128, S.o, nsweeps), block(128, 1, 1);
+	// one block per (k, sweep) row segment where it fits
+	const int bs = jspan <= 64 ? 64 : (jspan <= 128 ? 128 : 256);
+	const dim3 grid((jspan + bs - 1) / bs, S.o, nsweeps), block(bs, 1, 1);
 	for (int tau = 0; tau <= tmax + 8 * (nsweeps - 1); tau++) {
 		hipLaunchKernelGGL(k_gs_planes_skew, grid, block, 0, s, S, As, xs, rs, tau, tmax);
 		HIPCHECK_LAUNCH();
