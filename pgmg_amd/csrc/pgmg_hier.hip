@@ -224,7 +224,7 @@ void op_gs(pgmg_hier *h, int level, int mode, int nsweeps, bool from_zero = fals
 		// per-class kernel.
 		const bool pair_ok = !(L.v.prd & 1) || !row_lanes(L.v.m, (int)sizeof(T)).ext;
 		if constexpr (sizeof(T) == 8) {
-			if (exact && !mode && L.As) {
+			if (exact && !mode && L.As && h->ops_ready) {
 				// plane-skewed copies, ghosts refreshed on them before each sweep
 				ProfScope ps(h, KC_GS, level, L.points() * nsweeps);
 				launch_skew_vec(h->stream, L.v, L.sv, L.x, L.xs, true);
@@ -264,7 +264,9 @@ void op_gs(pgmg_hier *h, int level, int mode, int nsweeps, bool from_zero = fals
 		// forward sweeps of one call, pipelined eight hyperplanes apart
 		ProfScope ps(h, KC_GS, level, L.points() * nsweeps);
 		if constexpr (sizeof(T) == 8) {
-			if (L.As) {
+			// the copies match the operator from pgmg_compute_rats until an
+			// operator is set again (ops_ready false)
+			if (L.As && h->ops_ready) {
 				// on the plane-skewed copies: x and r in, x out
 				launch_skew_vec(h->stream, L.v, L.sv, L.x, L.xs, true);
 				launch_skew_vec(h->stream, L.v, L.sv, L.r, L.rs, true);

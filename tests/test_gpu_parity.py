@@ -337,3 +337,23 @@ def test_exact_pipelined_sweeps_bitwise_vs_oracle(dims, nsweeps):
         x = O.gs(A, x, b, 0)
     assert np.array_equal(h.get_x(0), x)
     h.close()
+
+
+def test_exact_sweep_after_new_operator_without_compute_rats():
+    # a sweep of level 0 needs only its own operator: after the operator is
+    # set again, without compute_rats, the sweep must use the new one
+    m, n, o = 70, 45, 40
+    A1 = O.gen_operator(1, 3, m, n, o)
+    A2 = O.gen_operator(3, 4, m, n, o)
+    b = O.gen_rhs(5, m, n, o)
+    h = P.Hierarchy(m, n, o, smoother=P.SMOOTH_EXACT)
+    h.set_operator27(A1)
+    h.compute_rats()
+    h.set_operator27(A2)
+    h.set_rhs(b)
+    h.gauss_seidel(0, 0, 2)
+    x = np.zeros((o, n, m))
+    for _ in range(2):
+        x = O.gs(A2, x, b, 0)
+    assert np.array_equal(h.get_x(0), x)
+    h.close()
