@@ -51,6 +51,7 @@ struct Level {
 	// take the one-workgroup kernel), made by pgmg_compute_rats
 	SkewView sv;
 	double *As = nullptr, *xs = nullptr, *rs = nullptr;
+	float *As32 = nullptr, *xs32 = nullptr, *rs32 = nullptr;  // mixed precision
 	long long points() const { return (long long)v.m * v.n * v.o; }
 };
 
@@ -176,6 +177,16 @@ template <> double *lr<double>(Level &L) { return L.r; }
 template <> float *lA<float>(Level &L) { return L.A32; }
 template <> float *lx<float>(Level &L) { return L.x32; }
 template <> float *lr<float>(Level &L) { return L.r32; }
+// plane-skewed copies of a level (null where none were made)
+template <typename T> T *sA(Level &L);
+template <typename T> T *sx(Level &L);
+template <typename T> T *sr(Level &L);
+template <> double *sA<double>(Level &L) { return L.As; }
+template <> double *sx<double>(Level &L) { return L.xs; }
+template <> double *sr<double>(Level &L) { return L.rs; }
+template <> float *sA<float>(Level &L) { return L.As32; }
+template <> float *sx<float>(Level &L) { return L.xs32; }
+template <> float *sr<float>(Level &L) { return L.rs32; }
 template <typename T> T *res_scratch(pgmg_hier *h);
 template <> double *res_scratch<double>(pgmg_hier *h) { return h->res64; }
 template <> float *res_scratch<float>(pgmg_hier *h) { return h->res32; }
@@ -223,19 +234,17 @@ void op_gs(pgmg_hier *h, int level, int mode, int nsweeps, bool from_zero = fals
 		// which takes it as zero: with a periodic x axis such rows use the
 		// per-class kernel.
 		const bool pair_ok = !(L.v.prd & 1) || !row_lanes(L.v.m, (int)sizeof(T)).ext;
-		if constexpr (sizeof(T) == 8) {
-			if (exact && !mode && L.As && h->ops_ready) {
-				// plane-skewed copies, ghosts refreshed on them before each sweep
-				ProfScope ps(h, KC_GS, level, L.points() * nsweeps);
-				launch_skew_vec(h->stream, L.v, L.sv, L.x, L.xs, true);
-				launch_skew_vec(h->stream, L.v, L.sv, L.r, L.rs, true);
-				for (int s = 0; s < nsweeps; s++) {
-					launch_skew_halo_wrap(h->stream, L.v, L.sv, L.xs);
-					launch_gs_planes_skew(h->stream, L.sv, L.As, L.xs, L.rs, 1);
-				}
-				launch_skew_vec(h->stream, L.v, L.sv, L.xs, L.x, false);
-				return;
+		if (exact && !mode && sA<T>(L) && h->ops_ready) {
+			// plane-skewed copies, ghosts refreshed on them before each sweep
+			ProfScope ps(h, KC_GS, level, L.points() * nsweeps);
+			launch_skew_vec<T>(h->stream, L.v, L.sv, lx<T>(L), sx<T>(L), true);
+			launch_skew_vec<T>(h->stream, L.v, L.sv, lr<T>(L), sr<T>(L), true);
+			for (int s = 0; s < nsweeps; s++) {
+				launch_skew_halo_wrap<T>(h->stream, L.v, L.sv, sx<T>(L));
+				launch_gs_planes_skew<T>(h->stream, L.sv, sA<T>(L), sx<T>(L), sr<T>(L), 1);
 			}
+			launch_skew_vec<T>(h->stream, L.v, L.sv, sx<T>(L), lx<T>(L), false);
+			return;
 		}
 		for (int s = 0; s < nsweeps; s++)
 			for (int pass = 0; pass < (mode ? 2 : 1); pass++) {
@@ -263,17 +272,15 @@ void op_gs(pgmg_hier *h, int level, int mode, int nsweeps, bool from_zero = fals
 	if (exact && !mode) {
 		// forward sweeps of one call, pipelined eight hyperplanes apart
 		ProfScope ps(h, KC_GS, level, L.points() * nsweeps);
-		if constexpr (sizeof(T) == 8) {
-			// the copies match the operator from pgmg_compute_rats until an
-			// operator is set again (ops_ready false)
-			if (L.As && h->ops_ready) {
-				// on the plane-skewed copies: x and r in, x out
-				launch_skew_vec(h->stream, L.v, L.sv, L.x, L.xs, true);
-				launch_skew_vec(h->stream, L.v, L.sv, L.r, L.rs, true);
-				launch_gs_planes_skew(h->stream, L.sv, L.As, L.xs, L.rs, nsweeps);
-				launch_skew_vec(h->stream, L.v, L.sv, L.xs, L.x, false);
-				return;
-			}
+		// the copies match the operator from pgmg_compute_rats until an
+		// operator is set again (ops_ready false)
+		if (sA<T>(L) && h->ops_ready) {
+			// on the plane-skewed copies: x and r in, x out
+			launch_skew_vec<T>(h->stream, L.v, L.sv, lx<T>(L), sx<T>(L), true);
+			launch_skew_vec<T>(h->stream, L.v, L.sv, lr<T>(L), sr<T>(L), true);
+			launch_gs_planes_skew<T>(h->stream, L.sv, sA<T>(L), sx<T>(L), sr<T>(L), nsweeps);
+			launch_skew_vec<T>(h->stream, L.v, L.sv, sx<T>(L), lx<T>(L), false);
+			return;
 		}
 		launch_gs_planes<T>(h->stream, L.v, lA<T>(L), lx<T>(L), lr<T>(L), nsweeps);
 		return;
@@ -466,7 +473,7 @@ int pgmg_hier_footprint(const pgmg_dims *d, const pgmg_opts *opts, uint64_t *byt
 			const LevelView v = make_level_view(dims[3 * l], dims[3 * l + 1], dims[3 * l + 2]);
 			const long long pts = (long long)v.m * v.n * v.o;
 			if (small_lds_bytes(v, 8) <= kSmallLdsBytes || pts <= kSmallExact) continue;
-			b += (uint64_t)make_skew_view(v).size * 29 * 8;
+			b += (uint64_t)make_skew_view(v).size * 29 * ((opts->precision == PGMG_PREC_MIXED) ? 12 : 8);
 		}
 	}
 	if (opts && opts->bottom == PGMG_BOTTOM_LU) {
@@ -558,6 +565,7 @@ int pgmg_hier_destroy(pgmg_hier *h) {
 		hipFree(L.A); hipFree(L.x); hipFree(L.r);
 		hipFree(L.A32); hipFree(L.x32); hipFree(L.r32);
 		hipFree(L.As); hipFree(L.xs); hipFree(L.rs);
+		hipFree(L.As32); hipFree(L.xs32); hipFree(L.rs32);
 	}
 	hipFree(h->res64); hipFree(h->res32); hipFree(h->partial); hipFree(h->dscalar); hipFree(h->aux64);
 	if (h->hscalar) hipHostFree(h->hscalar);
@@ -669,7 +677,24 @@ int pgmg_compute_rats(pgmg_hier *h) {
 				launch_fill<double>(h->stream, L.xs, (long long)n, 0.0);
 				launch_fill<double>(h->stream, L.rs, (long long)n, 0.0);
 			}
-			launch_skew_pack_op(h->stream, L.v, L.sv, L.A, L.As);
+			launch_skew_pack_op<double>(h->stream, L.v, L.sv, L.A, L.As);
+			if (h->opts.precision != PGMG_PREC_MIXED) continue;
+			if (!L.As32) {
+				const size_t n = (size_t)L.sv.size;
+				size_t fr = 0, tot = 0;
+				if (hipMemGetInfo(&fr, &tot) != hipSuccess || 29 * n * 4 > fr / 2) continue;
+				if (hipMalloc(&L.As32, 27 * n * 4) != hipSuccess || hipMalloc(&L.xs32, n * 4) != hipSuccess ||
+				    hipMalloc(&L.rs32, n * 4) != hipSuccess) {
+					hipFree(L.As32); hipFree(L.xs32); hipFree(L.rs32);
+					L.As32 = L.xs32 = L.rs32 = nullptr;
+					(void)hipGetLastError();
+					continue;
+				}
+				launch_fill<float>(h->stream, L.As32, 27 * (long long)n, 0.0f);
+				launch_fill<float>(h->stream, L.xs32, (long long)n, 0.0f);
+				launch_fill<float>(h->stream, L.rs32, (long long)n, 0.0f);
+			}
+			launch_skew_pack_op<float>(h->stream, L.v, L.sv, L.A32, L.As32);
 		}
 	}
 	HIP_TRY(hipStreamSynchronize(h->stream));

@@ -208,14 +208,15 @@ template <typename T> void launch_halo_wrap(hipStream_t s, const LevelView &L, T
 template <typename T> void launch_gs_plane(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int t);
 // Plane-skewed copies for the exact smoother: the 27 coefficient planes
 // (As[e*S.size + ...]) and one vector, both ways.
-void launch_skew_pack_op(hipStream_t s, const LevelView &L, const SkewView &S, const double *A, double *As);
-void launch_skew_vec(hipStream_t s, const LevelView &L, const SkewView &S, const double *v, double *vs, bool to_skew);
+template <typename T> void launch_skew_pack_op(hipStream_t s, const LevelView &L, const SkewView &S, const T *A, T *As);
+template <typename T> void launch_skew_vec(hipStream_t s, const LevelView &L, const SkewView &S, const T *v, T *vs,
+					   bool to_skew);
 // Ghost copies of the periodic axes of L on a skewed vector.
-void launch_skew_halo_wrap(hipStream_t s, const LevelView &L, const SkewView &S, double *xs);
+template <typename T> void launch_skew_halo_wrap(hipStream_t s, const LevelView &L, const SkewView &S, T *xs);
 // nsweeps forward lexicographic sweeps on skewed copies, pipelined eight
 // hyperplanes apart (bit-identical to the sweeps on the level layout).
-void launch_gs_planes_skew(hipStream_t s, const SkewView &S, const double *As, double *xs, const double *rs,
-			   int nsweeps);
+template <typename T> void launch_gs_planes_skew(hipStream_t s, const SkewView &S, const T *As, T *xs, const T *rs,
+						 int nsweeps);
 // nsweeps forward lexicographic sweeps with the sweeps pipelined eight
 // hyperplanes apart (bit-identical to nsweeps separate sweeps).
 template <typename T> void launch_gs_planes(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int nsweeps);

@@ -924,9 +924,9 @@ void launch_gs_planes(hipStream_t s, const LevelView &L, const T *A, T *x, const
 // coefficients and each of their neighbours are consecutive in memory, so
 // every load of a hyperplane is a coalesced row segment.  Per-point
 // arithmetic and order are gs_point's.
-__global__ __launch_bounds__(256) void k_gs_planes_skew(SkewView S, const double *__restrict__ As,
-							double *__restrict__ xs, const double *__restrict__ rs, int tau,
-							int tmax) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_gs_planes_skew(SkewView S, const T *__restrict__ As, T *__restrict__ xs,
+							const T *__restrict__ rs, int tau, int tmax) {
 	const int t = tau - 8 * (int)blockIdx.z;
 	if (t < 0 || t > tmax) return;
 	const int k = (int)blockIdx.y;
@@ -937,31 +937,32 @@ __global__ __launch_bounds__(256) void k_gs_planes_skew(SkewView S, const double
 	const int j = jlo + (int)(blockIdx.x * blockDim.x + threadIdx.x);
 	if (j > jhi) return;
 	const long long p = S.base + j + (long long)S.jp * sres + S.pk * k;
-	double ax = 0.0;
+	T ax = (T)0;
 #pragma unroll
 	for (int e = 0; e < 27; e++) {
 		if (e == 13) continue;
 		const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
 		ax = ax + As[e * S.size + p] * xs[p + dj + (long long)S.jp * (di + 2 * dj) + S.pk * dk];
 	}
-	xs[p] = (rs[p] - ax) * (1.0 / As[13 * S.size + p]);
+	xs[p] = (rs[p] - ax) * ((T)1 / As[13 * S.size + p]);
 }
 
-void launch_gs_planes_skew(hipStream_t s, const SkewView &S, const double *As, double *xs, const double *rs,
-			   int nsweeps) {
+template <typename T>
+void launch_gs_planes_skew(hipStream_t s, const SkewView &S, const T *As, T *xs, const T *rs, int nsweeps) {
 	const int tmax = (S.m - 1) + 2 * (S.n - 1) + 4 * (S.o - 1);
 	const int jspan = min(S.n, (S.m + 1) / 2 + 1);
 	// one block per (k, sweep) row segment where it fits
 	const int bs = jspan <= 64 ? 64 : (jspan <= 128 ? 128 : 256);
 	const dim3 grid((jspan + bs - 1) / bs, S.o, nsweeps), block(bs, 1, 1);
 	for (int tau = 0; tau <= tmax + 8 * (nsweeps - 1); tau++) {
-		hipLaunchKernelGGL(k_gs_planes_skew, grid, block, 0, s, S, As, xs, rs, tau, tmax);
+		hipLaunchKernelGGL(k_gs_planes_skew<T>, grid, block, 0, s, S, As, xs, rs, tau, tmax);
 		HIPCHECK_LAUNCH();
 	}
 }
 
-__global__ __launch_bounds__(256) void k_skew_pack_op(LevelView L, SkewView S, const double *__restrict__ A,
-						      double *__restrict__ As) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_skew_pack_op(LevelView L, SkewView S, const T *__restrict__ A,
+						      T *__restrict__ As) {
 	const long long N = (long long)L.m * L.n * L.o;
 	for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < N * 27; q += (long long)gridDim.x * 256) {
 		const int e = (int)(q / N);
@@ -971,13 +972,15 @@ __global__ __launch_bounds__(256) void k_skew_pack_op(LevelView L, SkewView S, c
 	}
 }
 
-void launch_skew_pack_op(hipStream_t s, const LevelView &L, const SkewView &S, const double *A, double *As) {
-	hipLaunchKernelGGL(k_skew_pack_op, dim3(4096), dim3(256), 0, s, L, S, A, As);
+template <typename T>
+void launch_skew_pack_op(hipStream_t s, const LevelView &L, const SkewView &S, const T *A, T *As) {
+	hipLaunchKernelGGL(k_skew_pack_op<T>, dim3(4096), dim3(256), 0, s, L, S, A, As);
 	HIPCHECK_LAUNCH();
 }
 
-__global__ __launch_bounds__(256) void k_skew_vec(LevelView L, SkewView S, const double *__restrict__ v,
-						  double *__restrict__ vs, int to_skew) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_skew_vec(LevelView L, SkewView S, const T *__restrict__ v,
+						  T *__restrict__ vs, int to_skew) {
 	const long long N = (long long)L.m * L.n * L.o;
 	for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < N; p += (long long)gridDim.x * 256) {
 		const int i = (int)(p % L.m), j = (int)((p / L.m) % L.n), k = (int)(p / ((long long)L.m * L.n));
@@ -988,22 +991,25 @@ __global__ __launch_bounds__(256) void k_skew_vec(LevelView L, SkewView S, const
 
 // Ghost copies of a periodic level's halo on its skewed copy (as
 // k_halo_wrap on the level layout).
-__global__ __launch_bounds__(256) void k_skew_halo_wrap(LevelView L, SkewView S, double *__restrict__ xs) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_skew_halo_wrap(LevelView L, SkewView S, T *__restrict__ xs) {
 	const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
 	int i, j, k, si, sj, sk;
 	if (q >= halo_points(L) || !halo_src(L, q, i, j, k, si, sj, sk)) return;
 	xs[sidx3(S, i, j, k)] = xs[sidx3(S, si, sj, sk)];
 }
 
-void launch_skew_halo_wrap(hipStream_t s, const LevelView &L, const SkewView &S, double *xs) {
+template <typename T>
+void launch_skew_halo_wrap(hipStream_t s, const LevelView &L, const SkewView &S, T *xs) {
 	if (!L.prd) return;
-	hipLaunchKernelGGL(k_skew_halo_wrap, dim3((unsigned)((halo_points(L) + 255) / 256)), dim3(256), 0, s, L, S, xs);
+	hipLaunchKernelGGL(k_skew_halo_wrap<T>, dim3((unsigned)((halo_points(L) + 255) / 256)), dim3(256), 0, s, L, S, xs);
 	HIPCHECK_LAUNCH();
 }
 
 // to_skew: vs (skewed) = v (level layout); else vs (level layout) = v (skewed).
-void launch_skew_vec(hipStream_t s, const LevelView &L, const SkewView &S, const double *v, double *vs, bool to_skew) {
-	hipLaunchKernelGGL(k_skew_vec, dim3(2048), dim3(256), 0, s, L, S, v, vs, to_skew ? 1 : 0);
+template <typename T>
+void launch_skew_vec(hipStream_t s, const LevelView &L, const SkewView &S, const T *v, T *vs, bool to_skew) {
+	hipLaunchKernelGGL(k_skew_vec<T>, dim3(2048), dim3(256), 0, s, L, S, v, vs, to_skew ? 1 : 0);
 	HIPCHECK_LAUNCH();
 }
 
@@ -2917,6 +2923,10 @@ void launch_b_add_x(hipStream_t s, const SlotDev *slots, WorkList wl) {
 	template void launch_gs_pair<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int, bool); \
 	template void launch_gs_plane<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int); \
 	template void launch_gs_planes<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int); \
+	template void launch_gs_planes_skew<T>(hipStream_t, const SkewView &, const T *, T *, const T *, int); \
+	template void launch_skew_pack_op<T>(hipStream_t, const LevelView &, const SkewView &, const T *, T *); \
+	template void launch_skew_vec<T>(hipStream_t, const LevelView &, const SkewView &, const T *, T *, bool); \
+	template void launch_skew_halo_wrap<T>(hipStream_t, const LevelView &, const SkewView &, T *); \
 	template void launch_gs_small<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int, int, int); \
 	template int launch_sumsq<T>(hipStream_t, const LevelView &, const T *, double *); \
 	template void launch_restrict<T>(hipStream_t, const LevelView &, const LevelView &, const T *, T *, T *); \

@@ -357,3 +357,24 @@ def test_exact_sweep_after_new_operator_without_compute_rats():
         x = O.gs(A2, x, b, 0)
     assert np.array_equal(h.get_x(0), x)
     h.close()
+
+
+@pytest.mark.parametrize("dims", [(70, 45, 40), (129, 20, 17)])
+def test_mixed_exact_large_levels_bitwise_vs_oracle(dims):
+    # FP32 exact sweeps of levels above the one-workgroup size run on the
+    # FP32 plane-skewed copies
+    say("case", dims)
+    m, n, o = dims
+    A = O.gen_operator(2, 8, m, n, o)
+    b = O.gen_rhs(8, m, n, o)
+    h = P.Hierarchy(m, n, o, smoother=P.SMOOTH_EXACT, precision=P.PREC_MIXED)
+    h.set_operator27(A)
+    h.compute_rats()
+    h.set_rhs(b)
+    mg = O.Multigrid(A)
+    mg.set_rhs(b)
+    c_o, hist_o = mg.solve(1e-30, 3, mixed=True)
+    c_g, hist_g = h.solve(1e-30, 3, check=False)
+    assert c_g == c_o == 3
+    assert np.array_equal(h.get_x(0), mg.get_x(0))
+    h.close()

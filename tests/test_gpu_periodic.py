@@ -278,3 +278,15 @@ def test_periodic_exact_large_level(dims, prd):
         mg.v_cycle()
     assert np.array_equal(h.get_x(0), mg.get_x(0))
     h.close()
+
+
+def test_periodic_mixed_exact_large_level():
+    m, n, o, prd = 48, 40, 30, 5
+    h, A, b = setup((1, 19, m, n, o, prd), smoother=P.SMOOTH_EXACT, precision=P.PREC_MIXED)
+    mg = O.Multigrid(A, prd=prd)
+    mg.set_rhs(b)
+    c_o, _ = mg.solve(1e-30, 3, mixed=True)
+    c_g, _ = h.solve(1e-30, 3, check=False)
+    assert c_g == c_o == 3
+    assert np.array_equal(h.get_x(0), mg.get_x(0))
+    h.close()
