@@ -58,3 +58,18 @@ def test_footprint_scales():
     assert 3.5e9 < f257 < 6e9
     assert f257m > f257 * 1.4
     assert pgmg_amd.footprint(513, 513, 513, precision=pgmg_amd.PREC_MIXED) < 80e9
+
+
+def test_footprint_counts_exact_skewed_copies():
+    # the exact smoother's plane-skewed copies: 29 arrays of
+    # jp * (m + 2n + 4) * (o + 2) elements per level above the one-workgroup
+    # size (FP64, plus FP32 in mixed precision)
+    def skew_elems(m, n, o):
+        jp = ((n + 2 + 15) // 16) * 16
+        return jp * (m + 2 * n + 4) * (o + 2)
+    big = [d for d in pgmg_amd.hierarchy_shape(257, 257, 257) if d[0] * d[1] * d[2] > 40000]
+    extra = sum(29 * skew_elems(*d) for d in big)
+    for prec, word in [(pgmg_amd.PREC_FP64, 8), (pgmg_amd.PREC_MIXED, 12)]:
+        base = pgmg_amd.footprint(257, 257, 257, precision=prec)
+        exact = pgmg_amd.footprint(257, 257, 257, precision=prec, smoother=pgmg_amd.SMOOTH_EXACT)
+        assert exact - base == extra * word
