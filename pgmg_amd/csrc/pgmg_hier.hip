@@ -42,6 +42,9 @@ const int kMaxExact = 2048;       // pgmg_max_exact, common.hh:65
 // call (all sweeps in one launch); larger levels launch per colour/plane.
 const long long kSmallColour = 20000;
 const long long kSmallExact = 40000;
+// Largest FP64 skewed copy (27 coefficient planes, x and r) a level gets for
+// the exact smoother; larger levels sweep the level layout.
+const unsigned long long kSkewMaxBytes = 64ULL << 30;
 
 struct Level {
 	LevelView v;
@@ -473,7 +476,9 @@ int pgmg_hier_footprint(const pgmg_dims *d, const pgmg_opts *opts, uint64_t *byt
 			const LevelView v = make_level_view(dims[3 * l], dims[3 * l + 1], dims[3 * l + 2]);
 			const long long pts = (long long)v.m * v.n * v.o;
 			if (small_lds_bytes(v, 8) <= kSmallLdsBytes || pts <= kSmallExact) continue;
-			b += (uint64_t)make_skew_view(v).size * 29 * ((opts->precision == PGMG_PREC_MIXED) ? 12 : 8);
+			const unsigned long long n = (unsigned long long)make_skew_view(v).size;
+			if (29ULL * n * 8 > kSkewMaxBytes) continue;
+			b += (uint64_t)n * 29 * ((opts->precision == PGMG_PREC_MIXED) ? 12 : 8);
 		}
 	}
 	if (opts && opts->bottom == PGMG_BOTTOM_LU) {
@@ -664,6 +669,7 @@ int pgmg_compute_rats(pgmg_hier *h) {
 				const SkewView sv = make_skew_view(L.v);
 				const size_t n = (size_t)sv.size;
 				size_t fr = 0, tot = 0;
+				if (29ULL * n * 8 > kSkewMaxBytes) continue;
 				if (hipMemGetInfo(&fr, &tot) != hipSuccess || 29 * n * 8 > fr / 2) continue;
 				if (hipMalloc(&L.As, 27 * n * 8) != hipSuccess || hipMalloc(&L.xs, n * 8) != hipSuccess ||
 				    hipMalloc(&L.rs, n * 8) != hipSuccess) {
