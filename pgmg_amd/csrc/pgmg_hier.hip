@@ -659,9 +659,9 @@ int pgmg_compute_rats(pgmg_hier *h) {
 		for (int l = 0; l < h->nl; l++) launch_round_f32(h->stream, h->lev[l].A, h->lev[l].A32, 27 * h->lev[l].v.npad);
 	}
 	if (h->opts.smoother == PGMG_SMOOTH_EXACT) {
-		// plane-skewed copies for the FP64 exact sweeps of large levels
-		// (about three times a level's size each: skipped, and the level
-		// layout swept instead, when they would not fit)
+		// plane-skewed copies for the exact sweeps of large levels (about 1.5
+		// times a level's size each: skipped, and the level layout swept
+		// instead, when they would not fit)
 		for (int l = 0; l < h->nl; l++) {
 			Level &L = h->lev[l];
 			if (gs_use_small<double>(h, L, 0)) continue;

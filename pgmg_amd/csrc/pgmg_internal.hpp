@@ -44,28 +44,36 @@ inline LevelView make_level_view(int m, int n, int o) {
 	return v;
 }
 
-// Plane-skewed copy of a level for the exact smoother: point (i,j,k) at
-// base + j + jp*(i + 2j) + pk*k, so the points of one hyperplane
-// i + 2j + 4k = t with the same k lie side by side in j, and so do each of
-// their neighbours.  Positions that are not grid points hold zero.
+// Plane-skewed copy of a level for the exact smoother.  Within a k-plane,
+// row u = i + 2j holds the points of that u (halo included: i = -1 .. m,
+// j = -1 .. n) in order of j, from j = jf(u): the smallest j whose i is at
+// most m when that makes rows shorter (cmp = 1; at most (m+1)/2 + 2
+// positions), else -1 (rows of n + 2).  Point (i,j,k) is at
+//   (u + 3)*R + (j - jf(u)) + pk*(k + 1),
+// so the points of one hyperplane i + 2j + 4k = t with the same k lie side
+// by side in j, and so do each of their neighbours.  Positions that are not
+// grid points hold zero.
 struct SkewView {
 	int m, n, o;
-	int jp;            // j pitch: n + 2 (j = -1 .. n), rounded up to 16
-	long long pk;      // k pitch: jp * (m + 2n + 4) (u = i + 2j = -3 .. m + 2n)
-	long long base;    // offset of point (0,0,0)
+	int R;             // row length
+	int cmp;           // 1: rows start at jf(u); 0: at j = -1
+	long long pk;      // k pitch: R * (m + 2n + 4) (u = -3 .. m + 2n)
 	long long size;    // elements per array, halo included
 };
 inline SkewView make_skew_view(const LevelView &L) {
 	SkewView v;
 	v.m = L.m; v.n = L.n; v.o = L.o;
-	v.jp = ((L.n + 2 + 15) / 16) * 16;
-	v.pk = (long long)v.jp * (L.m + 2 * L.n + 4);
-	v.base = 1 + 3LL * v.jp + v.pk;
+	const int rc = (L.m + 1) / 2 + 2, rf = L.n + 2;
+	v.cmp = rc < rf ? 1 : 0;
+	v.R = v.cmp ? rc : rf;
+	v.pk = (long long)v.R * (L.m + 2 * L.n + 4);
 	v.size = v.pk * (L.o + 2);
 	return v;
 }
 __host__ __device__ inline long long sidx3(const SkewView &S, int i, int j, int k) {
-	return S.base + j + (long long)S.jp * (i + 2 * j) + S.pk * k;
+	const int u = i + 2 * j;
+	const int jf = S.cmp ? max(-1, (u - S.m + 1) >> 1) : -1;  // ceil((u - m) / 2), at least -1
+	return (long long)(u + 3) * S.R + (j - jf) + S.pk * (k + 1);
 }
 
 // Point (i,j,k) of a per-point array.

@@ -936,13 +936,14 @@ __global__ __launch_bounds__(256) void k_gs_planes_skew(SkewView S, const T *__r
 	const int jhi = min(S.n - 1, sres >> 1);
 	const int j = jlo + (int)(blockIdx.x * blockDim.x + threadIdx.x);
 	if (j > jhi) return;
-	const long long p = S.base + j + (long long)S.jp * sres + S.pk * k;
+	const int i = sres - 2 * j;
+	const long long p = sidx3(S, i, j, k);
 	T ax = (T)0;
 #pragma unroll
 	for (int e = 0; e < 27; e++) {
 		if (e == 13) continue;
 		const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
-		ax = ax + As[e * S.size + p] * xs[p + dj + (long long)S.jp * (di + 2 * dj) + S.pk * dk];
+		ax = ax + As[e * S.size + p] * xs[sidx3(S, i + di, j + dj, k + dk)];
 	}
 	xs[p] = (rs[p] - ax) * ((T)1 / As[13 * S.size + p]);
 }

@@ -62,11 +62,11 @@ def test_footprint_scales():
 
 def test_footprint_counts_exact_skewed_copies():
     # the exact smoother's plane-skewed copies: 29 arrays of
-    # jp * (m + 2n + 4) * (o + 2) elements per level above the one-workgroup
+    # R * (m + 2n + 4) * (o + 2) elements per level above the one-workgroup
     # size (FP64, plus FP32 in mixed precision)
     def skew_elems(m, n, o):
-        jp = ((n + 2 + 15) // 16) * 16
-        return jp * (m + 2 * n + 4) * (o + 2)
+        R = min((m + 1) // 2 + 2, n + 2)
+        return R * (m + 2 * n + 4) * (o + 2)
     big = [d for d in pgmg_amd.hierarchy_shape(257, 257, 257) if d[0] * d[1] * d[2] > 40000]
     extra = sum(29 * skew_elems(*d) for d in big)
     for prec, word in [(pgmg_amd.PREC_FP64, 8), (pgmg_amd.PREC_MIXED, 12)]:
