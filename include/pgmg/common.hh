@@ -3,8 +3,10 @@
  * (reference: common.hh:12-65, common.cc:52-55).
  *
  * Header-only; the numerical work is done by libpgmg_amd.so through the C ABI
- * of pgmg_capi.h.  There is no MPI: one process drives one GPU, so the
- * reference's MPI tags and communicator constants have no counterpart. */
+ * of pgmg_capi.h.  The library uses no MPI: one process drives one GPU, so the
+ * reference's MPI tags have no counterpart.  A caller that includes mpi.h
+ * before these headers (as the reference's drivers do) gets the reference's
+ * communicator constant `world`. */
 #ifndef PGMG_AMD_COMMON_HH
 #define PGMG_AMD_COMMON_HH
 
@@ -13,6 +15,12 @@
 #include <limits>
 
 #include "../pgmg_capi.h"
+
+#ifdef MPI_VERSION
+/** The communicator of the reference's drivers (common.hh:17), e.g. for
+ * MPI_Barrier(world) in mg_parallel.cc:43.  No library call takes it. */
+const MPI_Comm world = MPI_COMM_WORLD;
+#endif
 
 /** The maximum number of multigrid levels (common.hh:15). */
 const int mg_max_levels = 16;

@@ -82,10 +82,15 @@ class multigrid {
 		/** The device hierarchy and its options. */
 		pgmg_hier *hier;
 		pgmg_opts opts;
+		/** Whether setup_matrices / compute_rats fill the host mirrors
+		 * reg[l]->S, A, Am, Asize (the reference's own storage; about 230
+		 * bytes per point of every level).  Default on. */
+		bool mirror_operator;
 
 		template <class p_class>
 		multigrid(p_class &pr, geometry &gm, comm_buffer &com_, const pgmg_opts *opts_ = NULL)
-			: v_down(2), v_bottom(20), v_up(2), rank(0), nlevels(0), mlevels(0), geom(&gm), hier(NULL), com(com_) {
+			: v_down(2), v_bottom(20), v_up(2), rank(0), nlevels(0), mlevels(0), geom(&gm), hier(NULL),
+			  mirror_operator(true), com(com_) {
 			(void)pr;
 			if (opts_) opts = *opts_;
 			else {
@@ -124,8 +129,29 @@ class multigrid {
 		template <class p_class>
 		void setup_r_field(p_class &pr) {reg[0]->setup_r_field(pr);}
 		inline void allclear_x_field() {reg[0]->allclear_x_field();}
-		/** multigrid::compute_rats (multigrid.cc:138-152). */
-		void compute_rats() {pgmg_check(pgmg_compute_rats(hier), "compute_rats");}
+		/** multigrid::compute_rats (multigrid.cc:138-152): the Galerkin
+		 * operators of every coarse level on the device (and the bottom LU
+		 * again where one is live), then the host mirrors: the coarse boxes
+		 * by the reference's bound rule (fill_rat_bounds) from the top
+		 * level's boxes, and the entries inside them. */
+		void compute_rats() {
+			pgmg_check(pgmg_compute_rats(hier), "compute_rats");
+			reg[nlevels - 1]->refresh_exact();
+			if (!mirror_operator || reg[0]->top_boxes.empty()) return;
+			std::vector<int> boxes(reg[0]->top_boxes);
+			reg[0]->mirror_operator(boxes);
+			for (int l = 1; l < nlevels; l++) {
+				std::vector<int> coarse;
+				const region &F = *reg[l - 1], &C = *reg[l];
+				pgmg_detail::rat_bounds(&boxes[0], F.m, F.n, F.o, F.prd_mask(), C.m, C.n, C.o, coarse);
+				reg[l]->mirror_operator(coarse);
+				boxes.swap(coarse);
+			}
+		}
+		/** multigrid::diagnostic_S / _A / _x (multigrid.hh:256-263). */
+		inline void diagnostic_S(int level = 0) {if (level < nlevels) reg[level]->diagnostic_S();}
+		inline void diagnostic_A(int level = 0) {if (level < nlevels) reg[level]->diagnostic_A();}
+		inline void diagnostic_x(int level = 0) {if (level < nlevels) reg[level]->diagnostic_x();}
 
 		/** multigrid::v_cycle (multigrid.cc:88-106). */
 		void v_cycle(int mode = 0) {

@@ -38,8 +38,9 @@ namespace pgmg_detail {
  * k,j,i order, those of the others from external_ptr, in the box's k,j,i
  * order. */
 template <class p_class>
-void ingest_operator(p_class &pr, int m, int n, int o, int prd, std::vector<double> &A27) {
+void ingest_operator(p_class &pr, int m, int n, int o, int prd, std::vector<double> &A27, std::vector<int> *S6 = NULL) {
 	A27.assign((size_t)m * n * o * 27, 0.0);
+	if (S6) S6->resize((size_t)m * n * o * 6);
 	std::vector<double> tmp(27);
 	for (int k = 0; k < o; k++) for (int j = 0; j < n; j++) for (int i = 0; i < m; i++) {
 		int xd = pr.range_xd(i, j, k), xu = pr.range_xu(i, j, k);
@@ -60,11 +61,91 @@ void ingest_operator(p_class &pr, int m, int n, int o, int prd, std::vector<doub
 		} else {
 			src = pr.external_ptr(i, j, k);
 		}
-		double *row = &A27[27 * ((size_t)i + (size_t)m * (j + (size_t)n * k))];
+		const size_t pt = (size_t)i + (size_t)m * (j + (size_t)n * k);
+		if (S6) {
+			int *Sp = &(*S6)[6 * pt];
+			Sp[0] = xd; Sp[1] = xu; Sp[2] = yd; Sp[3] = yu; Sp[4] = zd; Sp[5] = zu;
+		}
+		double *row = &A27[27 * pt];
 		for (int dk = zd; dk < zu; dk++) for (int dj = yd; dj < yu; dj++) for (int di = xd; di < xu; di++)
 			row[(dk + 1) * 9 + (dj + 1) * 3 + (di + 1)] = *(src++);
 	}
 }
+
+/** The fine points (unwrapped indices) whose interpolation weight onto
+ * coarse point I is non-zero, along an axis of fine size M (region.cc:665-746:
+ * the 2x2x2 cube scan and its masks at one rank; coarse size (M+2)>>1, or
+ * (M+1)>>1 when periodic).  On an even non-periodic axis the last fine point
+ * is injected into the last coarse point only; on an even periodic axis the
+ * last cube reaches coarse point 0 across the wrap (fine -1 of coarse 0). */
+inline int fine_support(int I, int M, bool prd, int *f) {
+	int c = 0;
+	if (prd && !(M & 1)) {
+		for (int d = -1; d <= 1; d++) f[c++] = 2 * I + d;
+		return c;
+	}
+	if (!(M & 1) && 2 * I == M) {  // the last coarse point of an even axis
+		f[c++] = M - 1;
+		return c;
+	}
+	const int top = (M & 1) ? M - 1 : M - 2;
+	for (int d = -1; d <= 1; d++)
+		if (2 * I + d >= 0 && 2 * I + d <= top) f[c++] = 2 * I + d;
+	return c;
+}
+
+/** region::grid_map (region.cc:1201-1217): a fine index range [lo, hi) on an
+ * axis of fine size M to the coarse points whose interpolation reaches it. */
+inline void grid_map(int &lo, int &hi, int M, bool prd) {
+	if (prd) {
+		lo = lo < 0 ? (lo - ((M & 1) ? 2 : 1)) / 2 : lo >> 1;
+		hi = (hi + (((M & 1) && hi > M) ? 3 : 2)) >> 1;
+	} else {
+		lo = (!(M & 1) && lo == M - 1 ? lo + 1 : lo) >> 1;
+		hi = (hi + 2) >> 1;
+	}
+}
+
+/** The coarse stencil boxes of the Galerkin operator (region::fill_rat_bounds,
+ * region.cc:598-623, with rat_bound_full/partial, b_red and b_ex,
+ * region.cc:916-1003, 1105-1110, 1276-1295): coarse point I starts with the
+ * box [0,1)^3 and is widened by the coarse image (grid_map) of the box of
+ * every fine point that interpolates from it.  Sf and Sc hold six bounds per
+ * point (xd, xu, yd, yu, zd, zu) in point order i + m*(j + n*k). */
+inline void rat_bounds(const int *Sf, int mf, int nf, int of, int prd, int mc, int nc, int oc, std::vector<int> &Sc) {
+	Sc.assign((size_t)mc * nc * oc * 6, 0);
+	const bool px = prd & 1, py = (prd & 2) != 0, pz = (prd & 4) != 0;
+	int fi[3], fj[3], fk[3];
+	for (int K = 0; K < oc; K++) {
+		const int ck = fine_support(K, of, pz, fk);
+		for (int J = 0; J < nc; J++) {
+			const int cj = fine_support(J, nf, py, fj);
+			for (int I = 0; I < mc; I++) {
+				const int ci = fine_support(I, mf, px, fi);
+				int b[6] = {0, 1, 0, 1, 0, 1};
+				for (int c3 = 0; c3 < ck; c3++) for (int c2 = 0; c2 < cj; c2++) for (int c1 = 0; c1 < ci; c1++) {
+					const int gi = fi[c1], gj = fj[c2], gk = fk[c3];
+					const int wi = (gi % mf + mf) % mf, wj = (gj % nf + nf) % nf, wk = (gk % of + of) % of;
+					const int *Sp = Sf + 6 * ((size_t)wi + (size_t)mf * (wj + (size_t)nf * wk));
+					int lo[3] = {gi + Sp[0], gj + Sp[2], gk + Sp[4]}, hi[3] = {gi + Sp[1], gj + Sp[3], gk + Sp[5]};
+					grid_map(lo[0], hi[0], mf, px);
+					grid_map(lo[1], hi[1], nf, py);
+					grid_map(lo[2], hi[2], of, pz);
+					const int ctr[3] = {I, J, K};
+					for (int a = 0; a < 3; a++) {
+						if (lo[a] - ctr[a] < b[2 * a]) b[2 * a] = lo[a] - ctr[a];
+						if (hi[a] - ctr[a] > b[2 * a + 1]) b[2 * a + 1] = hi[a] - ctr[a];
+					}
+				}
+				int *Sp = &Sc[6 * ((size_t)I + (size_t)mc * (J + (size_t)nc * K))];
+				for (int a = 0; a < 6; a++) Sp[a] = b[a];
+			}
+		}
+	}
+}
+
+/** Size of a box (region.hh:430-433, rat_size). */
+inline int box_size(const int *Sp) {return (Sp[1] - Sp[0]) * (Sp[3] - Sp[2]) * (Sp[5] - Sp[4]);}
 
 }  // namespace pgmg_detail
 
@@ -82,6 +163,21 @@ class region : public pgmg_region_counter<0> {
 		double *x, *x0, *r0;
 		int mg, ng, og, mng, indent;
 		int ai, aj, ak, bi, bj, bk, sm, sn, so, smn, smno;
+		/** Host mirror of the level operator in the reference's storage
+		 * (region.hh:57-61, 256): S holds six box bounds per point (xd, xu,
+		 * yd, yu, zd, zu, offsets from the point), A[p] points at point p's
+		 * entries in Am, in the box's k, j, i order, Asize is the number of
+		 * entries.  Filled when the operator is set up (setup_matrices,
+		 * compute_rats) unless the hierarchy's mirror_operator is off;
+		 * NULL before.  Changing them does not change the device operator. */
+		int *S;
+		double **A;
+		double *Am;
+		int Asize;
+		/** The bottom LU factor as dgetrf leaves it and its 1-based pivots
+		 * (region_tr.cc:472-534), NULL when the level has no live factor. */
+		double *Aexact;
+		int *ipiv;
 		/** The device hierarchy and the level this region views. */
 		pgmg_hier *hier;
 		int level;
@@ -90,7 +186,8 @@ class region : public pgmg_region_counter<0> {
 		template <class p_class>
 		region(p_class &pr, geometry &gm, comm_buffer &com_)
 			: p(NULL), rank(0), x_prd(gm.x_prd), y_prd(gm.y_prd), z_prd(gm.z_prd), gs_enable(true),
-			  m(gm.m), n(gm.n), o(gm.o), hier(NULL), level(0), owns(true) {
+			  m(gm.m), n(gm.n), o(gm.o), S(NULL), A(NULL), Am(NULL), Asize(0), Aexact(NULL), ipiv(NULL),
+			  hier(NULL), level(0), owns(true), exact_enabled(false) {
 			(void)com_;
 			pgmg_dims d = {m, n, o, x_prd ? 1 : 0, y_prd ? 1 : 0, z_prd ? 1 : 0};
 			pgmg_opts opt;
@@ -104,11 +201,104 @@ class region : public pgmg_region_counter<0> {
 		region(region *parent, pgmg_hier *h, int l, int m_, int n_, int o_, bool xp = false, bool yp = false,
 		       bool zp = false)
 			: p(parent), rank(0), x_prd(xp), y_prd(yp), z_prd(zp), gs_enable(true),
-			  m(m_), n(n_), o(o_), hier(h), level(l), owns(false) {init_fields();}
+			  m(m_), n(n_), o(o_), S(NULL), A(NULL), Am(NULL), Asize(0), Aexact(NULL), ipiv(NULL),
+			  hier(h), level(l), owns(false), exact_enabled(false) {init_fields();}
 		~region() {
 			delete[] x;
 			delete[] r0;
+			clear_operator_mirror();
+			delete[] Aexact;
+			delete[] ipiv;
 			if (owns) pgmg_hier_destroy(hier);
+		}
+		inline int prd_mask() const {return (x_prd ? 1 : 0) | (y_prd ? 2 : 0) | (z_prd ? 4 : 0);}
+
+		/** Fills S, A, Am and Asize from box bounds (six per point) and the
+		 * device operator of this level. */
+		void mirror_operator(const std::vector<int> &boxes) {
+			clear_operator_mirror();
+			const size_t N = (size_t)smno;
+			S = new int[6 * N];
+			memcpy(S, &boxes[0], sizeof(int) * 6 * N);
+			long long tot = 0;
+			for (size_t q = 0; q < N; q++) tot += pgmg_detail::box_size(S + 6 * q);
+			Asize = (int)tot;
+			Am = new double[(size_t)tot];
+			A = new double *[N];
+			std::vector<double> A27(27 * N);
+			pgmg_check(pgmg_get_operator27(hier, level, &A27[0]), "operator mirror");
+			double *ap = Am;
+			for (size_t q = 0; q < N; q++) {
+				const int *Sp = S + 6 * q;
+				A[q] = ap;
+				for (int dk = Sp[4]; dk < Sp[5]; dk++) for (int dj = Sp[2]; dj < Sp[3]; dj++)
+					for (int di = Sp[0]; di < Sp[1]; di++)
+						*(ap++) = A27[27 * q + (dk + 1) * 9 + (dj + 1) * 3 + (di + 1)];
+			}
+		}
+		void clear_operator_mirror() {
+			delete[] S; delete[] A; delete[] Am;
+			S = NULL; A = NULL; Am = NULL; Asize = 0;
+		}
+		/** region::enable_exact (region_tr.cc:472-478): with one rank, a
+		 * level of fewer than pgmg_max_exact points is eligible; here only the
+		 * bottom level of a hierarchy can hold the factor. */
+		void enable_exact() {exact_enabled = smno < pgmg_max_exact && is_bottom();}
+		/** region::lu_decomposition (region_tr.cc:483-534): assemble and
+		 * factor the level operator; on failure print the reference's
+		 * message and fall back on Gauss-Seidel (Aexact = NULL). */
+		void lu_decomposition() {
+			if (!exact_enabled) return;
+			pgmg_check(pgmg_lu_decomposition(hier), "lu_decomposition");
+			refresh_exact();
+			if (!Aexact) printf("#-->LAPACK LU decomposition failed\n#   Falling back on Gauss-Seidel\n");
+			exact_enabled = Aexact != NULL;
+		}
+		/** Reloads Aexact / ipiv after the device refactored the bottom. */
+		void refresh_exact() {
+			delete[] Aexact;
+			delete[] ipiv;
+			Aexact = NULL;
+			ipiv = NULL;
+			int on = 0;
+			pgmg_check(pgmg_lu_active(hier, &on), "lu_active");
+			if (!on || !is_bottom()) return;
+			Aexact = new double[(size_t)smno * smno];
+			ipiv = new int[(size_t)smno];
+			pgmg_check(pgmg_get_lu_factor(hier, Aexact, ipiv), "lu factor");
+		}
+		/** region::diagnostic_S / diagnostic_A / diagnostic_x
+		 * (region.cc:1766-1805): the boxes, the operator rows with their
+		 * sums, and x, printed in the reference's formats. */
+		void diagnostic_S() {
+			if (!S) p_fatal_error("No operator mirror to print", PGMG_SETUP_ERROR);
+			for (int k = 0; k < so; k++) for (int j = 0; j < sn; j++) for (int i = 0; i < sm; i++) {
+				const int *Sp = S + 6 * (i + sm * (j + sn * k));
+				printf("%d: (%d,%d,%d) [%d:%d] [%d:%d] [%d:%d]\n", rank, i, j, k, Sp[0], Sp[1], Sp[2], Sp[3], Sp[4], Sp[5]);
+			}
+		}
+		void diagnostic_A() {
+			if (!S) p_fatal_error("No operator mirror to print", PGMG_SETUP_ERROR);
+			for (int k = 0; k < so; k++) for (int j = 0; j < sn; j++) for (int i = 0; i < sm; i++) {
+				const int l = i + sm * (j + sn * k), ss = pgmg_detail::box_size(S + 6 * l);
+				const double *Amp = A[l];
+				double tot = 0;
+				printf("%d: (%d,%d,%d) [", rank, i + ai, j + aj, k + ak);
+				int q;
+				for (q = 0; q < ss - 1; q++) {
+					printf("%g,", Amp[q]);
+					tot += Amp[q];
+				}
+				printf("%g] ", Amp[q]);
+				tot += Amp[q];
+				printf(" tot %g\n", tot);
+			}
+		}
+		void diagnostic_x() {
+			pgmg_check(pgmg_get_x(hier, level, x0), "diagnostic_x");
+			printf("rhs size (%d %d %d), solution size (%d %d %d)\n", sm, sn, so, mg, ng, og);
+			for (int k = 0; k < og; k++) for (int j = 0; j < ng; j++) for (int i = 0; i < mg; i++)
+				printf("(%d, %d, %d): %g\n", i, j, k, x[i + j * mg + k * mng]);
 		}
 
 		/** region::setup_matrices (region.hh:95-101): the top-level operator
@@ -117,10 +307,18 @@ class region : public pgmg_region_counter<0> {
 		template <class p_class>
 		void setup_matrices(p_class &pr) {
 			std::vector<double> A27;
-			pgmg_detail::ingest_operator(pr, m, n, o, (x_prd ? 1 : 0) | (y_prd ? 2 : 0) | (z_prd ? 4 : 0), A27);
+			std::vector<int> boxes;
+			pgmg_detail::ingest_operator(pr, m, n, o, prd_mask(), A27, &boxes);
 			pgmg_check(pgmg_set_operator27(hier, &A27[0]), "setup_matrices");
-			if (owns) pgmg_check(pgmg_compute_rats(hier), "setup_matrices");
+			std::vector<double>().swap(A27);
+			top_boxes.swap(boxes);
+			if (owns) {
+				pgmg_check(pgmg_compute_rats(hier), "setup_matrices");
+				mirror_operator(top_boxes);
+			}
 		}
+		/** The top level's boxes as the problem class gave them. */
+		std::vector<int> top_boxes;
 		template <class p_class>
 		void setup_fields(p_class &pr) {setup_x_field(pr); setup_r_field(pr);}
 		/** region.hh:110-116. */
@@ -249,11 +447,17 @@ class region : public pgmg_region_counter<0> {
 		}
 		inline int total() {return m * n * o;}
 		inline int min_size() {return sm < sn ? (sm < so ? sm : so) : (sn < so ? sn : so);}
-		/** The level operator as 27-entry rows (the reference exposes A and S
-		 * per point; this is the same data in the fixed-box format). */
+		/** The level operator as 27-entry rows (the same data as A / Am / S
+		 * in the fixed box [-1,1]^3, zero where a box is smaller). */
 		void get_operator27(double *A27) {pgmg_check(pgmg_get_operator27(hier, level, A27), "get_operator27");}
 	private:
 		bool owns;
+		bool exact_enabled;
+		bool is_bottom() const {
+			int nl = 0;
+			pgmg_num_levels(hier, &nl);
+			return level == nl - 1;
+		}
 		void output_slice(const char *filename, int k, const double *f) {
 			if (k < 0 || k >= so) p_fatal_error("Output slice out of range", 1);
 			FILE *outf = fopen(filename, "wb");

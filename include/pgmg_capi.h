@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define PGMG_CAPI_VERSION 1
+#define PGMG_CAPI_VERSION 2
 
 /* Status codes: reference enum (common.hh:19-26) + 1, then new ones. */
 enum {
@@ -113,6 +113,18 @@ int pgmg_compute_rats(pgmg_hier *h);
 /* Reports whether the bottom LU factor is in use (0 = it fell back to GS24
  * by the rule of region_tr.cc:526-533). */
 int pgmg_lu_active(const pgmg_hier *h, int *active);
+/* region::enable_exact + region::lu_decomposition of the bottom level
+ * (region_tr.cc:472-534), called by a user after the operators exist: the
+ * dense bottom operator is assembled and factored now.  If the factor passes
+ * the test of region_tr.cc:526-533, the bottom solve uses it from then on
+ * and pgmg_compute_rats factors again (as compute_rats does while Aexact is
+ * set, region.cc:659); otherwise the bottom keeps 24 sweeps (Aexact = NULL).
+ * A bottom of pgmg_max_exact (2048) points or more is never factored. */
+int pgmg_lu_decomposition(pgmg_hier *h);
+/* The factor in use, as dgetrf leaves it (region_tr.cc:518): N*N doubles,
+ * column-major, L below the unit diagonal and U on and above it, and the N
+ * 1-based pivot rows.  N = the bottom level's point count. */
+int pgmg_get_lu_factor(pgmg_hier *h, double *LU, int *ipiv);
 
 /* region::setup_r_field / setup_x_field, region.hh:110-126, and read-back of
  * reg[level]->r0 / x0. */
@@ -201,8 +213,21 @@ int pgmg_batch_free_bytes(const pgmg_batch *b, uint64_t *bytes);
 /* Open-loop serving: requests arrive at the given times (seconds from the
  * call); each is admitted when a slot and arena space are free (FIFO), runs
  * with the others, and completes when its solution is on the host.
- * Results give admission and completion times (seconds from the call). */
-typedef struct { double arrival_s; int m, n, o; int family; uint64_t seed; double tol; int max_cycles; } pgmg_request;
+ * Results give admission and completion times (seconds from the call).
+ * A request's solution is written to its x (m*n*o doubles, point-major, the
+ * caller's host memory, p = i + m*(j + n*k)) before its done_s is taken: the
+ * x0 a solve_v_cycles call leaves in reg[0] (multigrid.hh:182-219).  With x
+ * NULL the solution is still copied off the device (the latency includes it)
+ * and then dropped. */
+typedef struct {
+	double arrival_s;
+	int m, n, o;
+	int family;
+	uint64_t seed;
+	double tol;
+	int max_cycles;
+	double *x;
+} pgmg_request;
 typedef struct { double admit_s, done_s; int cycles; int status; double rel_residual; } pgmg_result;
 int pgmg_serve_run(pgmg_batch *b, const pgmg_request *reqs, int n, pgmg_result *results, double *wall_s);
 

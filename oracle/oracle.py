@@ -306,6 +306,28 @@ class Multigrid:
     def v_cycle(self, mode=0):
         lib().orc_mg_v_cycle(self.h, mode)
 
+    def fmg_cycle(self, v_down=2, v_up=2, v_bottom=20):
+        """multigrid::fmg_cycle (multigrid.cc:109-134) composed from the level
+        operations: restrict to the bottom smoothing on the way, then for each
+        j from the bottom up, down from level j again, v_bottom sweeps at the
+        bottom and interpolate back to level j-1."""
+        L = self.nlevels
+        for i in range(L - 1):
+            self.restriction(i + 1)
+            for _ in range(v_down):
+                self.gauss_seidel(0, i + 1)
+        for j in range(L - 1, 0, -1):
+            for i in range(j, L - 1):
+                self.restriction(i + 1)
+                for _ in range(v_down):
+                    self.gauss_seidel(0, i + 1)
+            for _ in range(v_bottom):
+                self.gauss_seidel(0, L - 1)
+            for i in range(L - 1, j - 1, -1):
+                self.interpolation(i)
+                for _ in range(v_up):
+                    self.gauss_seidel(0, i - 1)
+
     def l2_error_all(self, level=0):
         return lib().orc_mg_l2_error_all(self.h, level)
 

@@ -16,6 +16,12 @@
  *   pgmg_ref fixture <family> <seed> <m> <n> <o> <out.bin>
  *   pgmg_ref solve   <family> <seed> <m> <n> <o> <bottom:gs24|lu> <tol> <maxcycles> <out.bin|->
  *   pgmg_ref time    <family> <seed> <m> <n> <o> <bottom> <tol> <maxcycles>
+ *   pgmg_ref fmg     <family> <seed> <m> <n> <o> <out.bin>
+ *   pgmg_ref diag    <family|simple> <seed> <m> <n> <o> <level> <S|A|x>
+ *     prints the reference's diagnostic_S / diagnostic_A / diagnostic_x of
+ *     one level (x after two V-cycles from x = 0); "simple" is the
+ *     reference's own problem_simple class (problem_simple.hh/.cc), whose
+ *     boundary rows have 1x1x1 boxes.
  *
  * Output is a flat little-endian record stream, each record
  *   char name[32] | int32 dtype (0=f64, 1=i32) | int64 count | data
@@ -28,6 +34,7 @@
 #include <string>
 #include "mpi.h"
 #include "multigrid.hh"
+#include "problem_simple.hh"
 #include "../include/pgmg_problems.h"
 
 class array_problem {
@@ -156,6 +163,30 @@ int main(int argc, char **argv) {
 		return 2;
 	}
 	std::string cmd = argv[1];
+	if (cmd == "diag") {
+		if (argc < 9) {fprintf(stderr, "usage: pgmg_ref diag family seed m n o level S|A|x\n"); MPI_Finalize(); return 2;}
+		const char *colon = strchr(argv[2], ':');
+		const int dprd = colon ? atoi(colon + 1) : 0;
+		const bool simple = strncmp(argv[2], "simple", 6) == 0;
+		int m = atoi(argv[4]), n = atoi(argv[5]), o = atoi(argv[6]), level = atoi(argv[7]);
+		comm_buffer com;
+		geometry gm(m, n, o, (dprd & 1) != 0, (dprd & 2) != 0, (dprd & 4) != 0);
+		problem_simple ps(gm);
+		array_problem pa(gm, simple ? 1 : atoi(argv[2]), strtoull(argv[3], NULL, 10));
+		multigrid *mg = simple ? new multigrid(ps, gm, com) : new multigrid(pa, gm, com);
+		if (simple) {mg->setup_matrices(ps); mg->setup_fields(ps);}
+		else {mg->setup_matrices(pa); mg->setup_fields(pa);}
+		char what = argv[8][0];
+		if (what == 'S') mg->diagnostic_S(level);
+		else if (what == 'A') mg->diagnostic_A(level);
+		else {mg->v_cycle(); mg->v_cycle(); mg->diagnostic_x(level);}
+		fflush(stdout);
+		mg->free();
+		delete mg;
+		gm.free();
+		MPI_Finalize();
+		return 0;
+	}
 	int family = atoi(argv[2]), prd = 0;
 	const char *colon = strchr(argv[2], ':');
 	if (colon) prd = atoi(colon + 1);
@@ -234,6 +265,21 @@ int main(int argc, char **argv) {
 			put_d("vcycle_res_lu", res);
 			put_d("x_v4_lu", level_x(r0));
 		}
+		fclose(outf);
+		printf("{\"ok\": true, \"levels\": %d}\n", mg.nlevels);
+		fflush(stdout);
+	} else if (cmd == "fmg") {
+		/* One full multigrid cycle from x = 0 (multigrid.cc:109-134), then
+		 * one V-cycle on top of it. */
+		outf = fopen(argv[7], "wb");
+		if (!outf) {perror("open"); return 1;}
+		mg.fmg_cycle();
+		put_d("x_fmg", level_x(mg.reg[0]));
+		if (mg.nlevels > 1) put_d("x1_fmg", level_x(mg.reg[1]));
+		put_d("l2_fmg", std::vector<double>(1, mg.l2_error_all()));
+		mg.v_cycle();
+		put_d("x_fmg_v", level_x(mg.reg[0]));
+		put_d("l2_fmg_v", std::vector<double>(1, mg.l2_error_all()));
 		fclose(outf);
 		printf("{\"ok\": true, \"levels\": %d}\n", mg.nlevels);
 		fflush(stdout);

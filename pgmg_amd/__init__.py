@@ -241,6 +241,10 @@ class Hierarchy:
     def v_cycle(self, mode=0):
         _check(lib().pgmg_v_cycle(self.h, mode))
 
+    def set_sweeps(self, v_down, v_up):
+        """The public v_down / v_up of multigrid.hh:122-128, changed."""
+        _check(lib().pgmg_set_sweeps(self.h, v_down, v_up))
+
     def l2_error_all(self, level=0):
         v = ctypes.c_double()
         _check(lib().pgmg_l2_error_all(self.h, level, ctypes.byref(v)))
@@ -315,7 +319,7 @@ class Hierarchy:
 class Request(ctypes.Structure):
     _fields_ = [("arrival_s", ctypes.c_double), ("m", ctypes.c_int), ("n", ctypes.c_int), ("o", ctypes.c_int),
                 ("family", ctypes.c_int), ("seed", ctypes.c_uint64), ("tol", ctypes.c_double),
-                ("max_cycles", ctypes.c_int)]
+                ("max_cycles", ctypes.c_int), ("x", _D)]
 
 
 class Result(ctypes.Structure):
@@ -425,16 +429,21 @@ class Batch:
         _check(lib().pgmg_batch_free_bytes(self.h, ctypes.byref(v)))
         return v.value
 
-    def serve(self, requests):
+    def serve(self, requests, keep_x=True):
         """requests: list of dicts (arrival_s, m, n, o, family, seed, tol,
-        max_cycles).  Returns (results as a list of dicts, wall seconds)."""
+        max_cycles).  Returns (results as a list of dicts, wall seconds); with
+        keep_x each result carries its solution as "x" (shape (o, n, m))."""
         n = len(requests)
         R = (Request * n)()
+        xs = []
         for q, r in enumerate(requests):
-            R[q] = Request(r["arrival_s"], r["m"], r["n"], r["o"], r["family"], r["seed"], r["tol"], r.get("max_cycles", 100))
+            x = np.zeros((r["o"], r["n"], r["m"])) if keep_x else None
+            xs.append(x)
+            R[q] = Request(r["arrival_s"], r["m"], r["n"], r["o"], r["family"], r["seed"], r["tol"],
+                           r.get("max_cycles", 100), _dptr(x) if keep_x else None)
         out = (Result * n)()
         wall = ctypes.c_double()
         _check(lib().pgmg_serve_run(self.h, R, n, out, ctypes.byref(wall)))
         res = [dict(admit_s=o_.admit_s, done_s=o_.done_s, cycles=o_.cycles, status=o_.status,
-                    rel_residual=o_.rel_residual) for o_ in out]
+                    rel_residual=o_.rel_residual, x=x) for o_, x in zip(out, xs)]
         return res, wall.value

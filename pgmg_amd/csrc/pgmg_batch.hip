@@ -246,9 +246,11 @@ int slot_setup(pgmg_batch *b, int s, const pgmg_dims *d, int family, uint64_t se
 	SlotDev &D = S.dev;
 	memset(&D, 0, sizeof D);
 	D.nl = S.nl;
-	// level 0 on the colour-pair kernels: the mixed correction's first sweep
-	// starts from zero without x32 being cleared
-	D.top_zero = b->mixed && S.nl > 1 && !batch_level_small(make_level_view(S.dims[0], S.dims[1], S.dims[2]), 4) ? 1 : 0;
+	// level 0 on the colour-pair kernels with at least one down-sweep: the
+	// mixed correction's first sweep starts from zero without x32 being
+	// cleared (with v_down = 0 the residual pass clears it)
+	D.top_zero = b->mixed && S.nl > 1 && b->opts.v_down >= 1 &&
+			     !batch_level_small(make_level_view(S.dims[0], S.dims[1], S.dims[2]), 4) ? 1 : 0;
 	char *p = b->arena.base + S.off;
 	auto take = [&](uint64_t bytes) { char *q = p; p += al(bytes); return (void *)q; };
 	for (int l = 0; l < S.nl; l++) {
@@ -538,6 +540,7 @@ struct Readouts {
 		int slot;
 		LevelView lv;
 		const double *x;
+		double *dst;  // the request's host array, or null (dropped)
 		bool done = false;
 		int rc = PGMG_OK;
 		double t_done = 0;
@@ -549,7 +552,7 @@ struct Readouts {
 	bool stop = false;
 	hipStream_t stream = nullptr;
 	std::thread worker;
-	std::vector<double> host;  // the worker's destination buffer
+	std::vector<double> host;  // destination of results nobody asked for
 
 	template <typename Clock>
 	void run(Clock now) {
@@ -565,13 +568,17 @@ struct Readouts {
 				jb = jobs[q];
 			}
 			const long long N = (long long)jb.lv.m * jb.lv.n * jb.lv.o;
-			host.resize((size_t)N);
+			double *dst = jb.dst;
+			if (!dst) {
+				host.resize((size_t)N);
+				dst = host.data();
+			}
 			int rc = PGMG_OK;
 			double *stage = nullptr;
 			if (hipMallocAsync((void **)&stage, sizeof(double) * (size_t)N, stream) != hipSuccess) rc = PGMG_ERR_HIP;
 			if (!rc) {
 				launch_unpack_vec(stream, jb.lv, jb.x, stage, 0, N);
-				if (hipMemcpyAsync(host.data(), stage, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost, stream) != hipSuccess)
+				if (hipMemcpyAsync(dst, stage, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost, stream) != hipSuccess)
 					rc = PGMG_ERR_HIP;
 				hipFreeAsync(stage, stream);
 				if (hipStreamSynchronize(stream) != hipSuccess) rc = PGMG_ERR_HIP;
@@ -688,6 +695,7 @@ int pgmg_serve_run(pgmg_batch *b, const pgmg_request *reqs, int n, pgmg_result *
 			jb.slot = s;
 			jb.lv = b->slots[s].dev.lv[0];
 			jb.x = b->slots[s].dev.x[0];
+			jb.dst = reqs[owner[s]].x;
 			std::lock_guard<std::mutex> lk(ro.mu);
 			reading[s] = (int)ro.jobs.size();
 			ro.jobs.push_back(jb);

@@ -210,10 +210,12 @@ bool gs_use_small(const pgmg_hier *h, const Level &L, int mode) {
 	return L.points() <= kSmallColour;
 }
 
-// Whether op_gs(..., from_zero) starts a forward FP32 colour sweep without
-// reading x (then x need not be cleared before it).
+// Whether the level-0 down-sweeps of a cycle start with a forward FP32
+// colour sweep that reads no x (then x need not be cleared before it).  With
+// no down-sweep at level 0 (v_down = 0) nothing overwrites x first, so it
+// must be cleared.
 bool gs_from_zero_ok(const pgmg_hier *h, int level, int mode) {
-	if (h->lev[level].v.prd) return false;
+	if (h->lev[level].v.prd || h->opts.v_down + level < 1) return false;
 	return h->opts.smoother != PGMG_SMOOTH_EXACT && mode == 0 && !gs_use_small<float>(h, h->lev[level], mode);
 }
 
@@ -631,29 +633,38 @@ int pgmg_generate_problem(pgmg_hier *h, int family, uint64_t seed) {
 	return PGMG_OK;
 }
 
+// enable_exact + lu_decomposition of the bottom level (region_tr.cc:472-534):
+// dense assembly and factor; lu_active is whether the factor passed the test
+// of region_tr.cc:526-533 (else the bottom falls back on 24 sweeps).
+static int factor_bottom(pgmg_hier *h) {
+	h->lu_active = 0;
+	Level &B = h->lev[h->nl - 1];
+	const long long N = B.points();
+	if (N >= kMaxExact) return PGMG_OK;  // enable_exact's size rule, region_tr.cc:472-478
+	if (!h->LU) {
+		HIP_TRY(hipMalloc(&h->LU, sizeof(double) * (size_t)(N * N)));
+		HIP_TRY(hipMalloc(&h->ipiv, sizeof(int) * (size_t)N));
+		HIP_TRY(hipMalloc(&h->lu_info, sizeof(int) * 2));
+	}
+	h->lu_N = (int)N;
+	launch_fill<double>(h->stream, h->LU, N * N, 0.0);
+	launch_lu_assemble(h->stream, B.v, B.A, h->LU);
+	launch_lu_factor(h->stream, (int)N, h->LU, h->ipiv, h->lu_info);
+	int info[2] = {0, 1};
+	HIP_TRY(hipMemcpyAsync(info, h->lu_info, sizeof(info), hipMemcpyDeviceToHost, h->stream));
+	HIP_TRY(hipStreamSynchronize(h->stream));
+	h->lu_active = info[1] == 0;
+	return PGMG_OK;
+}
+
 int pgmg_compute_rats(pgmg_hier *h) {
 	if (!h) return fail(PGMG_ERR_ARG, "null hierarchy");
 	free_graphs(h);
 	for (int l = 1; l < h->nl; l++) launch_rat(h->stream, h->lev[l - 1].v, h->lev[l].v, h->lev[l - 1].A, h->lev[l].A);
 	h->lu_active = 0;
 	if (h->opts.bottom == PGMG_BOTTOM_LU) {
-		Level &B = h->lev[h->nl - 1];
-		const long long N = B.points();
-		if (N < kMaxExact) {  // enable_exact, region_tr.cc:472-478
-			if (!h->LU) {
-				HIP_TRY(hipMalloc(&h->LU, sizeof(double) * (size_t)(N * N)));
-				HIP_TRY(hipMalloc(&h->ipiv, sizeof(int) * (size_t)N));
-				HIP_TRY(hipMalloc(&h->lu_info, sizeof(int) * 2));
-			}
-			h->lu_N = (int)N;
-			launch_fill<double>(h->stream, h->LU, N * N, 0.0);
-			launch_lu_assemble(h->stream, B.v, B.A, h->LU);
-			launch_lu_factor(h->stream, (int)N, h->LU, h->ipiv, h->lu_info);
-			int info[2] = {0, 1};
-			HIP_TRY(hipMemcpyAsync(info, h->lu_info, sizeof(info), hipMemcpyDeviceToHost, h->stream));
-			HIP_TRY(hipStreamSynchronize(h->stream));
-			h->lu_active = info[1] == 0;  // else fall back on 24 GS sweeps, region_tr.cc:527-533
-		}
+		int rc = factor_bottom(h);
+		if (rc) return rc;
 	}
 	if (h->opts.precision == PGMG_PREC_MIXED) {
 		for (int l = 0; l < h->nl; l++) launch_round_f32(h->stream, h->lev[l].A, h->lev[l].A32, 27 * h->lev[l].v.npad);
@@ -705,6 +716,28 @@ int pgmg_compute_rats(pgmg_hier *h) {
 	}
 	HIP_TRY(hipStreamSynchronize(h->stream));
 	h->ops_ready = true;
+	return PGMG_OK;
+}
+
+int pgmg_lu_decomposition(pgmg_hier *h) {
+	if (!h) return fail(PGMG_ERR_ARG, "null hierarchy");
+	free_graphs(h);
+	int rc = factor_bottom(h);
+	if (rc) return rc;
+	// as Aexact is kept (and refactored by compute_rats, region.cc:659)
+	// exactly when the factor succeeded
+	h->opts.bottom = h->lu_active ? PGMG_BOTTOM_LU : PGMG_BOTTOM_GS24;
+	return PGMG_OK;
+}
+
+int pgmg_get_lu_factor(pgmg_hier *h, double *LU, int *ipiv) {
+	if (!h || !LU || !ipiv) return fail(PGMG_ERR_ARG, "null argument");
+	if (!h->lu_active) return fail(PGMG_ERR_ARG, "no bottom LU factor is in use");
+	const size_t N = (size_t)h->lu_N;
+	HIP_TRY(hipMemcpyAsync(LU, h->LU, sizeof(double) * N * N, hipMemcpyDeviceToHost, h->stream));
+	HIP_TRY(hipMemcpyAsync(ipiv, h->ipiv, sizeof(int) * N, hipMemcpyDeviceToHost, h->stream));
+	HIP_TRY(hipStreamSynchronize(h->stream));
+	for (size_t q = 0; q < N; q++) ipiv[q] += 1;  // LAPACK's 1-based rows
 	return PGMG_OK;
 }
 

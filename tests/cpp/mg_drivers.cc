@@ -6,7 +6,8 @@
 //  - guide: guide.set_tol(scale) then solve_v_cycles() three times
 //    (guide_test.cc:26-50), printing the return code, residual and cycle
 //    prediction;
-//  - fmg: one fmg_cycle() from x = 0 (multigrid.cc:109-134), then l2;
+//  - fmg: one fmg_cycle() from x = 0 (multigrid.cc:109-134), then l2, and
+//    the iterate written to argv[7];
 //  - slices: two V-cycles, then output_x / output_r / output_residual of
 //    z-slice k (argv[8]) at level 0 and k/2 at level 1, files <x_out>_x0.bin
 //    etc. (as the reference's output routines name nothing, the names are
@@ -16,8 +17,14 @@
 //    source_soln_dot, printing |r|^2 per iteration; then the other
 //    outside-array helpers (copy_rhs_to_soln, soln_diff, overwrite_r_field,
 //    update_r_field) on the result.
-//   mg_drivers <cycles|guide|fmg|slices|pcg> <family> <seed> <m> <n> <o> <x_out> [smoother|k] [bottom]
-// <family> may carry the periodic axes as ":<mask>" (1 = x, 2 = y, 4 = z).
+//  - diag: diagnostic_S / diagnostic_A / diagnostic_x of level argv[7]
+//    (argv[8] = S, A or x; x after two V-cycles), nothing else printed;
+//  - lu: the bottom LU made live by enable_exact() + lu_decomposition() on
+//    the bottom region, then cycles to 1e-10, x written to argv[7];
+//  - ctor_lu: lu_decomposition() before the operators exist.
+//   mg_drivers <cycles|guide|fmg|slices|pcg|diag|lu|ctor_lu> <family> <seed> <m> <n> <o> <x_out|level> [smoother|k|S/A/x] [bottom]
+// <family> may carry the periodic axes as ":<mask>" (1 = x, 2 = y, 4 = z);
+// "simple" is problem_simple.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -26,21 +33,95 @@
 #include "../../include/pgmg/multigrid.hh"
 #include "../../include/pgmg/problems.hh"
 
+template <class P>
+int drive(int argc, char **argv, geometry &gm, P &p);
+
+// overwrite_r_field / update_r_field need the problem class's reset_r_field
+// and update_r_field, which only synthetic_problem has.
+void reset_source(multigrid &mg, synthetic_problem &p) {mg.overwrite_r_field(p);}
+void add_source(multigrid &mg, synthetic_problem &p) {mg.update_r_field(p);}
+void reset_source(multigrid &, problem_simple &) {p_fatal_error("problem_simple has no reset_r_field", 1);}
+void add_source(multigrid &, problem_simple &) {p_fatal_error("problem_simple has no update_r_field", 1);}
+
 int main(int argc, char **argv) {
 	if (argc < 8) {fprintf(stderr, "usage: mg_drivers cycles|guide|fmg family seed m n o x_out [smoother] [bottom]\n"); return 2;}
-	const char *mode = argv[1];
 	const char *colon = strchr(argv[2], ':');
 	const int prd = colon ? atoi(colon + 1) : 0;
 	geometry gm(atoi(argv[4]), atoi(argv[5]), atoi(argv[6]), (prd & 1) != 0, (prd & 2) != 0, (prd & 4) != 0);
+	if (strncmp(argv[2], "simple", 6) == 0) {
+		problem_simple p(gm);
+		return drive(argc, argv, gm, p);
+	}
 	synthetic_problem p(gm, atoi(argv[2]), strtoull(argv[3], NULL, 10));
+	return drive(argc, argv, gm, p);
+}
+
+template <class P>
+int drive(int argc, char **argv, geometry &gm, P &p) {
+	const char *mode = argv[1];
 	comm_buffer com;
 	pgmg_opts opts;
 	pgmg_default_opts(&opts);
-	opts.smoother = argc > 8 && strcmp(mode, "slices") != 0 ? atoi(argv[8]) : PGMG_SMOOTH_EXACT;
-	opts.bottom = argc > 9 ? atoi(argv[9]) : PGMG_BOTTOM_GS24;
+	const bool numeric_opts = strcmp(mode, "slices") != 0 && strcmp(mode, "diag") != 0;
+	opts.smoother = argc > 8 && numeric_opts ? atoi(argv[8]) : PGMG_SMOOTH_EXACT;
+	opts.bottom = argc > 9 && numeric_opts ? atoi(argv[9]) : PGMG_BOTTOM_GS24;
 	multigrid mg(p, gm, com, &opts);
 	mg.setup_matrices(p);
 	mg.setup_fields(p);
+	if (strcmp(mode, "diag") == 0) {
+		// region diagnostics of level argv[7] (S, A, or x after two V-cycles),
+		// alone on standard output
+		const int level = atoi(argv[7]);
+		const char what = argc > 8 ? argv[8][0] : 'S';
+		if (what == 'S') mg.diagnostic_S(level);
+		else if (what == 'A') mg.diagnostic_A(level);
+		else {mg.v_cycle(); mg.v_cycle(); mg.diagnostic_x(level);}
+		fflush(stdout);
+		mg.free();
+		return 0;
+	}
+	if (strcmp(mode, "lu") == 0) {
+		// the bottom LU made live as the reference's users do
+		// (reg[nl-1]->enable_exact(); reg[nl-1]->lu_decomposition();), then
+		// V-cycles to 1e-10 as "cycles"; prints whether Aexact is set
+		region *b = mg.reg[mg.nlevels - 1];
+		printf("aexact_before %d\n", b->Aexact != NULL);
+		b->enable_exact();
+		b->lu_decomposition();
+		printf("aexact_after %d\n", b->Aexact != NULL);
+		if (b->Aexact) printf("lu00 %.17g piv0 %d\n", b->Aexact[0], b->ipiv[0]);
+		mg.compute_rats();
+		printf("aexact_after_rats %d\n", b->Aexact != NULL);
+		double bn = mg.rhs_l2norm(), e = mg.l2_error_all();
+		int c = 0;
+		const int g0 = region::gs_num;
+		while (c < 100 && !(sqrt(e / bn) < 1e-10)) {mg.v_cycle(); c++; e = mg.l2_error_all();}
+		printf("cycles %d res %.17g sweeps_per_cycle %d\n", c, e, c ? (region::gs_num - g0) / c : 0);
+		mg.reg[0]->sync_from_device();
+		FILE *f = fopen(argv[7], "wb");
+		if (!f) return 1;
+		fwrite(mg.reg[0]->x0, sizeof(double), (size_t)mg.reg[0]->smno, f);
+		fclose(f);
+		mg.free();
+		return 0;
+	}
+	if (strcmp(mode, "ctor_lu") == 0) {
+		// lu_decomposition before any operator exists (what the reference's
+		// constructor does, multigrid.cc:81-84): the factor fails, Aexact
+		// stays NULL and the bottom keeps 24 sweeps after setup
+		multigrid mg2(p, gm, com, &opts);
+		region *b = mg2.reg[mg2.nlevels - 1];
+		b->enable_exact();
+		b->lu_decomposition();
+		printf("aexact_ctor %d\n", b->Aexact != NULL);
+		mg2.setup_matrices(p);
+		mg2.setup_fields(p);
+		printf("aexact_setup %d\n", b->Aexact != NULL);
+		const int g0 = region::gs_num;
+		mg2.v_cycle();
+		printf("sweeps_per_cycle %d\n", region::gs_num - g0);
+		return 0;
+	}
 	printf("levels %d", mg.nlevels);
 	for (int l = 0; l < mg.nlevels; l++) printf(" %dx%dx%d", mg.reg[l]->m, mg.reg[l]->n, mg.reg[l]->o);
 	printf("\n");
@@ -123,16 +204,21 @@ int main(int argc, char **argv) {
 		R.sync_x_to_device();
 		printf("diff_self %.17g\n", mg.soln_diff(u));
 		// source back to b, then x = r and soln_diff against b is 0
-		mg.overwrite_r_field(p);
+		reset_source(mg, p);
 		mg.copy_rhs_to_soln();
 		printf("diff_rhs %.17g\n", mg.soln_diff(b));
 		// update_r_field adds 0.5 b: rhs_l2norm grows by 2.25
-		mg.update_r_field(p);
+		add_source(mg, p);
 		printf("rhs_ratio %.17g\n", mg.rhs_l2norm() / bb);
 		delete[] b; delete[] u; delete[] r; delete[] z; delete[] pv; delete[] Ap;
 	} else {
 		mg.fmg_cycle();
 		printf("fmg %.17g rhs %.17g\n", mg.l2_error_all(), mg.rhs_l2norm());
+		mg.reg[0]->sync_from_device();
+		FILE *f = fopen(argv[7], "wb");
+		if (!f) return 1;
+		fwrite(mg.reg[0]->x0, sizeof(double), (size_t)mg.reg[0]->smno, f);
+		fclose(f);
 	}
 	mg.free();
 	return 0;

@@ -20,6 +20,7 @@ import os
 import struct
 import subprocess
 import sys
+import tempfile
 
 import numpy as np
 
@@ -63,6 +64,33 @@ SUMMARIES = [
 SAMPLE_POINTS = [0, 1, 257 * 128 + 77, 257 * 257 * 128 + 257 * 128 + 128, 257 ** 3 - 1]
 
 
+# One fmg_cycle from x = 0, then one V-cycle: (name, family, seed, m, n, o,
+# prd).
+FMGS = [
+    ("fmg_f7u_20x17x30", 1, 9, 20, 17, 30, 0),
+    ("fmg_q1_33x29x40", 3, 21, 33, 29, 40, 0),
+    ("fmg_prd_f7u_13x10x11_p5", 1, 9, 13, 10, 11, 5),
+]
+
+
+# The reference's diagnostic_S / diagnostic_A / diagnostic_x of every level
+# (x after two V-cycles): (name, family or "simple", seed, m, n, o, prd).
+# "simple" is the reference's own problem_simple (boundary boxes 1x1x1).
+DIAGS = [
+    ("diag_f7u_13x10x11", "1", 5, 13, 10, 11, 0),
+    ("diag_simple_14x12x10", "simple", 0, 14, 12, 10, 0),
+    ("diag_simple_18x9x21", "simple", 0, 18, 9, 21, 0),
+    ("diag_q1_18x13x22", "3", 15, 18, 13, 22, 0),
+    ("diag_prd_f7u_13x10x11_p5", "1", 5, 13, 10, 11, 5),
+    ("diag_prd_simple_12x10x14_p3", "simple", 0, 12, 10, 14, 3),
+    ("diag_prd_q1_10x14x12_p7", "3", 4, 10, 14, 12, 7),
+    # three levels (boxes only): coarse boxes from coarse boxes
+    ("diag_simple_30x20x25", "simple", 0, 30, 20, 25, 0),
+    ("diag_prd_simple_28x22x17_p5", "simple", 0, 28, 22, 17, 5),
+]
+DIAG_BOXES_ONLY = ("diag_simple_30x20x25", "diag_prd_simple_28x22x17_p5")
+
+
 # z-slice files written by the reference's output_x / output_r /
 # output_residual after two V-cycles: (name, family, seed, m, n, o, k).
 SLICES = [
@@ -87,6 +115,17 @@ def read_records(path):
     return out
 
 
+def run_text(args):
+    """The reference's standard output, without its constructor's '#' lines."""
+    env = dict(os.environ)
+    env["LD_LIBRARY_PATH"] = "/opt/conda/lib"
+    p = subprocess.run([REF] + [str(a) for a in args], capture_output=True, text=True, env=env, timeout=600)
+    if p.returncode != 0:
+        sys.stderr.write(p.stdout + p.stderr)
+        raise RuntimeError("reference run failed: %s" % args)
+    return "".join(l + "\n" for l in p.stdout.splitlines() if not l.startswith("#"))
+
+
 def run(args):
     env = dict(os.environ)
     env["LD_LIBRARY_PATH"] = "/opt/conda/lib"
@@ -108,8 +147,12 @@ def main():
 
     def want(name):
         return not only or name in only
-    tmp = os.path.join(ROOT, "_work", "golden_tmp.bin")
-    os.makedirs(os.path.dirname(tmp), exist_ok=True)
+    with tempfile.TemporaryDirectory() as tmpdir:
+        write_all(want, tmpdir)
+
+
+def write_all(want, tmpdir):
+    tmp = os.path.join(tmpdir, "records.bin")
     for name, fam, seed, m, n, o, prd in [f + (0,) for f in FIXTURES] + PFIXTURES:
         if not want(name):
             continue
@@ -161,7 +204,7 @@ def main():
     for name, fam, seed, m, n, o, k in SLICES:
         if not want(name):
             continue
-        prefix = os.path.join(ROOT, "_work", "golden_slice")
+        prefix = os.path.join(tmpdir, "slice")
         run(["slices", fam, seed, m, n, o, k, prefix])
         save = {"meta": np.array([fam, seed, m, n, o, k], dtype=np.int64)}
         for field in ("x", "r", "res"):
@@ -171,7 +214,37 @@ def main():
                 os.remove(path)
         np.savez_compressed(os.path.join(OUT, name + ".npz"), **save)
         print(name, flush=True)
-    os.remove(tmp)
+    for name, fam, seed, m, n, o, prd in DIAGS:
+        if not want(name):
+            continue
+        famarg = "%s:%d" % (fam, prd) if prd else fam
+        save = {"meta": np.array([0 if fam == "simple" else int(fam), seed, m, n, o, prd], dtype=np.int64)}
+        lev = 0
+        while True:
+            txt = run_text(["diag", famarg, seed, m, n, o, lev, "S"])
+            if not txt:
+                break
+            save["S%d" % lev] = np.frombuffer(txt.encode(), dtype=np.uint8)
+            if name in DIAG_BOXES_ONLY:
+                lev += 1
+                continue
+            save["A%d" % lev] = np.frombuffer(run_text(["diag", famarg, seed, m, n, o, lev, "A"]).encode(), dtype=np.uint8)
+            if not prd:
+                save["x%d" % lev] = np.frombuffer(run_text(["diag", famarg, seed, m, n, o, lev, "x"]).encode(),
+                                                  dtype=np.uint8)
+            lev += 1
+        save["levels"] = np.array(lev)
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), **save)
+        print(name, "levels", lev, flush=True)
+    for name, fam, seed, m, n, o, prd in FMGS:
+        if not want(name):
+            continue
+        run(["fmg", "%d:%d" % (fam, prd) if prd else fam, seed, m, n, o, tmp])
+        rec = read_records(tmp)
+        save = {"meta": np.array([fam, seed, m, n, o, prd], dtype=np.int64)}
+        save.update(rec)
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), **save)
+        print(name, flush=True)
 
 
 if __name__ == "__main__":

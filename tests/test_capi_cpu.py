@@ -36,7 +36,7 @@ def test_library_exports_all_declared_symbols():
 
 
 def test_version_and_defaults():
-    assert pgmg_amd.lib().pgmg_version() == 1
+    assert pgmg_amd.lib().pgmg_version() == 2
     o = pgmg_amd.default_opts()
     assert (o.precision, o.smoother, o.bottom, o.v_down, o.v_up, o.v_bottom) == (0, 1, 0, 2, 2, 20)
 

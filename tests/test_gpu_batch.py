@@ -129,6 +129,58 @@ def test_serving_open_loop_small():
     assert wall > 0
 
 
+def test_serving_returns_each_solution_bitwise_as_alone():
+    """Staggered requests of uneven sizes served by pgmg_serve_run, several
+    on the GPU at once: each returned x equals the same request solved by a
+    lone hierarchy bit for bit (tolerance 0), and one is also checked against
+    the oracle's mixed-precision solve of the same problem."""
+    rng = np.random.default_rng(11)
+    reqs = []
+    t = 0.0
+    for q in range(20):
+        t += rng.exponential(0.002)
+        dims = [int(round(np.exp(rng.uniform(np.log(17), np.log(56))))) for _ in range(3)]
+        fam = [P.FAM_Q1_ANISO, P.FAM_FACE7_UNIFORM][q % 2]
+        reqs.append(dict(arrival_s=t, m=dims[0], n=dims[1], o=dims[2], family=fam, seed=2000 + q,
+                         tol=1e-8, max_cycles=80))
+    with P.Batch(4 << 30, max_slots=8) as b:
+        res, wall = b.serve(reqs)
+        assert b.free_bytes() == 4 << 30
+    overlap = sum(1 for a in res for c in res if a is not c and a["admit_s"] < c["admit_s"] < a["done_s"])
+    say("requests", len(reqs), "wall", wall, "admissions during another solve", overlap)
+    assert overlap > 0
+    for q, (r, rq) in enumerate(zip(res, reqs)):
+        assert r["status"] == 0 and r["rel_residual"] < 1e-8, (q, r)
+        c, x = alone(rq["m"], rq["n"], rq["o"], rq["family"], rq["seed"], P.PREC_MIXED, P.BOTTOM_GS24, 1e-8, 80)
+        assert r["cycles"] == c, q
+        assert np.array_equal(r["x"], x), q
+    rq = reqs[1]
+    A = O.gen_operator(rq["family"], rq["seed"], rq["m"], rq["n"], rq["o"])
+    mg = O.Multigrid(A, colour8=True)
+    mg.set_rhs(O.gen_rhs(rq["seed"], rq["m"], rq["n"], rq["o"]))
+    c, hist = mg.solve(1e-8, 80, mixed=True)
+    assert c == res[1]["cycles"]
+    assert np.array_equal(res[1]["x"], mg.get_x(0))
+
+
+@pytest.mark.parametrize("v_down,v_up", [(0, 2), (1, 3)])
+def test_batch_mixed_sweep_counts_match_oracle(v_down, v_up):
+    """Mixed precision with other sweep counts, v_down = 0 included (no
+    level-0 down-sweep, so the FP32 correction must be cleared before the
+    cycle): the batch against the oracle's defect correction."""
+    m, n, o = 33, 29, 40
+    A = O.gen_operator(P.FAM_FACE7_UNIFORM, 12, m, n, o)
+    bvec = O.gen_rhs(12, m, n, o)
+    mg = O.Multigrid(A, colour8=True, v_down=v_down, v_up=v_up)
+    mg.set_rhs(bvec)
+    c, hist = mg.solve(1e-30, 4, mixed=True)
+    with P.Batch(1 << 30, max_slots=2, v_down=v_down, v_up=v_up) as b:
+        s = b.admit_operator(A, bvec, 1e-30, max_cycles=4)
+        b.run()
+        assert b.status(s)[1] == c == 4
+        assert np.array_equal(b.get_x(s), mg.get_x(0))
+
+
 def test_serving_request_that_never_fits():
     """A request larger than the whole arena is reported (admission error)
     instead of waiting forever; the others are served."""

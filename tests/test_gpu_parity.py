@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 FIXTURES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
-                  if not os.path.basename(p).startswith(("solve_", "slices_", "summary_", "prd_")))
+                  if not os.path.basename(p).startswith(("solve_", "slices_", "summary_", "prd_", "fmg_", "diag_")))
 NORM_RTOL = 1e-12
 
 
@@ -148,6 +148,34 @@ def test_mixed_precision_bitwise_vs_oracle(name, smoother):
     h.close()
 
 
+@pytest.mark.parametrize("v_down,v_up", [(0, 2), (1, 3), (3, 0)])
+@pytest.mark.parametrize("smoother", [P.SMOOTH_COLOUR8, P.SMOOTH_EXACT])
+def test_mixed_sweep_counts_match_oracle(v_down, v_up, smoother):
+    """Mixed-precision defect correction with other sweep counts: with
+    v_down = 0 no level-0 sweep overwrites the FP32 correction, which must
+    then be cleared before each cycle.  Graphs on and off."""
+    m, n, o = 70, 45, 40
+    say("case", v_down, v_up, smoother)
+    A = O.gen_operator(P.FAM_FACE7_UNIFORM, 13, m, n, o)
+    b = O.gen_rhs(13, m, n, o)
+    mg = O.Multigrid(A, colour8=smoother == P.SMOOTH_COLOUR8, v_down=v_down, v_up=v_up)
+    mg.set_rhs(b)
+    c2, hist2 = mg.solve(1e-30, 3, mixed=True)
+    for graph in (True, False):
+        h = P.Hierarchy(m, n, o, precision=P.PREC_MIXED, smoother=smoother, use_graph=graph)
+        h.set_operator27(A)
+        h.compute_rats()
+        h.set_rhs(b)
+        h.set_sweeps(2, 2)
+        h.solve(1e-30, 1, check=False)  # leaves a non-zero correction behind
+        h.set_sweeps(v_down, v_up)
+        h.set_x(np.zeros((o, n, m)))
+        c, hist = h.solve(1e-30, 3, check=False)
+        assert c == c2 == 3
+        assert np.array_equal(h.get_x(0), mg.get_x(0)), graph
+        h.close()
+
+
 def test_solve_65_exact_mode_matches_reference():
     z = load("solve_f7u_65_gs24")
     h, A, b, meta = setup(z, smoother=P.SMOOTH_EXACT)
@@ -156,6 +184,34 @@ def test_solve_65_exact_mode_matches_reference():
     np.testing.assert_allclose(hist, z["res"], rtol=NORM_RTOL)
     assert np.array_equal(h.get_x(0).reshape(-1), z["x"])
     assert h.gs_count() == c * 42
+    h.close()
+
+
+def test_solve_65_lu_bottom_matches_reference():
+    """The 65^3 case as stated: exact smoother, the bottom LU made live as in
+    region_tr.cc:472-534, to 1e-10.  Against the reference run with its LU
+    (MKL dgetrf/dgetrs): the same cycle count, x to 1e-13 of its largest
+    value, the residual history to 1e-6 (the late residuals are rounding
+    sized, where the two factorisations differ).  Against the oracle's LU:
+    bit for bit."""
+    z = load("solve_f7u_65_lu")
+    fam, seed, m, n, o = [int(v) for v in z["meta"]]
+    say("case 65^3 exact mode, LU bottom")
+    h = P.Hierarchy(m, n, o, smoother=P.SMOOTH_EXACT, bottom=P.BOTTOM_LU)
+    h.generate_problem(fam, seed)
+    h.compute_rats()
+    assert h.lu_active()
+    c, hist = h.solve(float(z["tol"]), 50)
+    assert c == len(z["res"]) - 1
+    np.testing.assert_allclose(hist, z["res"], rtol=1e-6)
+    x = h.get_x(0).reshape(-1)
+    assert np.abs(x - z["x"]).max() <= 1e-13 * np.abs(z["x"]).max()
+    assert h.gs_count() == c * (4 + 6 + 8)  # no sweeps at the bottom
+    mg = O.Multigrid(O.gen_operator(fam, seed, m, n, o), lu=True)
+    mg.set_rhs(O.gen_rhs(seed, m, n, o))
+    c2, hist2 = mg.solve(float(z["tol"]), 50)
+    assert c2 == c
+    assert np.array_equal(x, mg.get_x(0).reshape(-1))
     h.close()
 
 
