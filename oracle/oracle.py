@@ -82,6 +82,7 @@ def lib():
             "orc_mg_rhs_l2norm": (ctypes.c_double, [_VP, _i]),
             "orc_mg_solve": (_i, [_VP, ctypes.c_double, _i, _D]),
             "orc_mg_solve_mixed": (_i, [_VP, ctypes.c_double, _i, _D]),
+            "orc_mg_get_Af": (_i, [_VP, _i, _F]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -327,6 +328,11 @@ class Multigrid:
                 self.interpolation(i)
                 for _ in range(v_up):
                     self.gauss_seidel(0, i - 1)
+
+    def get_Af(self, l):
+        """FP32 operator of level l made by the last mixed solve (None before one)."""
+        A = np.zeros(self.shape(l) + (27,), dtype=np.float32)
+        return A if lib().orc_mg_get_Af(self.h, l, _p(A)) else None
 
     def l2_error_all(self, level=0):
         return lib().orc_mg_l2_error_all(self.h, level)

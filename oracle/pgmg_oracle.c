@@ -605,3 +605,9 @@ int orc_mg_solve_mixed(orc_mg *mg, double tol, int maxc, double *hist) {
 	free(res);
 	return c;
 }
+
+int orc_mg_get_Af(const orc_mg *mg, int l, float *A27) {
+	if (!mg->Af[l]) return 0;
+	memcpy(A27, mg->Af[l], sizeof(float) * 27 * LPTS(mg, l));
+	return 1;
+}

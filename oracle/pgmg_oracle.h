@@ -100,6 +100,9 @@ int orc_mg_solve(orc_mg *mg, double tol, int maxc, double *res_hist);
 /* Mixed precision: FP64 defect correction around an FP32 V-cycle whose
  * operators are the FP64 ones rounded to FP32; bottom solve in FP64. */
 int orc_mg_solve_mixed(orc_mg *mg, double tol, int maxc, double *res_hist);
+/* The FP32 operator of level l that orc_mg_solve_mixed made (27 entries per
+ * point).  Returns 0 if no mixed solve has run yet, else 1. */
+int orc_mg_get_Af(const orc_mg *mg, int level, float *A27);
 
 #ifdef __cplusplus
 }
