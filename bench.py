@@ -309,6 +309,38 @@ def serving(P, n_req=160, seed=11):
     return out
 
 
+def exact_breakdown(P, prec_name="fp64", graph=False):
+    """The 257^3 exact-mode solve to 1e-10 (the reference's sweep order),
+    timed whole, then once per level with HIP events around that level's
+    smoothing calls (eager launches while a hook is on), and around the
+    level-0 residual and the restrictions."""
+    prec = P.PREC_FP64 if prec_name == "fp64" else P.PREC_MIXED
+    he = P.Hierarchy(N, N, N, precision=prec, smoother=P.SMOOTH_EXACT, use_graph=graph)
+    he.generate_problem(P.FAM_FACE7_UNIFORM, SEED)
+    he.compute_rats()
+    he.solve(TOL, 1, check=False)
+    he.synchronize()
+    he.clear_x(0)
+    t0 = time.perf_counter()
+    ce, hist = he.solve(TOL, 100)
+    he.synchronize()
+    out = {"solve_s": round(time.perf_counter() - t0, 4), "cycles": ce, "graph": graph,
+           "precision": prec_name, "levels": {}}
+    for kc, name in [(P.KC_GS, "gs"), (P.KC_RESIDUAL, "residual"), (P.KC_RESTRICT, "restrict"),
+                     (P.KC_INTERP, "interp")]:
+        for l in range(he.nlevels):
+            he.clear_x(0)
+            he.prof_enable(kc, l)
+            he.solve(TOL, 100)
+            n, ms, pts = he.prof_read()
+            he.prof_enable(0, 0)
+            if n:
+                out["levels"].setdefault(str(l), {})[name] = {"calls": n, "ms_total": round(ms, 3),
+                                                               "points": pts}
+    he.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -317,7 +349,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 513^3, batch and serving legs")
-    ap.add_argument("--leg", choices=["pcie", "q1_513", "batch_mixed", "batch_fp64", "serving", "periodic_256"],
+    ap.add_argument("--leg", choices=["pcie", "q1_513", "batch_mixed", "batch_fp64", "serving", "periodic_256",
+                                      "exact", "exact_mixed"],
                     help="run one extra leg alone and print its JSON (for profiling)")
     ap.add_argument("--periodic-modes", default="exact_fp64,colour8_fp64,colour8_mixed",
                     help="with --leg periodic_256: the smoother/precision runs to time (comma-separated)")
@@ -339,7 +372,9 @@ def main():
               "batch_fp64": lambda: batch_256(P, P.PREC_FP64), "serving": lambda: serving(P),
               "periodic_256": lambda: periodic_256(P, tests=args.periodic_tests,
                                                    modes=tuple(args.periodic_modes.split(",")),
-                                                   ref_cpu=not args.no_cpu)}[args.leg]
+                                                   ref_cpu=not args.no_cpu),
+              "exact": lambda: exact_breakdown(P, "fp64"),
+              "exact_mixed": lambda: exact_breakdown(P, "mixed")}[args.leg]
         print(json.dumps({"leg": args.leg, "result": fn()}), flush=True)
         return
 

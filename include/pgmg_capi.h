@@ -176,8 +176,16 @@ int pgmg_x_diff2(pgmg_hier *h, int level, const double *ext_x, double *out);
  * NULL) receives l2_error_all before and after each cycle.  Returns
  * PGMG_ERR_NOT_CONVERGED if max_cycles are used up, PGMG_ERR_NAN on NaN. */
 int pgmg_solve(pgmg_hier *h, double tol, int max_cycles, int *cycles, double *hist);
-/* Wait for all work on the hierarchy's stream. */
+/* Wait for all work on the hierarchy's stream.  Also reports (and clears)
+ * an exact sweep that gave up waiting (see pgmg_set_wait_limit). */
 int pgmg_synchronize(pgmg_hier *h);
+/* No counterpart in the reference.  Exact-mode sweeps of a level run in one
+ * launch in which the workgroup of each k-plane waits for its neighbour
+ * planes' progress; each wait gives up after `polls` polls (default 2^21,
+ * about a second), marks the sweep failed and lets the launch end.  The
+ * failure is returned (PGMG_ERR_HIP) by the next pgmg_synchronize,
+ * pgmg_get_x, norm or solve call.  A small limit is a test of that path. */
+int pgmg_set_wait_limit(pgmg_hier *h, int polls);
 /* region::gs_num, region.hh:17: Gauss-Seidel sweeps performed so far. */
 int pgmg_gs_count(const pgmg_hier *h, long long *count);
 

@@ -81,6 +81,7 @@ _SIGS = {
     "pgmg_rhs_x_dot": (_i, [_VP, _i, _D]),
     "pgmg_x_diff2": (_i, [_VP, _i, _D, _D]),
     "pgmg_synchronize": (_i, [_VP]),
+    "pgmg_set_wait_limit": (_i, [_VP, _i]),
     "pgmg_gs_count": (_i, [_VP, ctypes.POINTER(ctypes.c_longlong)]),
     "pgmg_prof_enable": (_i, [_VP, _i, _i]),
     "pgmg_prof_read": (_i, [_VP, ctypes.POINTER(ctypes.c_longlong), _D, ctypes.POINTER(ctypes.c_longlong)]),
@@ -296,6 +297,10 @@ class Hierarchy:
 
     def synchronize(self):
         _check(lib().pgmg_synchronize(self.h))
+
+    def set_wait_limit(self, polls):
+        """Polls after which an exact sweep's wait between planes gives up."""
+        _check(lib().pgmg_set_wait_limit(self.h, polls))
 
     def gs_count(self):
         v = ctypes.c_longlong()

@@ -45,6 +45,10 @@ const long long kSmallExact = 40000;
 // Largest FP64 skewed copy (27 coefficient planes, x and r) a level gets for
 // the exact smoother; larger levels sweep the level layout.
 const unsigned long long kSkewMaxBytes = 64ULL << 30;
+// Polls a plane of the exact sweeps makes before it gives up waiting for a
+// neighbour plane (a poll takes about a microsecond; a wait in a working
+// sweep lasts a few).
+const int kFlowLimit = 1 << 21;
 
 struct Level {
 	LevelView v;
@@ -76,8 +80,11 @@ struct pgmg_hier {
 	float *res32 = nullptr;
 	double *partial = nullptr;
 	int partial_cap = 0;
-	double *dscalar = nullptr;  // device reduction result
-	double *hscalar = nullptr;  // pinned host mirror
+	double *dscalar = nullptr;  // device reduction result; the int after it is the
+	                            // exact sweeps' wait flag (set when a wait gave up)
+	double *hscalar = nullptr;  // pinned host mirror of both
+	int *flow_prog = nullptr;   // plane progress counters of the exact sweeps
+	int flow_limit = kFlowLimit;
 	double *LU = nullptr;
 	int *ipiv = nullptr, *lu_info = nullptr;
 	int lu_N = 0, lu_active = 0;
@@ -219,6 +226,18 @@ bool gs_from_zero_ok(const pgmg_hier *h, int level, int mode) {
 	return h->opts.smoother != PGMG_SMOOTH_EXACT && mode == 0 && !gs_use_small<float>(h, h->lev[level], mode);
 }
 
+int *flow_err(pgmg_hier *h) { return (int *)(h->dscalar + 1); }
+
+// Forward exact sweeps on a level's plane-skewed copies: one launch per
+// call where all the level's planes fit on the device at once, else one
+// launch per hyperplane.
+template <typename T>
+void skew_sweeps(pgmg_hier *h, Level &L, int nsweeps) {
+	if (!h->flow_prog || !launch_gs_flow<T>(h->stream, L.sv, sA<T>(L), sx<T>(L), sr<T>(L), nsweeps, h->flow_prog,
+						 flow_err(h), h->flow_limit))
+		launch_gs_planes_skew<T>(h->stream, L.sv, sA<T>(L), sx<T>(L), sr<T>(L), nsweeps);
+}
+
 // from_zero: x is 0 on entry (and may not be cleared in memory); only with
 // gs_from_zero_ok.
 template <typename T>
@@ -246,7 +265,7 @@ void op_gs(pgmg_hier *h, int level, int mode, int nsweeps, bool from_zero = fals
 			launch_skew_vec<T>(h->stream, L.v, L.sv, lr<T>(L), sr<T>(L), true);
 			for (int s = 0; s < nsweeps; s++) {
 				launch_skew_halo_wrap<T>(h->stream, L.v, L.sv, sx<T>(L));
-				launch_gs_planes_skew<T>(h->stream, L.sv, sA<T>(L), sx<T>(L), sr<T>(L), 1);
+				skew_sweeps<T>(h, L, 1);
 			}
 			launch_skew_vec<T>(h->stream, L.v, L.sv, sx<T>(L), lx<T>(L), false);
 			return;
@@ -283,7 +302,7 @@ void op_gs(pgmg_hier *h, int level, int mode, int nsweeps, bool from_zero = fals
 			// on the plane-skewed copies: x and r in, x out
 			launch_skew_vec<T>(h->stream, L.v, L.sv, lx<T>(L), sx<T>(L), true);
 			launch_skew_vec<T>(h->stream, L.v, L.sv, lr<T>(L), sr<T>(L), true);
-			launch_gs_planes_skew<T>(h->stream, L.sv, sA<T>(L), sx<T>(L), sr<T>(L), nsweeps);
+			skew_sweeps<T>(h, L, nsweeps);
 			launch_skew_vec<T>(h->stream, L.v, L.sv, sx<T>(L), lx<T>(L), false);
 			return;
 		}
@@ -372,11 +391,29 @@ int ensure_partial(pgmg_hier *h, int n) {
 	return PGMG_OK;
 }
 
+// The exact sweeps' wait flag as read back into hscalar[1]: clears it and
+// reports the error if it was set.
+int flow_flag_result(pgmg_hier *h) {
+	int e = 0;
+	memcpy(&e, h->hscalar + 1, sizeof(int));
+	if (!e) return PGMG_OK;
+	launch_fill<int>(h->stream, flow_err(h), 1, 0);
+	hipStreamSynchronize(h->stream);
+	return fail(PGMG_ERR_HIP, "exact sweep: a plane gave up waiting for its neighbour after %d polls; "
+		    "the level's x is incomplete", h->flow_limit);
+}
+
+int check_flow(pgmg_hier *h) {
+	HIP_TRY(hipMemcpyAsync(h->hscalar + 1, h->dscalar + 1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+	HIP_TRY(hipStreamSynchronize(h->stream));
+	return flow_flag_result(h);
+}
+
 int read_scalar(pgmg_hier *h, double *out) {
-	HIP_TRY(hipMemcpyAsync(h->hscalar, h->dscalar, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+	HIP_TRY(hipMemcpyAsync(h->hscalar, h->dscalar, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
 	HIP_TRY(hipStreamSynchronize(h->stream));
 	*out = *h->hscalar;
-	return PGMG_OK;
+	return flow_flag_result(h);
 }
 
 // Fine residual norm (FP64); optionally write the FP32 residual for the
@@ -554,6 +591,7 @@ int pgmg_hier_create(const pgmg_dims *d, const pgmg_opts *opts, pgmg_hier **out)
 		launch_fill<float>(h->stream, h->res32, (long long)np0, 0.0f);
 	}
 	ALLOC(h->dscalar, 16);
+	launch_fill<double>(h->stream, h->dscalar, 2, 0.0);
 	if (hipHostMalloc((void **)&h->hscalar, 16) != hipSuccess) { pgmg_hier_destroy(h); return fail(PGMG_ERR_MEMORY, "pinned alloc failed"); }
 	rc = ensure_partial(h, residual_partial_count(h->lev[0].v));
 	if (rc) { pgmg_hier_destroy(h); return rc; }
@@ -576,7 +614,7 @@ int pgmg_hier_destroy(pgmg_hier *h) {
 	}
 	hipFree(h->res64); hipFree(h->res32); hipFree(h->partial); hipFree(h->dscalar); hipFree(h->aux64);
 	if (h->hscalar) hipHostFree(h->hscalar);
-	hipFree(h->LU); hipFree(h->ipiv); hipFree(h->lu_info);
+	hipFree(h->LU); hipFree(h->ipiv); hipFree(h->lu_info); hipFree(h->flow_prog);
 	for (auto e : h->prof_ev) hipEventDestroy(e);
 	if (h->stream) hipStreamDestroy(h->stream);
 	delete h;
@@ -670,6 +708,13 @@ int pgmg_compute_rats(pgmg_hier *h) {
 		for (int l = 0; l < h->nl; l++) launch_round_f32(h->stream, h->lev[l].A, h->lev[l].A32, 27 * h->lev[l].v.npad);
 	}
 	if (h->opts.smoother == PGMG_SMOOTH_EXACT) {
+		if (!h->flow_prog) {
+			// one progress counter per plane of the largest level, and the
+			// count of workgroups that have left
+			const long long nc = h->lev[0].v.o + 2;
+			HIP_TRY(hipMalloc(&h->flow_prog, sizeof(int) * (size_t)nc));
+			launch_fill<int>(h->stream, h->flow_prog, nc, 0);
+		}
 		// plane-skewed copies for the exact sweeps of large levels (about 1.5
 		// times a level's size each: skipped, and the level layout swept
 		// instead, when they would not fit)
@@ -788,7 +833,8 @@ int pgmg_get_x(pgmg_hier *h, int level, double *x) {
 	int rc = check_level(h, level);
 	if (rc) return rc;
 	if (!x) return fail(PGMG_ERR_ARG, "null array");
-	return vec_io(h, level, h->lev[level].x, nullptr, x);
+	rc = vec_io(h, level, h->lev[level].x, nullptr, x);
+	return rc ? rc : check_flow(h);
 }
 
 int pgmg_get_operator27(pgmg_hier *h, int level, double *A27) {
@@ -1010,6 +1056,16 @@ int pgmg_x_diff2(pgmg_hier *h, int level, const double *ext_x, double *out) {
 int pgmg_synchronize(pgmg_hier *h) {
 	if (!h) return fail(PGMG_ERR_ARG, "null hierarchy");
 	HIP_TRY(hipStreamSynchronize(h->stream));
+	return check_flow(h);
+}
+
+int pgmg_set_wait_limit(pgmg_hier *h, int polls) {
+	if (!h) return fail(PGMG_ERR_ARG, "null hierarchy");
+	if (polls < 0) return fail(PGMG_ERR_ARG, "negative wait limit");
+	if (polls != h->flow_limit) {
+		free_graphs(h);  // the limit is a launch argument
+		h->flow_limit = polls;
+	}
 	return PGMG_OK;
 }
 

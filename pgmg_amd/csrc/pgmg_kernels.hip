@@ -961,6 +961,186 @@ void launch_gs_planes_skew(hipStream_t s, const SkewView &S, const T *As, T *xs,
 	}
 }
 
+// ---- exact sweeps in one launch: planes signal their progress ----------
+//
+// Workgroup b owns plane k (b renumbered so that each XCD holds a run of
+// consecutive planes) of the plane-skewed copies and walks its rows
+// u = 0 .. umax in order; row u of plane k is the part of hyperplane
+// t = u + 4k in that plane.  The nsweeps forward sweeps of a call run
+// together, sweep s eight rows behind sweep s-1: step tau updates row
+// tau - 8s of every sweep s that has one.  Thread (s, jj) updates point
+// j = jlo(u) + jj of sweep s's row.
+//
+// The stencil of a point in row (k, u) reads
+//   plane k-1, rows u-3 .. u+3: hyperplanes t-7 .. t-1, which must hold
+//     their new values (this sweep's);
+//   plane k, rows u-3 .. u-1 (this sweep, earlier steps here) and
+//     u+1 .. u+3 (the previous sweep's values, written 5 to 7 steps ago);
+//   plane k+1, rows u-3 .. u+3: hyperplanes t+1 .. t+7, which must still
+//     hold the previous sweep's values (the start values for sweep 0).
+// So step tau waits until plane k-1 has finished row min(u+3, umax) of
+// sweep s, and plane k+1 row min(u+3, umax) of sweep s-1.  The converse
+// holds as well: plane k+1 overwrites a row u' in sweep s only after this
+// plane has finished row u'+3 of sweep s, and plane k-1 overwrites row u'
+// in sweep s+1 only after this plane has finished row u'+3 of sweep s, so
+// no value is overwritten before all its readers have read it.  Every point
+// therefore reads the values it reads in the sweeps run one after another in
+// lexicographic order (the hyperplane argument of k_gs_plane): bit-identical
+// results, with the per-point arithmetic of gs_point.
+//
+// prog[k] counts the steps plane k has finished.  A step's x stores are
+// made at agent scope and waited for before the count is published, also
+// at agent scope; x of the neighbouring planes is read at agent scope, so no
+// XCD's L2 hands out a line that another XCD has since written.  The waits
+// form no cycle: with F(s, k, u) = u + 8s + 4k, every wait of an item is for
+// an item of strictly smaller F, so the launch completes whenever all its
+// workgroups are resident together, which the launcher checks.  Every wait
+// is bounded: after `limit` polls it sets *err and its workgroup leaves; the
+// other workgroups leave when they see *err.  The last workgroup out (count
+// in prog[o]) clears the counters for the next launch.
+template <typename T>
+__device__ __forceinline__ T ld_agent(const T *p) {
+	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_agent(T *p, T v) {
+	__hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// s_waitcnt vmcnt(0): every vector memory access of the wave has completed.
+__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Points of the longest row (k, u) of a level: rows of the plane-skewed
+// layout hold the j with i = u - 2j in [0, m).
+__host__ __device__ inline int flow_row_points(const SkewView &S) { return min(S.n, (S.m + 1) / 2); }
+
+// Step dependencies of a step: the counts plane k-1 (*need_b) and plane
+// k+1 (*need_a) must have reached.
+__device__ __forceinline__ void flow_needs(int tau, int nsw, int umax, int *need_b, int *need_a) {
+	int nb = 0, na = 0;
+	for (int s = 0; s < nsw; s++) {
+		const int u = tau - 8 * s;
+		if (u < 0 || u > umax) continue;
+		const int top = min(u + 3, umax);
+		nb = max(nb, top + 8 * s + 1);
+		if (s > 0) na = max(na, top + 8 * (s - 1) + 1);
+	}
+	*need_b = nb;
+	*need_a = na;
+}
+
+// Polls counter *c until it reaches need (known: last value seen).
+// Returns false when the wait gives up (limit polls, or *err set).
+__device__ __forceinline__ bool flow_wait(const int *c, int need, int &known, int *err, int limit) {
+	int spins = 0;
+	while (known < need) {
+		known = ld_agent(c);
+		if (known >= need) break;
+		if (ld_agent(err) != 0) return false;
+		if (++spins > limit) {
+			st_agent(err, 1);
+			return false;
+		}
+		__builtin_amdgcn_s_sleep(1);
+	}
+	return true;
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_gs_flow(SkewView S, const T *__restrict__ As, T *__restrict__ xs,
+						  const T *__restrict__ rs, int nsw, int rpts, int *__restrict__ prog,
+						  int *__restrict__ err, int limit) {
+	__shared__ int stop;
+	const int k = xcd_block((int)blockIdx.x, (int)gridDim.x);
+	const int umax = (S.m - 1) + 2 * (S.n - 1);
+	const int nsteps = umax + 1 + 8 * (nsw - 1);
+	const int tid = (int)threadIdx.x;
+	const int s = tid / rpts, jj = tid - s * rpts;
+	int known_b = 0, known_a = 0;
+	if (tid == 0) stop = 0;
+	__syncthreads();
+	for (int tau = 0; tau < nsteps; tau++) {
+		if (tid == 0) {
+			int nb, na;
+			flow_needs(tau, nsw, umax, &nb, &na);
+			bool ok = true;
+			if (k > 0) ok = flow_wait(prog + k - 1, nb, known_b, err, limit);
+			if (ok && k + 1 < S.o) ok = flow_wait(prog + k + 1, na, known_a, err, limit);
+			if (!ok) stop = 1;
+		}
+		__syncthreads();
+		if (stop) break;
+		const int u = tau - 8 * s;
+		if (s < nsw && u >= 0 && u <= umax) {
+			const int jlo = max(0, (u - (S.m - 1) + 1) >> 1);
+			const int jhi = min(S.n - 1, u >> 1);
+			const int j = jlo + jj;
+			if (j <= jhi) {
+				const int i = u - 2 * j;
+				const long long p = sidx3(S, i, j, k);
+				T ax = (T)0;
+#pragma unroll
+				for (int e = 0; e < 27; e++) {
+					if (e == 13) continue;
+					const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
+					const T a = __builtin_nontemporal_load(As + e * S.size + p);
+					const T *q = xs + sidx3(S, i + di, j + dj, k + dk);
+					const T v = dk == 0 ? *q : ld_agent(q);
+					ax = ax + a * v;
+				}
+				const T a13 = __builtin_nontemporal_load(As + 13 * S.size + p);
+				st_agent(xs + p, (rs[p] - ax) * ((T)1 / a13));
+			}
+		}
+		wait_vm_all();
+		__syncthreads();
+		if (tid == 0) st_agent(prog + k, tau + 1);
+	}
+	__syncthreads();
+	if (tid == 0) {
+		const int old = __hip_atomic_fetch_add(prog + S.o, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (old == (int)gridDim.x - 1)
+			for (int q = 0; q <= S.o; q++) st_agent(prog + q, 0);
+	}
+}
+
+// Sweeps per launch of k_gs_flow for a level (0: the level cannot take it).
+template <typename T>
+int flow_max_sweeps(const SkewView &S) {
+	const int rpts = flow_row_points(S);
+	if (rpts > 1024 || S.o < 1) return 0;
+	const int nsw = min(16, 1024 / rpts);
+	const int threads = ((nsw * rpts + 63) / 64) * 64;
+	static int cus = -1;
+	if (cus < 0) {
+		int dev = 0;
+		hipDeviceProp_t pr;
+		cus = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&pr, dev) == hipSuccess) ? pr.multiProcessorCount : 0;
+	}
+	int per_cu = 0;
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gs_flow<T>, threads, 0) != hipSuccess) {
+		(void)hipGetLastError();
+		return 0;
+	}
+	// every plane's workgroup must be resident at once (see k_gs_flow)
+	if ((long long)per_cu * cus < S.o) return 0;
+	return nsw;
+}
+
+template <typename T>
+bool launch_gs_flow(hipStream_t s, const SkewView &S, const T *As, T *xs, const T *rs, int nsweeps, int *prog,
+		    int *err, int limit) {
+	const int smax = flow_max_sweeps<T>(S);
+	if (smax < 1) return false;
+	const int rpts = flow_row_points(S);
+	for (int done = 0; done < nsweeps; done += smax) {
+		const int nsw = min(smax, nsweeps - done);
+		const int threads = ((nsw * rpts + 63) / 64) * 64;
+		hipLaunchKernelGGL(k_gs_flow<T>, dim3(S.o), dim3(threads), 0, s, S, As, xs, rs, nsw, rpts, prog, err, limit);
+		HIPCHECK_LAUNCH();
+	}
+	return true;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_skew_pack_op(LevelView L, SkewView S, const T *__restrict__ A,
 						      T *__restrict__ As) {
@@ -2925,6 +3105,8 @@ void launch_b_add_x(hipStream_t s, const SlotDev *slots, WorkList wl) {
 	template void launch_gs_plane<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int); \
 	template void launch_gs_planes<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int); \
 	template void launch_gs_planes_skew<T>(hipStream_t, const SkewView &, const T *, T *, const T *, int); \
+	template bool launch_gs_flow<T>(hipStream_t, const SkewView &, const T *, T *, const T *, int, int *, int *, int); \
+	template int flow_max_sweeps<T>(const SkewView &); \
 	template void launch_skew_pack_op<T>(hipStream_t, const LevelView &, const SkewView &, const T *, T *); \
 	template void launch_skew_vec<T>(hipStream_t, const LevelView &, const SkewView &, const T *, T *, bool); \
 	template void launch_skew_halo_wrap<T>(hipStream_t, const LevelView &, const SkewView &, T *); \

@@ -434,3 +434,33 @@ def test_mixed_exact_large_levels_bitwise_vs_oracle(dims):
     assert c_g == c_o == 3
     assert np.array_equal(h.get_x(0), mg.get_x(0))
     h.close()
+
+
+def test_exact_sweep_wait_gives_up_and_recovers():
+    # The host forces the failure path of the one-launch exact sweeps: with
+    # a wait limit of zero polls a plane gives up as soon as its neighbour
+    # is not yet far enough; the launch still ends (every workgroup leaves),
+    # the failure is reported at the next synchronisation, and sweeps with
+    # the default limit are exact again afterwards.
+    m, n, o = 70, 45, 40
+    say("case", (m, n, o))
+    A = O.gen_operator(1, 11, m, n, o)
+    b = O.gen_rhs(11, m, n, o)
+    x0 = O.test_field(m, n, o, 12)
+    h = P.Hierarchy(m, n, o, smoother=P.SMOOTH_EXACT, max_levels=1)
+    h.set_operator27(A)
+    h.compute_rats()
+    h.set_rhs(b)
+    h.set_x(x0)
+    h.set_wait_limit(0)
+    h.gauss_seidel(0, 0, 2)
+    with pytest.raises(P.PgmgError, match="gave up waiting"):
+        h.synchronize()
+    h.synchronize()  # reported once
+    h.set_wait_limit(1 << 21)
+    for rep in range(2):
+        h.set_x(x0)
+        h.gauss_seidel(0, 0, 2)
+        x = O.gs(O.gs(A, x0, b, 0), b, 0)
+        assert np.array_equal(h.get_x(0), x)
+    h.close()
