@@ -40,8 +40,9 @@ const int kGsExact = 24;          // pgmg_gs_exact, common.hh:62
 const int kMaxExact = 2048;       // pgmg_max_exact, common.hh:65
 // Levels with at most this many points are smoothed by one workgroup per
 // call (all sweeps in one launch); larger levels launch per colour/plane.
+// Exact mode keeps the one workgroup only for levels whose x fits in LDS;
+// larger ones take the plane-parallel sweeps (k_gs_flow).
 const long long kSmallColour = 20000;
-const long long kSmallExact = 40000;
 // Largest FP64 skewed copy (27 coefficient planes, x and r) a level gets for
 // the exact smoother; larger levels sweep the level layout.
 const unsigned long long kSkewMaxBytes = 64ULL << 30;
@@ -212,7 +213,7 @@ template <typename T>
 bool gs_use_small(const pgmg_hier *h, const Level &L, int mode) {
 	const bool exact = h->opts.smoother == PGMG_SMOOTH_EXACT;
 	const bool lds = small_lds_bytes(L.v, (int)sizeof(T)) <= kSmallLdsBytes;
-	if (exact) return lds || L.points() <= kSmallExact;
+	if (exact) return lds;
 	if (mode == 0) return lds && L.points() <= 1024;
 	return L.points() <= kSmallColour;
 }
@@ -260,11 +261,11 @@ void op_gs(pgmg_hier *h, int level, int mode, int nsweeps, bool from_zero = fals
 		const bool pair_ok = !(L.v.prd & 1) || !row_lanes(L.v.m, (int)sizeof(T)).ext;
 		if (exact && !mode && sA<T>(L) && h->ops_ready) {
 			// plane-skewed copies, ghosts refreshed on them before each sweep
-			ProfScope ps(h, KC_GS, level, L.points() * nsweeps);
 			launch_skew_vec<T>(h->stream, L.v, L.sv, lx<T>(L), sx<T>(L), true);
 			launch_skew_vec<T>(h->stream, L.v, L.sv, lr<T>(L), sr<T>(L), true);
 			for (int s = 0; s < nsweeps; s++) {
 				launch_skew_halo_wrap<T>(h->stream, L.v, L.sv, sx<T>(L));
+				ProfScope ps(h, KC_GS, level, L.points());
 				skew_sweeps<T>(h, L, 1);
 			}
 			launch_skew_vec<T>(h->stream, L.v, L.sv, sx<T>(L), lx<T>(L), false);
@@ -295,17 +296,21 @@ void op_gs(pgmg_hier *h, int level, int mode, int nsweeps, bool from_zero = fals
 	const int tmax = (L.v.m - 1) + 2 * (L.v.n - 1) + 4 * (L.v.o - 1);
 	if (exact && !mode) {
 		// forward sweeps of one call, pipelined eight hyperplanes apart
-		ProfScope ps(h, KC_GS, level, L.points() * nsweeps);
 		// the copies match the operator from pgmg_compute_rats until an
 		// operator is set again (ops_ready false)
 		if (sA<T>(L) && h->ops_ready) {
-			// on the plane-skewed copies: x and r in, x out
+			// on the plane-skewed copies: x and r in, x out (the timing
+			// hook brackets the sweeps alone)
 			launch_skew_vec<T>(h->stream, L.v, L.sv, lx<T>(L), sx<T>(L), true);
 			launch_skew_vec<T>(h->stream, L.v, L.sv, lr<T>(L), sr<T>(L), true);
-			skew_sweeps<T>(h, L, nsweeps);
+			{
+				ProfScope ps(h, KC_GS, level, L.points() * nsweeps);
+				skew_sweeps<T>(h, L, nsweeps);
+			}
 			launch_skew_vec<T>(h->stream, L.v, L.sv, sx<T>(L), lx<T>(L), false);
 			return;
 		}
+		ProfScope ps(h, KC_GS, level, L.points() * nsweeps);
 		launch_gs_planes<T>(h->stream, L.v, lA<T>(L), lx<T>(L), lr<T>(L), nsweeps);
 		return;
 	}
@@ -398,6 +403,9 @@ int flow_flag_result(pgmg_hier *h) {
 	memcpy(&e, h->hscalar + 1, sizeof(int));
 	if (!e) return PGMG_OK;
 	launch_fill<int>(h->stream, flow_err(h), 1, 0);
+	// the failed launch's last workgroup cleared the counters; clear them
+	// here as well before the next sweep
+	if (h->flow_prog) launch_fill<int>(h->stream, h->flow_prog, h->lev[0].v.o + 2, 0);
 	hipStreamSynchronize(h->stream);
 	return fail(PGMG_ERR_HIP, "exact sweep: a plane gave up waiting for its neighbour after %d polls; "
 		    "the level's x is incomplete", h->flow_limit);
@@ -513,8 +521,7 @@ int pgmg_hier_footprint(const pgmg_dims *d, const pgmg_opts *opts, uint64_t *byt
 		// plane-skewed copies of the levels the one-workgroup kernel does not take
 		for (int l = 0; l < nl; l++) {
 			const LevelView v = make_level_view(dims[3 * l], dims[3 * l + 1], dims[3 * l + 2]);
-			const long long pts = (long long)v.m * v.n * v.o;
-			if (small_lds_bytes(v, 8) <= kSmallLdsBytes || pts <= kSmallExact) continue;
+			if (small_lds_bytes(v, 8) <= kSmallLdsBytes) continue;
 			const unsigned long long n = (unsigned long long)make_skew_view(v).size;
 			if (29ULL * n * 8 > kSkewMaxBytes) continue;
 			b += (uint64_t)n * 29 * ((opts->precision == PGMG_PREC_MIXED) ? 12 : 8);

@@ -52,7 +52,7 @@ inline LevelView make_level_view(int m, int n, int o) {
 //   (u + 3)*R + (j - jf(u)) + pk*(k + 1),
 // so the points of one hyperplane i + 2j + 4k = t with the same k lie side
 // by side in j, and so do each of their neighbours.  Positions that are not
-// grid points hold zero.
+// grid points hold zero.  Coefficients: see sarow.
 struct SkewView {
 	int m, n, o;
 	int R;             // row length
@@ -74,6 +74,19 @@ __host__ __device__ inline long long sidx3(const SkewView &S, int i, int j, int 
 	const int u = i + 2 * j;
 	const int jf = S.cmp ? max(-1, (u - S.m + 1) >> 1) : -1;  // ceil((u - m) / 2), at least -1
 	return (long long)(u + 3) * S.R + (j - jf) + S.pk * (k + 1);
+}
+// The 27 coefficient planes of a skewed copy are interleaved by row: row
+// (u, k) of plane e sits at ((row(u, k) * 27 + e) * R, so the coefficients of
+// one row of points are one contiguous block of 27 R values (a few pages,
+// not 27 rows a plane apart, which a sweep over every plane at once would
+// spread over thousands of pages).
+__host__ __device__ inline long long sarow(const SkewView &S, int u, int k) {
+	return ((long long)(u + 3) + (long long)(S.m + 2 * S.n + 4) * (k + 1)) * 27 * S.R;
+}
+__host__ __device__ inline long long saidx3(const SkewView &S, int e, int i, int j, int k) {
+	const int u = i + 2 * j;
+	const int jf = S.cmp ? max(-1, (u - S.m + 1) >> 1) : -1;
+	return sarow(S, u, k) + (long long)e * S.R + (j - jf);
 }
 
 // Point (i,j,k) of a per-point array.
@@ -215,7 +228,7 @@ template <typename T> void launch_halo_wrap(hipStream_t s, const LevelView &L, T
 // Gauss-Seidel on the hyperplane i+2j+4k = t.
 template <typename T> void launch_gs_plane(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int t);
 // Plane-skewed copies for the exact smoother: the 27 coefficient planes
-// (As[e*S.size + ...]) and one vector, both ways.
+// (As[saidx3(S, e, i, j, k)]) and one vector, both ways.
 template <typename T> void launch_skew_pack_op(hipStream_t s, const LevelView &L, const SkewView &S, const T *A, T *As);
 template <typename T> void launch_skew_vec(hipStream_t s, const LevelView &L, const SkewView &S, const T *v, T *vs,
 					   bool to_skew);

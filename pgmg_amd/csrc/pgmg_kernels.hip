@@ -943,9 +943,9 @@ __global__ __launch_bounds__(256) void k_gs_planes_skew(SkewView S, const T *__r
 	for (int e = 0; e < 27; e++) {
 		if (e == 13) continue;
 		const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
-		ax = ax + As[e * S.size + p] * xs[sidx3(S, i + di, j + dj, k + dk)];
+		ax = ax + As[saidx3(S, e, i, j, k)] * xs[sidx3(S, i + di, j + dj, k + dk)];
 	}
-	xs[p] = (rs[p] - ax) * ((T)1 / As[13 * S.size + p]);
+	xs[p] = (rs[p] - ax) * ((T)1 / As[saidx3(S, 13, i, j, k)]);
 }
 
 template <typename T>
@@ -967,15 +967,16 @@ void launch_gs_planes_skew(hipStream_t s, const SkewView &S, const T *As, T *xs,
 // consecutive planes) of the plane-skewed copies and walks its rows
 // u = 0 .. umax in order; row u of plane k is the part of hyperplane
 // t = u + 4k in that plane.  The nsweeps forward sweeps of a call run
-// together, sweep s eight rows behind sweep s-1: step tau updates row
-// tau - 8s of every sweep s that has one.  Thread (s, jj) updates point
-// j = jlo(u) + jj of sweep s's row.
+// together, sweep s kFlowLag rows behind sweep s-1: step tau updates row
+// tau - kFlowLag*s of every sweep s that has one.  Thread (s, jj) updates
+// point j = jlo(u) + jj of sweep s's row.
 //
 // The stencil of a point in row (k, u) reads
 //   plane k-1, rows u-3 .. u+3: hyperplanes t-7 .. t-1, which must hold
 //     their new values (this sweep's);
 //   plane k, rows u-3 .. u-1 (this sweep, earlier steps here) and
-//     u+1 .. u+3 (the previous sweep's values, written 5 to 7 steps ago);
+//     u+1 .. u+3 (the previous sweep's values, written kFlowLag-3 or more
+//     steps ago);
 //   plane k+1, rows u-3 .. u+3: hyperplanes t+1 .. t+7, which must still
 //     hold the previous sweep's values (the start values for sweep 0).
 // So step tau waits until plane k-1 has finished row min(u+3, umax) of
@@ -988,16 +989,29 @@ void launch_gs_planes_skew(hipStream_t s, const SkewView &S, const T *As, T *xs,
 // lexicographic order (the hyperplane argument of k_gs_plane): bit-identical
 // results, with the per-point arithmetic of gs_point.
 //
-// prog[k] counts the steps plane k has finished.  A step's x stores are
-// made at agent scope and waited for before the count is published, also
-// at agent scope; x of the neighbouring planes is read at agent scope, so no
-// XCD's L2 hands out a line that another XCD has since written.  The waits
-// form no cycle: with F(s, k, u) = u + 8s + 4k, every wait of an item is for
-// an item of strictly smaller F, so the launch completes whenever all its
-// workgroups are resident together, which the launcher checks.  Every wait
-// is bounded: after `limit` polls it sets *err and its workgroup leaves; the
-// other workgroups leave when they see *err.  The last workgroup out (count
-// in prog[o]) clears the counters for the next launch.
+// A step: wait (from counts read one step earlier where they suffice, else
+// by polling), load every x value the step's points read, wait for all of
+// the wave's memory accesses, publish that the previous step is done, store
+// the new values, and load the coefficients and r of the next step's points
+// (they depend on no other plane).  One wavefront of the workgroup does the
+// waiting and publishing and nothing else, so its polls never queue behind
+// those loads.  prog[k] counts the finished steps.  The count is published
+// one step late because that costs no wait of its own: the previous step's
+// stores were issued before this step's loads, so once those loads are back
+// and vmcnt is 0 the stores are complete at agent scope, in every wave of
+// the workgroup after the barrier.  x stores and the loads of the
+// neighbouring planes are agent-scope (sc1), so no XCD's L2 hands out a line
+// that another XCD has since written.
+//
+// The waits form no cycle: with the late count, with F(k, tau) = tau + 5k
+// every wait of a step is for a step of strictly smaller F when
+// kFlowLag >= 15, so the launch completes whenever all its workgroups are
+// resident together, which the launcher checks.  Every wait is bounded:
+// after `limit` polls it sets *err and its workgroup leaves; the other
+// workgroups leave when they see *err.  The last workgroup out (count in
+// prog[o]) clears the counters for the next launch.
+const int kFlowLag = 16;
+
 template <typename T>
 __device__ __forceinline__ T ld_agent(const T *p) {
 	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1009,20 +1023,57 @@ __device__ __forceinline__ void st_agent(T *p, T v) {
 // s_waitcnt vmcnt(0): every vector memory access of the wave has completed.
 __device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+// Loads and stores of x in k_gs_flow: raw buffer accesses (32-bit byte
+// offsets; the launcher keeps each skewed array under 2 GiB), with the
+// agent-scope cache policy (sc1) where another workgroup writes or reads the
+// value.  Unlike atomic loads, the compiler issues these together, so a
+// point's 26 neighbours cost one memory round trip, not 26.
+constexpr int kLoadSc1 = 16;
+template <typename T>
+__device__ __forceinline__ T ld_x(const __amdgpu_buffer_rsrc_t &rs, int off, bool agent) {
+	if constexpr (sizeof(T) == 8) {
+		typedef unsigned u2v __attribute__((ext_vector_type(2)));
+		const u2v w = agent ? __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, kLoadSc1)
+				    : __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
+		T v;
+		__builtin_memcpy(&v, &w, 8);
+		return v;
+	} else {
+		const unsigned w = agent ? __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, kLoadSc1)
+					 : __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
+		T v;
+		__builtin_memcpy(&v, &w, 4);
+		return v;
+	}
+}
+template <typename T>
+__device__ __forceinline__ void st_x(const __amdgpu_buffer_rsrc_t &rs, int off, T v) {
+	if constexpr (sizeof(T) == 8) {
+		typedef unsigned u2v __attribute__((ext_vector_type(2)));
+		u2v w;
+		__builtin_memcpy(&w, &v, 8);
+		__builtin_amdgcn_raw_buffer_store_b64(w, rs, off, 0, kStoreSc1);
+	} else {
+		unsigned w;
+		__builtin_memcpy(&w, &v, 4);
+		__builtin_amdgcn_raw_buffer_store_b32(w, rs, off, 0, kStoreSc1);
+	}
+}
+
 // Points of the longest row (k, u) of a level: rows of the plane-skewed
 // layout hold the j with i = u - 2j in [0, m).
 __host__ __device__ inline int flow_row_points(const SkewView &S) { return min(S.n, (S.m + 1) / 2); }
 
-// Step dependencies of a step: the counts plane k-1 (*need_b) and plane
-// k+1 (*need_a) must have reached.
+// The counts plane k-1 (*need_b) and plane k+1 (*need_a) must have reached
+// before step tau.
 __device__ __forceinline__ void flow_needs(int tau, int nsw, int umax, int *need_b, int *need_a) {
 	int nb = 0, na = 0;
 	for (int s = 0; s < nsw; s++) {
-		const int u = tau - 8 * s;
+		const int u = tau - kFlowLag * s;
 		if (u < 0 || u > umax) continue;
 		const int top = min(u + 3, umax);
-		nb = max(nb, top + 8 * s + 1);
-		if (s > 0) na = max(na, top + 8 * (s - 1) + 1);
+		nb = max(nb, top + kFlowLag * s + 1);
+		if (s > 0) na = max(na, top + kFlowLag * (s - 1) + 1);
 	}
 	*need_b = nb;
 	*need_a = na;
@@ -1045,21 +1096,55 @@ __device__ __forceinline__ bool flow_wait(const int *c, int need, int &known, in
 	return true;
 }
 
+// Block size of k_gs_flow: rpad = the longest row rounded up to whole
+// wavefronts, per sweep in flight, so that each wavefront holds a segment of
+// one sweep's row (its row bases are wave-uniform), and one wavefront more.
+__host__ __device__ inline int flow_row_pad(const SkewView &S) { return ((flow_row_points(S) + 63) / 64) * 64; }
+const int kFlowMaxThreads = 512;
+const int kFlowMaxPlanes = 160;
+
 template <typename T>
-__global__ __launch_bounds__(1024) void k_gs_flow(SkewView S, const T *__restrict__ As, T *__restrict__ xs,
-						  const T *__restrict__ rs, int nsw, int rpts, int *__restrict__ prog,
-						  int *__restrict__ err, int limit) {
+__global__ __launch_bounds__(kFlowMaxThreads) void k_gs_flow(SkewView S, const T *__restrict__ As, T *__restrict__ xs,
+							     const T *__restrict__ rs, int nsw, int rpad,
+							     int *__restrict__ prog, int *__restrict__ err, int limit) {
 	__shared__ int stop;
 	const int k = xcd_block((int)blockIdx.x, (int)gridDim.x);
 	const int umax = (S.m - 1) + 2 * (S.n - 1);
-	const int nsteps = umax + 1 + 8 * (nsw - 1);
+	const int nsteps = umax + 1 + kFlowLag * (nsw - 1);
 	const int tid = (int)threadIdx.x;
-	const int s = tid / rpts, jj = tid - s * rpts;
-	int known_b = 0, known_a = 0;
+	// waves 0 .. nsw*rpad/64 - 1 update points; the last wave (the watcher)
+	// waits for and publishes the counts, with no loads of its own in flight
+	const int s = __builtin_amdgcn_readfirstlane(tid / rpad), jj = tid - s * rpad;
+	const bool watcher = s >= nsw;
+	const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(xs, 0, 0x7fffffff, kBufferWord3);
+	int known_b = 0, known_a = 0, seen_b = 0, seen_a = 0;
+	// this thread's point of a step (if any), and its coefficients and r,
+	// loaded a step ahead
+	T a[27], rv = (T)0;
+	auto point = [&](int t, int &i, int &j) -> bool {
+		const int u = t - kFlowLag * s;
+		if (watcher || u < 0 || u > umax) return false;
+		j = max(0, (u - (S.m - 1) + 1) >> 1) + jj;
+		i = u - 2 * j;
+		return j <= min(S.n - 1, u >> 1);
+	};
+	auto fetch = [&](int t) {
+		int i, j;
+		if (!point(t, i, j)) return;
+		const T *Ar = As + saidx3(S, 0, i, j, k);
+#pragma unroll
+		for (int e = 0; e < 27; e++) a[e] = Ar[e * S.R];
+		rv = rs[sidx3(S, i, j, k)];
+	};
+	fetch(0);
 	if (tid == 0) stop = 0;
 	__syncthreads();
-	for (int tau = 0; tau < nsteps; tau++) {
-		if (tid == 0) {
+	int tau = 0;
+	for (; tau < nsteps; tau++) {
+		if (watcher && jj == 0) {
+			// counts read during the previous step, then polls if short
+			known_b = max(known_b, seen_b);
+			known_a = max(known_a, seen_a);
 			int nb, na;
 			flow_needs(tau, nsw, umax, &nb, &na);
 			bool ok = true;
@@ -1069,60 +1154,96 @@ __global__ __launch_bounds__(1024) void k_gs_flow(SkewView S, const T *__restric
 		}
 		__syncthreads();
 		if (stop) break;
-		const int u = tau - 8 * s;
-		if (s < nsw && u >= 0 && u <= umax) {
-			const int jlo = max(0, (u - (S.m - 1) + 1) >> 1);
-			const int jhi = min(S.n - 1, u >> 1);
-			const int j = jlo + jj;
-			if (j <= jhi) {
-				const int i = u - 2 * j;
-				const long long p = sidx3(S, i, j, k);
-				T ax = (T)0;
-#pragma unroll
-				for (int e = 0; e < 27; e++) {
-					if (e == 13) continue;
-					const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
-					const T a = __builtin_nontemporal_load(As + e * S.size + p);
-					const T *q = xs + sidx3(S, i + di, j + dj, k + dk);
-					const T v = dk == 0 ? *q : ld_agent(q);
-					ax = ax + a * v;
-				}
-				const T a13 = __builtin_nontemporal_load(As + 13 * S.size + p);
-				st_agent(xs + p, (rs[p] - ax) * ((T)1 / a13));
-			}
+		if (watcher && jj == 0) {
+			// for the next step's wait, in flight with this step's loads
+			if (k > 0) seen_b = ld_agent(prog + k - 1);
+			if (k + 1 < S.o) seen_a = ld_agent(prog + k + 1);
 		}
+		int i = 0, j = 0;
+		const bool live = point(tau, i, j);
+		long long p = 0;
+		T val = (T)0;
+		if (live) {
+			p = sidx3(S, i, j, k);
+			T v[27];
+#pragma unroll
+			for (int e = 0; e < 27; e++) {
+				if (e == 13) continue;
+				const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
+				v[e] = ld_x<T>(xr, (int)(sidx3(S, i + di, j + dj, k + dk) * (long long)sizeof(T)), dk != 0);
+			}
+			T ax = (T)0;
+#pragma unroll
+			for (int e = 0; e < 27; e++) {
+				if (e == 13) continue;
+				ax = ax + a[e] * v[e];
+			}
+			val = (rv - ax) * ((T)1 / a[13]);
+		}
+		// this step's loads are back, so the previous step's stores (issued
+		// before them) are complete, in every wave after the barrier
 		wait_vm_all();
 		__syncthreads();
-		if (tid == 0) st_agent(prog + k, tau + 1);
+		if (watcher && jj == 0 && tau > 0) st_agent(prog + k, tau);
+		if (live) st_x<T>(xr, (int)(p * (long long)sizeof(T)), val);
+		fetch(tau + 1);
 	}
+	wait_vm_all();
 	__syncthreads();
-	if (tid == 0) {
+	if (watcher && jj == 0) {
+		if (tau == nsteps) st_agent(prog + k, nsteps);
+		// this plane's last count must be in place before it counts itself
+		// out, or it could land after the last workgroup's reset
+		wait_vm_all();
 		const int old = __hip_atomic_fetch_add(prog + S.o, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		if (old == (int)gridDim.x - 1)
 			for (int q = 0; q <= S.o; q++) st_agent(prog + q, 0);
 	}
 }
 
-// Sweeps per launch of k_gs_flow for a level (0: the level cannot take it).
+struct FlowOccupancy {
+	int cus;
+	int per_cu[17];  // resident workgroups per CU of 64*w threads
+};
+template <typename T>
+FlowOccupancy flow_occupancy() {
+	FlowOccupancy f;
+	int dev = 0;
+	hipDeviceProp_t pr;
+	f.cus = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&pr, dev) == hipSuccess) ? pr.multiProcessorCount : 0;
+	f.per_cu[0] = 0;
+	for (int w = 1; w <= 16; w++) {
+		if (64 * w > kFlowMaxThreads ||
+		    hipOccupancyMaxActiveBlocksPerMultiprocessor(&f.per_cu[w], k_gs_flow<T>, 64 * w, 0) != hipSuccess) {
+			f.per_cu[w] = 0;
+			(void)hipGetLastError();
+		}
+	}
+	return f;
+}
+
+// Most sweeps per launch of k_gs_flow for a level (0: the level cannot
+// take it): the block must fit kFlowMaxThreads, and every plane's workgroup
+// must be resident at once (see k_gs_flow).
 template <typename T>
 int flow_max_sweeps(const SkewView &S) {
-	const int rpts = flow_row_points(S);
-	if (rpts > 1024 || S.o < 1) return 0;
-	const int nsw = min(16, 1024 / rpts);
-	const int threads = ((nsw * rpts + 63) / 64) * 64;
-	static int cus = -1;
-	if (cus < 0) {
-		int dev = 0;
-		hipDeviceProp_t pr;
-		cus = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&pr, dev) == hipSuccess) ? pr.multiProcessorCount : 0;
-	}
-	int per_cu = 0;
-	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gs_flow<T>, threads, 0) != hipSuccess) {
-		(void)hipGetLastError();
-		return 0;
-	}
-	// every plane's workgroup must be resident at once (see k_gs_flow)
-	if ((long long)per_cu * cus < S.o) return 0;
+	const int rpad = flow_row_pad(S);
+	if (rpad + 64 > kFlowMaxThreads || S.o < 1) return 0;
+	// x is addressed with 32-bit byte offsets
+	if (S.size * (long long)sizeof(T) >= (1LL << 31)) return 0;
+	// Each plane trails the one below by about five steps (four rows of the
+	// hyperplane offset t = u + 4k, one for the count published a step
+	// late), so a level of o planes takes about umax + 5(o - 1) steps of a
+	// few microseconds each.  Up to kFlowMaxPlanes planes that beats one
+	// launch per hyperplane; beyond it (level 0 of 257^3: 11.4 ms per
+	// two-sweep call against 10.1 ms) the hyperplane launches are kept.
+	if (S.o > kFlowMaxPlanes) return 0;
+	// resident workgroups of each block size, queried once (the queries
+	// take long against a sweep); initialised once even with several
+	// host threads
+	static const FlowOccupancy occ = flow_occupancy<T>();
+	int nsw = min(16, (kFlowMaxThreads - 64) / rpad);
+	while (nsw > 0 && (long long)occ.per_cu[nsw * rpad / 64 + 1] * occ.cus < S.o) nsw--;
 	return nsw;
 }
 
@@ -1131,11 +1252,11 @@ bool launch_gs_flow(hipStream_t s, const SkewView &S, const T *As, T *xs, const 
 		    int *err, int limit) {
 	const int smax = flow_max_sweeps<T>(S);
 	if (smax < 1) return false;
-	const int rpts = flow_row_points(S);
+	const int rpad = flow_row_pad(S);
 	for (int done = 0; done < nsweeps; done += smax) {
 		const int nsw = min(smax, nsweeps - done);
-		const int threads = ((nsw * rpts + 63) / 64) * 64;
-		hipLaunchKernelGGL(k_gs_flow<T>, dim3(S.o), dim3(threads), 0, s, S, As, xs, rs, nsw, rpts, prog, err, limit);
+		hipLaunchKernelGGL(k_gs_flow<T>, dim3(S.o), dim3(nsw * rpad + 64), 0, s, S, As, xs, rs, nsw, rpad, prog, err,
+				   limit);
 		HIPCHECK_LAUNCH();
 	}
 	return true;
@@ -1149,7 +1270,7 @@ __global__ __launch_bounds__(256) void k_skew_pack_op(LevelView L, SkewView S, c
 		const int e = (int)(q / N);
 		const long long p = q - (long long)e * N;
 		const int i = (int)(p % L.m), j = (int)((p / L.m) % L.n), k = (int)(p / ((long long)L.m * L.n));
-		As[e * S.size + sidx3(S, i, j, k)] = A[aidx3(L, e, i, j, k)];
+		As[saidx3(S, e, i, j, k)] = A[aidx3(L, e, i, j, k)];
 	}
 }
 

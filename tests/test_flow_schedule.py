@@ -2,12 +2,18 @@
 (k_gs_flow in pgmg_amd/csrc/pgmg_kernels.hip).
 
 Each k-plane is a workgroup that walks steps tau = 0, 1, ...; step tau
-updates row u = tau - 8s (u = i + 2j) of plane k for every sweep s in
-flight, once plane k-1 has finished row min(u+3, umax) of sweep s and plane
-k+1 row min(u+3, umax) of sweep s-1 (progress counted in steps).  Here the
-planes advance in random interleavings, one whole step at a time (no step
-that may run at the same time as another writes a row that step reads), and
-the result must equal nsw lexicographic sweeps of the oracle bit for bit;
+updates row u = tau - LAG*s (u = i + 2j) of plane k for every sweep s in
+flight.  A step has four phases, and the planes interleave at phase
+granularity in random orders:
+  wait    until plane k-1 has published row min(u+3, umax) of sweep s as
+          done, and plane k+1 row min(u+3, umax) of sweep s-1;
+  read    every neighbour value the step's points use (one snapshot);
+  publish that the previous step is done (its stores are in memory: the
+          kernel knows it once this step's loads, issued after them, have
+          completed);
+  write   the new values.
+The counts are published one step late, and published after the last step.
+The result must equal nsw lexicographic sweeps of the oracle bit for bit, and
 no interleaving may leave every unfinished plane waiting."""
 import random
 
@@ -16,16 +22,18 @@ import pytest
 
 from oracle import oracle as O
 
+LAG = 16  # rows between consecutive sweeps in flight (kFlowLag)
 
-def needs(tau, nsw, umax):
+
+def needs(tau, nsw, umax, lag=LAG):
     nb = na = 0
     for s in range(nsw):
-        u = tau - 8 * s
+        u = tau - lag * s
         if 0 <= u <= umax:
             top = min(u + 3, umax)
-            nb = max(nb, top + 8 * s + 1)
+            nb = max(nb, top + lag * s + 1)
             if s > 0:
-                na = max(na, top + 8 * (s - 1) + 1)
+                na = max(na, top + lag * (s - 1) + 1)
     return nb, na
 
 
@@ -41,32 +49,69 @@ def gs_point(A, x, r, i, j, k):
     x[k, j, i] = (float(r[k, j, i]) - ax) * (1.0 / float(A[k, j, i, 13]))
 
 
-def simulate(A, x0, r, nsw, rng):
+def gs_value(A, x, r, i, j, k):
+    """gs_point's new value for (i, j, k), without writing it."""
+    o, n, m = x.shape
+    ax = 0.0
+    for e in range(27):
+        if e == 13:
+            continue
+        qi, qj, qk = i + e % 3 - 1, j + (e // 3) % 3 - 1, k + e // 9 - 1
+        if 0 <= qi < m and 0 <= qj < n and 0 <= qk < o:
+            ax = ax + float(A[k, j, i, e]) * float(x[qk, qj, qi])
+    return (float(r[k, j, i]) - ax) * (1.0 / float(A[k, j, i, 13]))
+
+
+def simulate(A, x0, r, nsw, rng, lag=LAG, need=None):
+    need = need or needs
     o, n, m = x0.shape
     umax = (m - 1) + 2 * (n - 1)
-    nsteps = umax + 1 + 8 * (nsw - 1)
+    nsteps = umax + 1 + lag * (nsw - 1)
     x = x0.copy()
-    prog = [0] * o
+    pub = [0] * o           # published counts
+    step = [0] * o          # step each plane is in
+    phase = ["wait"] * o    # wait -> read -> publish -> write -> (next step)
+    vals = [None] * o
     while True:
         ready = []
         for k in range(o):
-            if prog[k] == nsteps:
+            if phase[k] == "done":
                 continue
-            nb, na = needs(prog[k], nsw, umax)
-            if (k == 0 or prog[k - 1] >= nb) and (k == o - 1 or prog[k + 1] >= na):
-                ready.append(k)
+            if phase[k] == "wait":
+                nb, na = need(step[k], nsw, umax, lag)
+                if not ((k == 0 or pub[k - 1] >= nb) and (k == o - 1 or pub[k + 1] >= na)):
+                    continue
+            ready.append(k)
         if not ready:
-            assert all(p == nsteps for p in prog), "every unfinished plane waits: %s" % prog
+            assert all(p == "done" for p in phase), "every unfinished plane waits: %s %s" % (step, pub)
             return x
-        k = rng.choice(ready)
-        tau = prog[k]
-        for s in range(nsw):
-            u = tau - 8 * s
-            if not 0 <= u <= umax:
-                continue
-            for j in range(max(0, (u - (m - 1) + 1) >> 1), min(n - 1, u >> 1) + 1):
-                gs_point(A, x, r, u - 2 * j, j, k)
-        prog[k] = tau + 1
+        # mostly the highest plane that may move: upper planes then read as
+        # early as the waits allow, where a missing wait would show
+        k = max(ready) if rng.random() < 0.7 else rng.choice(ready)
+        tau = step[k]
+        if phase[k] == "wait":
+            phase[k] = "read"
+        elif phase[k] == "read":
+            v = []
+            for s in range(nsw):
+                u = tau - lag * s
+                if 0 <= u <= umax:
+                    for j in range(max(0, (u - (m - 1) + 1) >> 1), min(n - 1, u >> 1) + 1):
+                        v.append((u - 2 * j, j, gs_value(A, x, r, u - 2 * j, j, k)))
+            vals[k] = v
+            phase[k] = "publish"
+        elif phase[k] == "publish":
+            pub[k] = tau
+            phase[k] = "write"
+        else:
+            for (i, j, val) in vals[k]:
+                x[k, j, i] = val
+            step[k] = tau + 1
+            if step[k] == nsteps:
+                pub[k] = nsteps
+                phase[k] = "done"
+            else:
+                phase[k] = "wait"
 
 
 @pytest.mark.parametrize("dims,nsw,seed", [((9, 7, 6), 1, 1), ((9, 7, 6), 2, 2), ((13, 10, 11), 3, 3),
@@ -95,15 +140,24 @@ def test_flow_schedule_catches_a_short_wait():
     x0 = rng.uniform(-1, 1, size=(o, n, m))
     r = rng.uniform(-1, 1, size=(o, n, m))
     ref = O.gs(A, x0, r, 0)
-    global needs
-    orig = needs
-
-    def short(tau, nsw, umax):
-        nb, na = orig(tau, nsw, umax)
+    def short(tau, nsw, umax, lag):
+        nb, na = needs(tau, nsw, umax, lag)
         return max(0, nb - 1), na
-    needs = short
-    try:
-        bad = any(not np.array_equal(simulate(A, x0, r, 1, random.Random(t)), ref) for t in range(20))
-    finally:
-        needs = orig
-    assert bad
+    assert any(not np.array_equal(simulate(A, x0, r, 1, random.Random(t), need=short), ref) for t in range(40))
+
+
+def test_flow_schedule_needs_the_sweep_lag():
+    """With sweeps in flight only eight rows apart the late publication could
+    leave two neighbouring planes each waiting for the other."""
+    m, n, o = 9, 7, 6
+    A = O.gen_operator(1, 61, m, n, o)
+    rng = np.random.default_rng(9)
+    x0 = rng.uniform(-1, 1, size=(o, n, m))
+    r = rng.uniform(-1, 1, size=(o, n, m))
+    stuck = 0
+    for t in range(20):
+        try:
+            simulate(A, x0, r, 3, random.Random(t), lag=8)
+        except AssertionError as e:
+            stuck += "waits" in str(e)
+    assert stuck > 0
