@@ -170,6 +170,16 @@ int pgmg_copy_rhs_to_x(pgmg_hier *h, int level);
 int pgmg_rhs_x_dot(pgmg_hier *h, int level, double *out);
 /* region::soln_diff, region.hh:226-236: sum of (ext_x - x)^2. */
 int pgmg_x_diff2(pgmg_hier *h, int level, const double *ext_x, double *out);
+/* No counterpart in the reference: pgmg_x_diff2 for callers that cannot
+ * allocate device memory (code under graph capture, tight serving or PCG
+ * loops, harnesses that forbid hipMalloc).  It allocates nothing: the
+ * caller's vector, the block sums and the result go through one device
+ * array of 8 Mi doubles (64 MiB) that each GPU the library runs on holds
+ * until the process ends.  Takes levels with points + block sums + 1 <=
+ * 8 Mi doubles (about 8.3 M points; PGMG_ERR_ARG beyond); the result is bit
+ * for bit pgmg_x_diff2's.  Calls on one GPU share that array: calls that
+ * overlap on one GPU (several host threads) race. */
+int pgmg_x_diff2_noalloc(pgmg_hier *h, int level, const double *ext_x, double *out);
 /* Solve to a relative residual: V-cycles (FP64) or FP64 defect correction
  * around FP32 V-cycles (MIXED) from the current x until
  * sqrt(l2_error_all/rhs_l2norm) < tol.  hist (max_cycles+1 doubles, may be

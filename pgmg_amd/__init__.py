@@ -80,6 +80,7 @@ _SIGS = {
     "pgmg_copy_rhs_to_x": (_i, [_VP, _i]),
     "pgmg_rhs_x_dot": (_i, [_VP, _i, _D]),
     "pgmg_x_diff2": (_i, [_VP, _i, _D, _D]),
+    "pgmg_x_diff2_noalloc": (_i, [_VP, _i, _D, _D]),
     "pgmg_synchronize": (_i, [_VP]),
     "pgmg_set_wait_limit": (_i, [_VP, _i]),
     "pgmg_gs_count": (_i, [_VP, ctypes.POINTER(ctypes.c_longlong)]),
@@ -293,6 +294,15 @@ class Hierarchy:
         assert y.size == int(np.prod(self.shape(level)))
         v = ctypes.c_double()
         _check(lib().pgmg_x_diff2(self.h, level, _dptr(y), ctypes.byref(v)))
+        return v.value
+
+    def x_diff2_noalloc(self, y, level=0):
+        """x_diff2 through the allocation-free entry point (levels of up to
+        about 8.3 M points)."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        assert y.size == int(np.prod(self.shape(level)))
+        v = ctypes.c_double()
+        _check(lib().pgmg_x_diff2_noalloc(self.h, level, _dptr(y), ctypes.byref(v)))
         return v.value
 
     def synchronize(self):

@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1058,6 +1059,61 @@ int pgmg_x_diff2(pgmg_hier *h, int level, const double *ext_x, double *out) {
 	launch_pair_sum(h->stream, L.v, dext, L.x, 1, h->partial);
 	launch_reduce(h->stream, h->partial, n, h->dscalar);
 	return read_scalar(h, out);
+}
+
+// ---- pgmg_x_diff2_noalloc: no device allocation ----
+// Device memory of the entry point: the caller's vector (point-major), the
+// block sums and the result, in one array that every GPU the library runs
+// on holds from the first use until the process ends.  Host code reaches it
+// only through the address hipGetSymbolAddress returns for the current
+// device (the array's name is a host-side stand-in).
+const long long kScratchDoubles = 8LL << 20;  // 64 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+const int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kScratchDevices] = {};
+
+// The scratch array's address on device `dev`, looked up once per device
+// with that device current.
+static int scratch_on(int dev, double **out) {
+	if (dev < 0 || dev >= kScratchDevices) return fail(PGMG_ERR_ARG, "device %d beyond the scratch table", dev);
+	std::lock_guard<std::mutex> lock(g_scratch_mu);
+	if (!g_scratch_ptr[dev]) {
+		int cur = -1;
+		HIP_TRY(hipGetDevice(&cur));
+		if (cur != dev) HIP_TRY(hipSetDevice(dev));
+		void *p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (cur != dev) (void)hipSetDevice(cur);
+		if (e != hipSuccess || !p) return fail(PGMG_ERR_HIP, "scratch address: %s", hipGetErrorString(e));
+		g_scratch_ptr[dev] = (double *)p;
+	}
+	*out = g_scratch_ptr[dev];
+	return PGMG_OK;
+}
+
+int pgmg_x_diff2_noalloc(pgmg_hier *h, int level, const double *ext_x, double *out) {
+	int rc = check_level(h, level);
+	if (rc) return rc;
+	if (!ext_x || !out) return fail(PGMG_ERR_ARG, "null argument");
+	Level &L = h->lev[level];
+	const long long N = L.points();
+	const long long nparts = (long long)((L.v.m + 63) / 64) * ((L.v.n + 3) / 4) * L.v.o;
+	// the vector, one sum per block, the result
+	if (N + nparts + 1 > kScratchDoubles)
+		return fail(PGMG_ERR_ARG, "level %d has %lld points: the allocation-free soln_diff takes levels whose "
+			    "points plus %lld block sums fit %lld doubles; use pgmg_x_diff2", level, N, nparts, kScratchDoubles - 1);
+	double *sp = nullptr;
+	rc = scratch_on(h->device, &sp);
+	if (rc) return rc;
+	double *ext = sp, *partial = sp + N, *res = sp + N + nparts;
+	HIP_TRY(hipMemcpyAsync(ext, ext_x, sizeof(double) * (size_t)N, hipMemcpyHostToDevice, h->stream));
+	launch_diff2_compact(h->stream, L.v, ext, L.x, partial);
+	launch_reduce(h->stream, partial, (int)nparts, res);
+	HIP_TRY(hipMemcpyAsync(h->hscalar, res, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+	HIP_TRY(hipStreamSynchronize(h->stream));
+	*out = *h->hscalar;
+	return PGMG_OK;
 }
 
 int pgmg_synchronize(pgmg_hier *h) {

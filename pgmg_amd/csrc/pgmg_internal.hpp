@@ -267,6 +267,9 @@ int residual_partial_count(const LevelView &L);
 void launch_apply(hipStream_t s, const LevelView &L, const double *A, const double *in, double *out);
 // Block sums of a*b (diff = 0) or (a-b)^2 (diff = 1); returns the count.
 int launch_pair_sum(hipStream_t s, const LevelView &L, const double *a, const double *b, int diff, double *partial);
+// The block sums of launch_pair_sum(diff = 1) with `ext` point-major
+// (i + m*(j + n*k)) rather than in the level layout; returns the count.
+int launch_diff2_compact(hipStream_t s, const LevelView &L, const double *ext, const double *x, double *partial);
 // out[0] = sum of partial[0..n) in a fixed order.
 void launch_reduce(hipStream_t s, const double *partial, int n, double *out);
 // partial sums of r^2 (rhs_l2norm); returns count.

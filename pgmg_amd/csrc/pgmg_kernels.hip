@@ -1928,6 +1928,29 @@ int launch_pair_sum(hipStream_t s, const LevelView &L, const double *a, const do
 	return ((L.m + 63) / 64) * ((L.n + 3) / 4) * L.o;
 }
 
+// The same block sums of (ext - x)^2 with ext given point-major
+// (i + m*(j + n*k)) instead of in the level layout: the same blocks, the same
+// per-thread values, so the same partial sums as k_pair_sum with diff = 1.
+__global__ __launch_bounds__(256) void k_diff2_compact(LevelView L, const double *__restrict__ ext,
+						       const double *__restrict__ x, double *__restrict__ partial) {
+	const int i = (int)(blockIdx.x * 64 + threadIdx.x), j = (int)(blockIdx.y * 4 + threadIdx.y), k = (int)blockIdx.z;
+	double v = 0.0;
+	if (i < L.m && j < L.n) {
+		const double d = ext[i + (long long)L.m * (j + (long long)L.n * k)] - x[idx3(L, i, j, k)];
+		v = d * d;
+	}
+	const double sblk = block_sum_256(v);
+	if (threadIdx.x == 0 && threadIdx.y == 0)
+		partial[blockIdx.x + gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z)] = sblk;
+}
+
+int launch_diff2_compact(hipStream_t s, const LevelView &L, const double *ext, const double *x, double *partial) {
+	dim3 grid((L.m + 63) / 64, (L.n + 3) / 4, L.o), block(64, 4, 1);
+	hipLaunchKernelGGL(k_diff2_compact, grid, block, 0, s, L, ext, x, partial);
+	HIPCHECK_LAUNCH();
+	return ((L.m + 63) / 64) * ((L.n + 3) / 4) * L.o;
+}
+
 // Fixed-order reduction: thread t sums partial[t], partial[t+1024], ... in
 // order, then a fixed binary tree.
 __global__ __launch_bounds__(1024) void k_reduce(const double *__restrict__ partial, int n, double *__restrict__ out) {
