@@ -306,13 +306,14 @@ def pcie_inclusive(P):
             "bytes_uploaded": int(28 * 8 * N ** 3)}
 
 
-def batch_256(P, precision, tol=1e-8, count=256, max_cycles=1000):
+def batch_256(P, precision, tol=1e-8, count=256, max_cycles=1000, use_graph=True):
     """256 independent problems, axes drawn from {33, 65, 129}, per-face
-    a = 10^U[-2,2], solved together to tol."""
+    a = 10^U[-2,2], solved together to tol.  Reports the tile shapes the
+    batch chose for its launches and how many cycles ran as graph replays."""
     rng = np.random.default_rng(2024)
     sizes = [tuple(int(v) for v in rng.choice([33, 65, 129], size=3)) for _ in range(count)]
     need = sum(P.footprint(m, n, o, precision=precision) for (m, n, o) in sizes)
-    with P.Batch(int(need * 1.05) + (1 << 30), max_slots=count, precision=precision) as b:
+    with P.Batch(int(need * 1.05) + (1 << 30), max_slots=count, precision=precision, use_graph=use_graph) as b:
         # Timed from admission (generation and Galerkin set-up on the GPU
         # included) to the last slot's convergence.
         t0 = time.perf_counter()
@@ -320,13 +321,14 @@ def batch_256(P, precision, tol=1e-8, count=256, max_cycles=1000):
         steps = b.run(max_steps=max_cycles + 5)
         t = time.perf_counter() - t0
         st = [b.status(s) for s in slots]
+        tiles, replays = b.tiles()
     conv = sum(1 for s in st if s[0] == P.SLOT_CONVERGED)
     pts = sum(m * n * o for (m, n, o) in sizes)
     cyc = [s[1] for s in st]
     return {"problems": count, "converged": conv, "solve_s": round(t, 3), "steps": steps,
             "cycles_mean": round(float(np.mean(cyc)), 1), "cycles_max": int(max(cyc)),
             "points_total": pts, "dof_cycles_per_s": round(pts * float(np.mean(cyc)) / t, 1),
-            "arena_gb": round(need / 1e9, 2)}
+            "arena_gb": round(need / 1e9, 2), "tiles": tiles, "graph_replays": replays}
 
 
 def serving(P, n_req=160, seed=11):
@@ -403,8 +405,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 513^3, batch and serving legs")
-    ap.add_argument("--leg", choices=["pcie", "q1_513", "batch_mixed", "batch_fp64", "serving", "periodic_256",
-                                      "exact", "exact_mixed"],
+    ap.add_argument("--leg", choices=["pcie", "q1_513", "batch_mixed", "batch_fp64", "batch_mixed_eager", "serving",
+                                      "periodic_256", "exact", "exact_mixed"],
                     help="run one extra leg alone and print its JSON (for profiling)")
     ap.add_argument("--periodic-modes", default="exact_fp64,colour8_fp64,colour8_mixed",
                     help="with --leg periodic_256: the smoother/precision runs to time (comma-separated)")
@@ -424,6 +426,7 @@ def main():
     if args.leg:
         fn = {"pcie": lambda: pcie_inclusive(P), "q1_513": lambda: q1_513(P), "batch_mixed": lambda: batch_256(P, P.PREC_MIXED),
               "batch_fp64": lambda: batch_256(P, P.PREC_FP64), "serving": lambda: serving(P),
+              "batch_mixed_eager": lambda: batch_256(P, P.PREC_MIXED, use_graph=False),
               "periodic_256": lambda: periodic_256(P, tests=args.periodic_tests,
                                                    modes=tuple(args.periodic_modes.split(",")),
                                                    ref_cpu=not args.no_cpu),

@@ -228,6 +228,17 @@ int pgmg_batch_get_x(pgmg_batch *b, int slot, double *x);
 /* Return a finished slot's memory to the arena. */
 int pgmg_batch_release(pgmg_batch *b, int slot);
 int pgmg_batch_free_bytes(const pgmg_batch *b, uint64_t *bytes);
+/* Tile shapes of the batched launches: the work units (rows or planes) per
+ * workgroup chosen for each launch class, in the order GS pair sweeps,
+ * residual, level-0 residual, restriction, interpolation, x += x32.  The
+ * batch's first eight steps try 1, 2, 4 and 8 units per workgroup (twice
+ * each) with timing events around the launches and keep the fastest per
+ * unit; results do not depend on the choice.  *tuned is 1 once chosen.
+ * With use_graph, the cycle is then captured as a hipGraph on the second
+ * step of an unchanged plan and replayed until the plan changes (a slot
+ * admitted or finished); *graph_launches counts the replays. */
+enum { PGMG_BATCH_TILE_CLASSES = 6 };
+int pgmg_batch_tiles(const pgmg_batch *b, int *chunks, int *tuned, long long *graph_launches);
 /* Open-loop serving: requests arrive at the given times (seconds from the
  * call); each is admitted when a slot and arena space are free (FIFO), runs
  * with the others, and completes when its solution is on the host.

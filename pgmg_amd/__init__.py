@@ -352,6 +352,8 @@ _BATCH_SIGS = {
     "pgmg_batch_get_x": (_i, [_VP, _i, _D]),
     "pgmg_batch_release": (_i, [_VP, _i]),
     "pgmg_batch_free_bytes": (_i, [_VP, ctypes.POINTER(ctypes.c_uint64)]),
+    "pgmg_batch_tiles": (_i, [_VP, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                              ctypes.POINTER(ctypes.c_longlong)]),
     "pgmg_serve_run": (_i, [_VP, ctypes.POINTER(Request), _i, ctypes.POINTER(Result), _D]),
 }
 _SIGS.update(_BATCH_SIGS)
@@ -362,8 +364,12 @@ SLOT_FREE, SLOT_ACTIVE, SLOT_CONVERGED, SLOT_FAILED = 0, 1, 2, 3
 class Batch:
     """Independent hierarchies of any sizes advanced together (pgmg_batch_*)."""
 
-    def __init__(self, arena_bytes, max_slots=256, precision=PREC_MIXED, bottom=BOTTOM_GS24, v_down=2, v_up=2):
-        self.opts = default_opts(precision=precision, smoother=SMOOTH_COLOUR8, bottom=bottom, v_down=v_down, v_up=v_up)
+    TILE_CLASSES = ("gs_pair", "residual", "residual_top", "restrict", "interp", "add_x")
+
+    def __init__(self, arena_bytes, max_slots=256, precision=PREC_MIXED, bottom=BOTTOM_GS24, v_down=2, v_up=2,
+                 use_graph=True):
+        self.opts = default_opts(precision=precision, smoother=SMOOTH_COLOUR8, bottom=bottom, v_down=v_down, v_up=v_up,
+                                 use_graph=use_graph)
         h = _VP()
         _check(lib().pgmg_batch_create(ctypes.byref(self.opts), int(arena_bytes), max_slots, ctypes.byref(h)))
         self.h = h
@@ -443,6 +449,14 @@ class Batch:
         v = ctypes.c_uint64()
         _check(lib().pgmg_batch_free_bytes(self.h, ctypes.byref(v)))
         return v.value
+
+    def tiles(self):
+        """(work units per workgroup by launch class or None before the
+        trial steps are done, number of graph replays of the cycle)."""
+        c = (ctypes.c_int * len(self.TILE_CLASSES))()
+        t, g = ctypes.c_int(), ctypes.c_longlong()
+        _check(lib().pgmg_batch_tiles(self.h, c, ctypes.byref(t), ctypes.byref(g)))
+        return (dict(zip(self.TILE_CLASSES, list(c))) if t.value else None), g.value
 
     def serve(self, requests, keep_x=True):
         """requests: list of dicts (arrival_s, m, n, o, family, seed, tol,

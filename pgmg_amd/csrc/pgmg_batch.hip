@@ -114,6 +114,31 @@ int shape_dims(int m, int n, int o, int *dims) {
 enum { L_PAIR0 = 0, L_PAIR1, L_PAIR2, L_PAIR3, L_SMALL, L_RES, L_RESTRICT, L_INTERP, L_LU, L_GS24, L_PER_LEVEL };
 enum { L_TOP_ROWS = 0, L_TOP_ONE, L_TOP_PTS, L_TOP_COUNT };
 
+// Tile shapes of the batched launches: work units per workgroup, one choice
+// per launch class, measured on the batch's own first steps.  Trial t
+// (t = 0 .. kTuneSteps-1, two rounds over the candidates) runs the launches
+// between two convergence checks with candidate t % kNumChunks, each
+// bracketed by timing events; each class then keeps the chunk with the
+// least time per work unit (the smaller of its rounds).  A unit's arithmetic
+// does not depend on the chunk, so the trial steps are ordinary steps with
+// the same results.
+enum { TC_PAIR = 0, TC_RES, TC_RESTOP, TC_RESTRICT, TC_INTERP, TC_ADD, TC_COUNT };
+const int kChunks[] = {1, 2, 4, 8};
+const int kNumChunks = 4;
+const int kTuneSteps = 2 * kNumChunks;
+
+struct TileTuner {
+	int step = 0;           // trial steps done
+	bool done = false;
+	int best[TC_COUNT] = {1, 1, 1, 1, 1, 1};
+	double ms[TC_COUNT][kNumChunks][2] = {};        // per candidate and round
+	long long units[TC_COUNT][kNumChunks][2] = {};
+	struct Mark { int cls, cand, round; long long units; hipEvent_t a, b; };
+	std::vector<Mark> marks;    // this step's timed launches
+	std::vector<hipEvent_t> pool;
+	size_t used = 0;
+};
+
 }  // namespace
 
 struct pgmg_batch {
@@ -134,7 +159,23 @@ struct pgmg_batch {
 	int *d_info = nullptr;
 	int max_nl = 0;
 	uint64_t admitted = 0;
+	TileTuner tune;
+	// the batched cycle as a hipGraph, captured once the tile shapes are
+	// chosen and the plan has stayed the same for a step; dropped when the
+	// plan changes
+	hipGraphExec_t cycle_graph = nullptr;
+	int plan_steps = 0;   // consecutive steps on the current plan
+	long long graph_launches = 0;
 };
+
+namespace {
+
+void drop_graph(pgmg_batch *b) {
+	if (b->cycle_graph) { hipGraphExecDestroy(b->cycle_graph); b->cycle_graph = nullptr; }
+	b->plan_steps = 0;
+}
+
+}  // namespace
 
 namespace {
 
@@ -191,7 +232,66 @@ int build_plan(pgmg_batch *b) {
 	}
 	BHIP(hipMemcpyAsync(b->d_plan, b->h_plan.data(), sizeof(int) * b->h_plan.size(), hipMemcpyHostToDevice, b->stream));
 	b->plan_dirty = false;
+	drop_graph(b);  // its launches were sized for the old lists
 	return PGMG_OK;
+}
+
+int chunk_of(pgmg_batch *b, int cls) {
+	return b->tune.done ? b->tune.best[cls] : kChunks[b->tune.step % kNumChunks];
+}
+
+// Runs launch(chunk) for class cls; during the trial steps it is bracketed
+// by timing events (units: the list's work units).
+template <typename F>
+void tuned(pgmg_batch *b, int cls, const WorkList &wl, F launch) {
+	if (wl.n <= 0) return;
+	const int c = chunk_of(b, cls);
+	TileTuner &t = b->tune;
+	if (t.done) { launch(c); return; }
+	if (t.used + 2 > t.pool.size()) {
+		for (int q = 0; q < 64; q++) {
+			hipEvent_t e;
+			if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) { launch(c); return; }
+			t.pool.push_back(e);
+		}
+	}
+	TileTuner::Mark mk{cls, t.step % kNumChunks, t.step / kNumChunks, wl.total, t.pool[t.used], t.pool[t.used + 1]};
+	t.used += 2;
+	hipEventRecord(mk.a, b->stream);
+	launch(c);
+	hipEventRecord(mk.b, b->stream);
+	t.marks.push_back(mk);
+}
+
+// Once the stream is synchronised: books the timed launches under their
+// candidates, moves to the next trial, and after the last one picks the
+// chunks.
+void tune_collect(pgmg_batch *b) {
+	TileTuner &t = b->tune;
+	if (t.done) return;
+	for (const auto &mk : t.marks) {
+		float ms = 0;
+		if (hipEventElapsedTime(&ms, mk.a, mk.b) == hipSuccess) {
+			t.ms[mk.cls][mk.cand][mk.round] += ms;
+			t.units[mk.cls][mk.cand][mk.round] += mk.units;
+		}
+	}
+	t.marks.clear();
+	t.used = 0;
+	if (++t.step < kTuneSteps) return;
+	for (int c = 0; c < TC_COUNT; c++) {
+		double best = -1;
+		for (int q = 0; q < kNumChunks; q++) {
+			double per = -1;
+			for (int r = 0; r < 2; r++) {
+				if (t.units[c][q][r] <= 0) continue;
+				const double v = t.ms[c][q][r] / (double)t.units[c][q][r];
+				if (per < 0 || v < per) per = v;
+			}
+			if (per >= 0 && (best < 0 || per < best)) { best = per; t.best[c] = kChunks[q]; }
+		}
+	}
+	t.done = true;
 }
 
 // from_zero: level 0 of the mixed correction, whose pair-list slots did not
@@ -200,8 +300,12 @@ template <typename T>
 void batched_gs(pgmg_batch *b, int l, int sweeps, bool from_zero = false) {
 	if (sweeps <= 0) return;
 	for (int s = 0; s < sweeps; s++)
-		for (int q = 0; q < 4; q++)
-			launch_b_gs_pair<T>(b->stream, b->d_slots, wlist(b, list_index(l, L_PAIR0 + q)), l, q, from_zero && s == 0);
+		for (int q = 0; q < 4; q++) {
+			const WorkList wl = wlist(b, list_index(l, L_PAIR0 + q));
+			tuned(b, TC_PAIR, wl, [&](int c) {
+				launch_b_gs_pair<T>(b->stream, b->d_slots, wl, l, q, from_zero && s == 0, c);
+			});
+		}
 	launch_b_gs_small<T>(b->stream, b->d_slots, wlist(b, list_index(l, L_SMALL)), l, sweeps);
 }
 
@@ -211,15 +315,50 @@ void batched_cycle(pgmg_batch *b) {
 	const int L = b->max_nl;
 	for (int l = 0; l < L; l++) {
 		batched_gs<T>(b, l, b->opts.v_down + l, b->mixed && l == 0);
-		launch_b_residual<T>(b->stream, b->d_slots, wlist(b, list_index(l, L_RES)), l);
-		if (l + 1 < kBatchMaxLevels) launch_b_restrict<T>(b->stream, b->d_slots, wlist(b, list_index(l + 1, L_RESTRICT)), l + 1);
+		const WorkList wr = wlist(b, list_index(l, L_RES));
+		tuned(b, TC_RES, wr, [&](int c) { launch_b_residual<T>(b->stream, b->d_slots, wr, l, c); });
+		if (l + 1 < kBatchMaxLevels) {
+			const WorkList wt = wlist(b, list_index(l + 1, L_RESTRICT));
+			tuned(b, TC_RESTRICT, wt, [&](int c) { launch_b_restrict<T>(b->stream, b->d_slots, wt, l + 1, c); });
+		}
 		launch_b_lu_solve<T>(b->stream, b->d_slots, wlist(b, list_index(l, L_LU)), l);
 		launch_b_gs_small<T>(b->stream, b->d_slots, wlist(b, list_index(l, L_GS24)), l, kGs24);
 	}
 	for (int l = L - 1; l > 0; l--) {
-		launch_b_interp<T>(b->stream, b->d_slots, wlist(b, list_index(l, L_INTERP)), l);
+		const WorkList wi = wlist(b, list_index(l, L_INTERP));
+		tuned(b, TC_INTERP, wi, [&](int c) { launch_b_interp<T>(b->stream, b->d_slots, wi, l, c); });
 		batched_gs<T>(b, l - 1, b->opts.v_up + l - 1);
 	}
+	if (b->mixed) {
+		// x += (double) x32 on level 0
+		const WorkList wa = wlist(b, L_TOP_PTS);
+		tuned(b, TC_ADD, wa, [&](int c) { launch_b_add_x(b->stream, b->d_slots, wa, c); });
+	}
+}
+
+// The cycle of a step (and in mixed precision x += x32): eager during the
+// tile trials and on a plan's first step, then as a hipGraph captured on
+// the plan's second step and replayed until the plan changes (its launches
+// hold the lists' sizes).  The graph holds kernel launches only.
+int run_cycle(pgmg_batch *b) {
+	auto eager = [&]() { if (b->mixed) batched_cycle<float>(b); else batched_cycle<double>(b); };
+	if (!b->opts.use_graph || !b->tune.done) { eager(); return PGMG_OK; }
+	if (!b->cycle_graph && ++b->plan_steps >= 2) {
+		hipGraph_t g;
+		BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+		eager();
+		BHIP(hipStreamEndCapture(b->stream, &g));
+		const hipError_t e = hipGraphInstantiate(&b->cycle_graph, g, nullptr, nullptr, 0);
+		hipGraphDestroy(g);
+		if (e != hipSuccess) { b->cycle_graph = nullptr; return bfail(PGMG_ERR_HIP, "batch graph instantiate: %s", hipGetErrorString(e)); }
+	}
+	if (b->cycle_graph) {
+		BHIP(hipGraphLaunch(b->cycle_graph, b->stream));
+		b->graph_launches++;
+	} else {
+		eager();
+	}
+	return PGMG_OK;
 }
 
 uint64_t slot_bytes(const pgmg_batch *b, int nl, const int *dims) {
@@ -405,6 +544,8 @@ int pgmg_batch_destroy(pgmg_batch *b) {
 	if (b->stream) hipStreamSynchronize(b->stream);
 	if (b->setup) hipStreamSynchronize(b->setup);
 	for (auto &S : b->slots) if (S.ready) hipEventDestroy(S.ready);
+	drop_graph(b);
+	for (auto e : b->tune.pool) hipEventDestroy(e);
 	hipFree(b->arena.base);
 	hipFree(b->d_slots); hipFree(b->d_plan); hipFree(b->d_norms); hipFree(b->d_aux); hipFree(b->d_info);
 	if (b->h_norms) hipHostFree(b->h_norms);
@@ -452,10 +593,13 @@ int pgmg_batch_step(pgmg_batch *b, int *active, int *finished) {
 	if (b->list_n[L_TOP_ROWS] > 0) {
 		// residual of the current iterate: convergence test, and the FP32
 		// defect for the mixed-precision cycle
-		launch_b_residual_top(b->stream, b->d_slots, wlist(b, L_TOP_ROWS), b->mixed);
+		const WorkList wtop = wlist(b, L_TOP_ROWS);
+		tuned(b, TC_RESTOP, wtop, [&](int c) { launch_b_residual_top(b->stream, b->d_slots, wtop, b->mixed, c); });
 		launch_b_reduce(b->stream, b->d_slots, wlist(b, L_TOP_ONE), b->d_norms);
 		BHIP(hipMemcpyAsync(b->h_norms, b->d_norms, sizeof(double) * b->max_slots, hipMemcpyDeviceToHost, b->stream));
 		BHIP(hipStreamSynchronize(b->stream));
+		// the previous step's cycle and this residual are complete
+		tune_collect(b);
 		for (int s = 0; s < b->max_slots; s++) {
 			SlotHost &S = b->slots[s];
 			if (S.state != ST_ACTIVE) continue;
@@ -466,19 +610,23 @@ int pgmg_batch_step(pgmg_batch *b, int *active, int *finished) {
 		}
 		if (nfin) { rc = build_plan(b); if (rc) return rc; }
 		if (b->list_n[L_TOP_ROWS] > 0) {
-			if (b->mixed) {
-				// x32 was cleared by the residual launch above
-				batched_cycle<float>(b);
-				launch_b_add_x(b->stream, b->d_slots, wlist(b, L_TOP_PTS));
-			} else {
-				batched_cycle<double>(b);
-			}
+			rc = run_cycle(b);
+			if (rc) return rc;
 			BHIP(hipGetLastError());
 			for (auto &S : b->slots) if (S.state == ST_ACTIVE) { S.cycles++; nact++; }
 		}
 	}
 	if (active) *active = nact;
 	if (finished) *finished = nfin;
+	return PGMG_OK;
+}
+
+int pgmg_batch_tiles(const pgmg_batch *b, int *chunks, int *tuned, long long *graph_launches) {
+	if (!b) return bfail(PGMG_ERR_ARG, "null batch");
+	if (chunks)
+		for (int c = 0; c < TC_COUNT; c++) chunks[c] = b->tune.done ? b->tune.best[c] : 0;
+	if (tuned) *tuned = b->tune.done ? 1 : 0;
+	if (graph_launches) *graph_launches = b->graph_launches;
 	return PGMG_OK;
 }
 

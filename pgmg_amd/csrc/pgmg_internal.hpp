@@ -173,19 +173,21 @@ enum { BK_PAIR0 = 0, BK_PAIR1, BK_PAIR2, BK_PAIR3, BK_ONE, BK_ROWS, BK_PTS, BK_C
 int batch_units(int kind, const LevelView &L, int elem);
 
 // Batched launchers: one launch covers the level `level` of every slot in
-// the list; blocks walk the units grid-stride.
+// the list; blocks walk the units grid-stride.  chunk: work units per
+// workgroup (the grid is ceil(units / chunk)); a unit's arithmetic does not
+// depend on it, so every chunk gives the same results.
 // from_zero (FP32): the first sweep of level 0 of a mixed correction (slots
 // with top_zero set; the others are in the small-level list).
 template <typename T> void launch_b_gs_pair(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int pair,
-					    bool from_zero = false);
+					    bool from_zero = false, int chunk = 1);
 template <typename T> void launch_b_gs_small(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int nsweeps);
-template <typename T> void launch_b_residual(hipStream_t s, const SlotDev *slots, WorkList wl, int level);
-void launch_b_residual_top(hipStream_t s, const SlotDev *slots, WorkList wl, bool to_f32);
+template <typename T> void launch_b_residual(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int chunk = 1);
+void launch_b_residual_top(hipStream_t s, const SlotDev *slots, WorkList wl, bool to_f32, int chunk = 1);
 void launch_b_reduce(hipStream_t s, const SlotDev *slots, WorkList wl, double *norms);
-template <typename T> void launch_b_restrict(hipStream_t s, const SlotDev *slots, WorkList wl, int level);
-template <typename T> void launch_b_interp(hipStream_t s, const SlotDev *slots, WorkList wl, int level);
+template <typename T> void launch_b_restrict(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int chunk = 1);
+template <typename T> void launch_b_interp(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int chunk = 1);
 template <typename T> void launch_b_lu_solve(hipStream_t s, const SlotDev *slots, WorkList wl, int level);
-void launch_b_add_x(hipStream_t s, const SlotDev *slots, WorkList wl);
+void launch_b_add_x(hipStream_t s, const SlotDev *slots, WorkList wl, int chunk = 1);
 
 // Records the calling thread's last error message (pgmg_last_error) and
 // returns code.

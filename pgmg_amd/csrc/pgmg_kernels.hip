@@ -3177,22 +3177,25 @@ __global__ __launch_bounds__(256) void kb_add_x(const SlotDev *__restrict__ slot
 }
 #undef BATCH_LOOP
 
-// One workgroup per work unit: the hardware schedules the units of all
-// slots as it would a single hierarchy's launch.
-static unsigned batch_grid(const WorkList &wl) { return (unsigned)(wl.total > 0 ? wl.total : 1); }
+// chunk work units per workgroup (1: the hardware schedules the units of
+// all slots as it would a single hierarchy's launch).
+static unsigned batch_grid(const WorkList &wl, int chunk = 1) {
+	const int c = chunk > 0 ? chunk : 1;
+	return (unsigned)(wl.total > 0 ? (wl.total + c - 1) / c : 1);
+}
 
 template <typename T>
-void launch_b_gs_pair(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int pair, bool from_zero) {
+void launch_b_gs_pair(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int pair, bool from_zero, int chunk) {
 	if (wl.n <= 0) return;
 	if constexpr (sizeof(T) == 4) {
 		if (from_zero) {
-			hipLaunchKernelGGL((kb_gs_pair<T, true>), dim3(batch_grid(wl)), dim3(64, 4), 0, s, slots, wl, level, pair & 1,
+			hipLaunchKernelGGL((kb_gs_pair<T, true>), dim3(batch_grid(wl, chunk)), dim3(64, 4), 0, s, slots, wl, level, pair & 1,
 					   (pair >> 1) & 1);
 			HIPCHECK_LAUNCH();
 			return;
 		}
 	}
-	hipLaunchKernelGGL((kb_gs_pair<T, false>), dim3(batch_grid(wl)), dim3(64, 4), 0, s, slots, wl, level, pair & 1,
+	hipLaunchKernelGGL((kb_gs_pair<T, false>), dim3(batch_grid(wl, chunk)), dim3(64, 4), 0, s, slots, wl, level, pair & 1,
 			   (pair >> 1) & 1);
 	HIPCHECK_LAUNCH();
 }
@@ -3203,14 +3206,14 @@ void launch_b_gs_small(hipStream_t s, const SlotDev *slots, WorkList wl, int lev
 	HIPCHECK_LAUNCH();
 }
 template <typename T>
-void launch_b_residual(hipStream_t s, const SlotDev *slots, WorkList wl, int level) {
+void launch_b_residual(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int chunk) {
 	if (wl.n <= 0) return;
-	hipLaunchKernelGGL(kb_residual<T>, dim3(batch_grid(wl)), dim3(64, 4), 0, s, slots, wl, level);
+	hipLaunchKernelGGL(kb_residual<T>, dim3(batch_grid(wl, chunk)), dim3(64, 4), 0, s, slots, wl, level);
 	HIPCHECK_LAUNCH();
 }
-void launch_b_residual_top(hipStream_t s, const SlotDev *slots, WorkList wl, bool to_f32) {
+void launch_b_residual_top(hipStream_t s, const SlotDev *slots, WorkList wl, bool to_f32, int chunk) {
 	if (wl.n <= 0) return;
-	hipLaunchKernelGGL(kb_residual_top, dim3(batch_grid(wl)), dim3(64, 4), 0, s, slots, wl, to_f32 ? 1 : 0);
+	hipLaunchKernelGGL(kb_residual_top, dim3(batch_grid(wl, chunk)), dim3(64, 4), 0, s, slots, wl, to_f32 ? 1 : 0);
 	HIPCHECK_LAUNCH();
 }
 void launch_b_reduce(hipStream_t s, const SlotDev *slots, WorkList wl, double *norms) {
@@ -3219,15 +3222,15 @@ void launch_b_reduce(hipStream_t s, const SlotDev *slots, WorkList wl, double *n
 	HIPCHECK_LAUNCH();
 }
 template <typename T>
-void launch_b_restrict(hipStream_t s, const SlotDev *slots, WorkList wl, int level) {
+void launch_b_restrict(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int chunk) {
 	if (wl.n <= 0) return;
-	hipLaunchKernelGGL(kb_restrict<T>, dim3(batch_grid(wl)), dim3(64, 4), 0, s, slots, wl, level);
+	hipLaunchKernelGGL(kb_restrict<T>, dim3(batch_grid(wl, chunk)), dim3(64, 4), 0, s, slots, wl, level);
 	HIPCHECK_LAUNCH();
 }
 template <typename T>
-void launch_b_interp(hipStream_t s, const SlotDev *slots, WorkList wl, int level) {
+void launch_b_interp(hipStream_t s, const SlotDev *slots, WorkList wl, int level, int chunk) {
 	if (wl.n <= 0) return;
-	hipLaunchKernelGGL(kb_interp<T>, dim3(batch_grid(wl)), dim3(64, 4), 0, s, slots, wl, level);
+	hipLaunchKernelGGL(kb_interp<T>, dim3(batch_grid(wl, chunk)), dim3(64, 4), 0, s, slots, wl, level);
 	HIPCHECK_LAUNCH();
 }
 template <typename T>
@@ -3236,9 +3239,9 @@ void launch_b_lu_solve(hipStream_t s, const SlotDev *slots, WorkList wl, int lev
 	hipLaunchKernelGGL(kb_lu_solve<T>, dim3(batch_grid(wl)), dim3(1024), 0, s, slots, wl, level);
 	HIPCHECK_LAUNCH();
 }
-void launch_b_add_x(hipStream_t s, const SlotDev *slots, WorkList wl) {
+void launch_b_add_x(hipStream_t s, const SlotDev *slots, WorkList wl, int chunk) {
 	if (wl.n <= 0) return;
-	hipLaunchKernelGGL(kb_add_x, dim3(batch_grid(wl)), dim3(64, 4), 0, s, slots, wl);
+	hipLaunchKernelGGL(kb_add_x, dim3(batch_grid(wl, chunk)), dim3(64, 4), 0, s, slots, wl);
 	HIPCHECK_LAUNCH();
 }
 
@@ -3260,11 +3263,11 @@ void launch_b_add_x(hipStream_t s, const SlotDev *slots, WorkList wl) {
 	template void launch_interp<T>(hipStream_t, const LevelView &, const LevelView &, const T *, T *); \
 	template void launch_lu_solve<T>(hipStream_t, const LevelView &, int, const double *, const int *, const T *, T *); \
 	template void launch_fill<T>(hipStream_t, T *, long long, T); \
-	template void launch_b_gs_pair<T>(hipStream_t, const SlotDev *, WorkList, int, int, bool); \
+	template void launch_b_gs_pair<T>(hipStream_t, const SlotDev *, WorkList, int, int, bool, int); \
 	template void launch_b_gs_small<T>(hipStream_t, const SlotDev *, WorkList, int, int); \
-	template void launch_b_residual<T>(hipStream_t, const SlotDev *, WorkList, int); \
-	template void launch_b_restrict<T>(hipStream_t, const SlotDev *, WorkList, int); \
-	template void launch_b_interp<T>(hipStream_t, const SlotDev *, WorkList, int); \
+	template void launch_b_residual<T>(hipStream_t, const SlotDev *, WorkList, int, int); \
+	template void launch_b_restrict<T>(hipStream_t, const SlotDev *, WorkList, int, int); \
+	template void launch_b_interp<T>(hipStream_t, const SlotDev *, WorkList, int, int); \
 	template void launch_b_lu_solve<T>(hipStream_t, const SlotDev *, WorkList, int);
 INST(double)
 INST(float)

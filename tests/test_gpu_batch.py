@@ -193,3 +193,36 @@ def test_serving_request_that_never_fits():
         assert b.free_bytes() == int(fp * 1.5)
     assert res[0]["status"] == 0 and res[2]["status"] == 0
     assert res[1]["status"] == 10  # PGMG_ERR_ADMISSION
+
+
+def test_batch_tiles_and_graph_keep_the_bits():
+    """The trial steps of the tile shapes and the graph-captured cycles that
+    follow give the bits of eager cycles at one unit per workgroup; a slot
+    admitted after the graph is captured (the plan changes, the graph is
+    dropped and captured again) still equals its lone solve."""
+    tol, maxc = 1e-30, 30  # every slot runs its 30 cycles
+    p0, p1 = PROBLEMS[1], PROBLEMS[3]
+    ref0 = alone(*p0, P.PREC_MIXED, P.BOTTOM_GS24, tol, maxc)
+    ref1 = alone(*p1, P.PREC_MIXED, P.BOTTOM_GS24, tol, maxc)
+    with P.Batch(2 << 30, max_slots=4) as b:
+        s0 = b.admit(*p0, tol, maxc)
+        for _ in range(12):
+            b.step()
+        tiles, replays = b.tiles()
+        say("tiles", tiles, "graph replays", replays)
+        assert tiles is not None and set(tiles.values()) <= {1, 2, 4, 8}
+        assert replays >= 1
+        s1 = b.admit(*p1, tol, maxc)
+        b.run()
+        tiles2, replays2 = b.tiles()
+        say("graph replays at the end", replays2)
+        assert tiles2 == tiles and replays2 > replays
+        for s, ref in ((s0, ref0), (s1, ref1)):
+            st, c, rel = b.status(s)
+            assert c == ref[0] == maxc, (s, c, ref[0])
+            assert np.array_equal(b.get_x(s), ref[1]), s
+    with P.Batch(1 << 30, max_slots=2, use_graph=False) as b:
+        s0 = b.admit(*p0, tol, maxc)
+        b.run()
+        assert b.tiles()[1] == 0
+        assert np.array_equal(b.get_x(s0), ref0[1])
