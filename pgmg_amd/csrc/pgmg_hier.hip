@@ -235,8 +235,8 @@ int *flow_err(pgmg_hier *h) { return (int *)(h->dscalar + 1); }
 // launch per hyperplane.
 template <typename T>
 void skew_sweeps(pgmg_hier *h, Level &L, int nsweeps) {
-	if (!h->flow_prog || !launch_gs_flow<T>(h->stream, L.sv, sA<T>(L), sx<T>(L), sr<T>(L), nsweeps, h->flow_prog,
-						 flow_err(h), h->flow_limit))
+	if (!h->flow_prog || !launch_gs_flow<T>(h->stream, L.sv, L.v.prd != 0, sA<T>(L), sx<T>(L), sr<T>(L), nsweeps,
+						 h->flow_prog, flow_err(h), h->flow_limit))
 		launch_gs_planes_skew<T>(h->stream, L.sv, sA<T>(L), sx<T>(L), sr<T>(L), nsweeps);
 }
 

@@ -246,10 +246,11 @@ template <typename T> void launch_gs_planes_skew(hipStream_t s, const SkewView &
 // sweeps).  prog: o + 1 ints, zero on entry and left zero; err: set when a
 // wait gave up after `limit` polls (the sweep is then incomplete).  Returns
 // false, launching nothing, when the level cannot take the kernel (all its
-// planes' workgroups would not be resident at once).
-template <typename T> bool launch_gs_flow(hipStream_t s, const SkewView &S, const T *As, T *xs, const T *rs,
-					  int nsweeps, int *prog, int *err, int limit);
-template <typename T> int flow_max_sweeps(const SkewView &S);
+// planes' workgroups would not be resident at once).  prd: the level has
+// periodic axes (its halo holds copies of x).
+template <typename T> bool launch_gs_flow(hipStream_t s, const SkewView &S, bool prd, const T *As, T *xs,
+					  const T *rs, int nsweeps, int *prog, int *err, int limit);
+template <typename T> int flow_max_sweeps(const SkewView &S, bool prd);
 // nsweeps forward lexicographic sweeps with the sweeps pipelined eight
 // hyperplanes apart (bit-identical to nsweeps separate sweeps).
 template <typename T> void launch_gs_planes(hipStream_t s, const LevelView &L, const T *A, T *x, const T *r, int nsweeps);

@@ -1103,10 +1103,20 @@ __host__ __device__ inline int flow_row_pad(const SkewView &S) { return ((flow_r
 const int kFlowMaxThreads = 512;
 const int kFlowMaxPlanes = 160;
 
-template <typename T>
+// First j of row u in the plane-skewed layout's row storage (sidx3).
+__device__ __forceinline__ int skew_jf(const SkewView &S, int u) { return S.cmp ? max(-1, (u - S.m + 1) >> 1) : -1; }
+
+// PRD: the level has periodic axes, whose halo holds copies of x, not zeros.
+template <typename T, bool PRD>
 __global__ __launch_bounds__(kFlowMaxThreads) void k_gs_flow(SkewView S, const T *__restrict__ As, T *__restrict__ xs,
 							     const T *__restrict__ rs, int nsw, int rpad,
 							     int *__restrict__ prog, int *__restrict__ err, int limit) {
+	// the last four rows of each sweep in flight, [sweep][u & 3][sidx3's
+	// position in the row]: the in-plane neighbours a point takes from the
+	// rows just before its own, which this workgroup stored in the last
+	// three steps (a load from memory would wait for those stores)
+	extern __shared__ char flow_lds_[];
+	T *const ring = reinterpret_cast<T *>(flow_lds_);
 	__shared__ int stop;
 	const int k = xcd_block((int)blockIdx.x, (int)gridDim.x);
 	const int umax = (S.m - 1) + 2 * (S.n - 1);
@@ -1117,7 +1127,7 @@ __global__ __launch_bounds__(kFlowMaxThreads) void k_gs_flow(SkewView S, const T
 	const int s = __builtin_amdgcn_readfirstlane(tid / rpad), jj = tid - s * rpad;
 	const bool watcher = s >= nsw;
 	const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(xs, 0, 0x7fffffff, kBufferWord3);
-	int known_b = 0, known_a = 0, seen_b = 0, seen_a = 0;
+	int known_b = 0, known_a = 0;
 	// this thread's point of a step (if any), and its coefficients and r,
 	// loaded a step ahead
 	T a[27], rv = (T)0;
@@ -1142,9 +1152,7 @@ __global__ __launch_bounds__(kFlowMaxThreads) void k_gs_flow(SkewView S, const T
 	int tau = 0;
 	for (; tau < nsteps; tau++) {
 		if (watcher && jj == 0) {
-			// counts read during the previous step, then polls if short
-			known_b = max(known_b, seen_b);
-			known_a = max(known_a, seen_a);
+			// polls only when the counts last seen are short
 			int nb, na;
 			flow_needs(tau, nsw, umax, &nb, &na);
 			bool ok = true;
@@ -1154,11 +1162,6 @@ __global__ __launch_bounds__(kFlowMaxThreads) void k_gs_flow(SkewView S, const T
 		}
 		__syncthreads();
 		if (stop) break;
-		if (watcher && jj == 0) {
-			// for the next step's wait, in flight with this step's loads
-			if (k > 0) seen_b = ld_agent(prog + k - 1);
-			if (k + 1 < S.o) seen_a = ld_agent(prog + k + 1);
-		}
 		int i = 0, j = 0;
 		const bool live = point(tau, i, j);
 		long long p = 0;
@@ -1170,6 +1173,18 @@ __global__ __launch_bounds__(kFlowMaxThreads) void k_gs_flow(SkewView S, const T
 			for (int e = 0; e < 27; e++) {
 				if (e == 13) continue;
 				const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
+				if (dk == 0 && di + 2 * dj < 0) {
+					// rows u-1 .. u-3 of this plane and sweep: from the ring,
+					// or the halo outside the grid
+					const int ii = i + di, jn = j + dj, un = ii + 2 * jn;
+					if (ii >= 0 && ii < S.m && jn >= 0)
+						v[e] = ring[(s * 4 + (un & 3)) * S.R + (jn - skew_jf(S, un))];
+					else if (PRD)
+						v[e] = ld_x<T>(xr, (int)(sidx3(S, ii, jn, k) * (long long)sizeof(T)), false);
+					else
+						v[e] = (T)0;
+					continue;
+				}
 				v[e] = ld_x<T>(xr, (int)(sidx3(S, i + di, j + dj, k + dk) * (long long)sizeof(T)), dk != 0);
 			}
 			T ax = (T)0;
@@ -1185,7 +1200,11 @@ __global__ __launch_bounds__(kFlowMaxThreads) void k_gs_flow(SkewView S, const T
 		wait_vm_all();
 		__syncthreads();
 		if (watcher && jj == 0 && tau > 0) st_agent(prog + k, tau);
-		if (live) st_x<T>(xr, (int)(p * (long long)sizeof(T)), val);
+		if (live) {
+			st_x<T>(xr, (int)(p * (long long)sizeof(T)), val);
+			const int u = i + 2 * j;
+			ring[(s * 4 + (u & 3)) * S.R + (j - skew_jf(S, u))] = val;
+		}
 		fetch(tau + 1);
 	}
 	wait_vm_all();
@@ -1203,30 +1222,40 @@ __global__ __launch_bounds__(kFlowMaxThreads) void k_gs_flow(SkewView S, const T
 
 struct FlowOccupancy {
 	int cus;
-	int per_cu[17];  // resident workgroups per CU of 64*w threads
+	long long lds;   // LDS bytes per CU
+	int per_cu[2][17];  // resident workgroups per CU of 64*w threads (no dynamic LDS), [PRD]
 };
 template <typename T>
 FlowOccupancy flow_occupancy() {
 	FlowOccupancy f;
 	int dev = 0;
 	hipDeviceProp_t pr;
-	f.cus = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&pr, dev) == hipSuccess) ? pr.multiProcessorCount : 0;
-	f.per_cu[0] = 0;
-	for (int w = 1; w <= 16; w++) {
-		if (64 * w > kFlowMaxThreads ||
-		    hipOccupancyMaxActiveBlocksPerMultiprocessor(&f.per_cu[w], k_gs_flow<T>, 64 * w, 0) != hipSuccess) {
-			f.per_cu[w] = 0;
-			(void)hipGetLastError();
+	const bool ok = hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&pr, dev) == hipSuccess;
+	f.cus = ok ? pr.multiProcessorCount : 0;
+	f.lds = ok ? (long long)pr.maxSharedMemoryPerMultiProcessor : 0;
+	for (int q = 0; q < 2; q++) {
+		f.per_cu[q][0] = 0;
+		for (int w = 1; w <= 16; w++) {
+			const hipError_t e = 64 * w > kFlowMaxThreads ? hipErrorInvalidValue
+				: q ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&f.per_cu[q][w], k_gs_flow<T, true>, 64 * w, 0)
+				    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&f.per_cu[q][w], k_gs_flow<T, false>, 64 * w, 0);
+			if (e != hipSuccess) {
+				f.per_cu[q][w] = 0;
+				(void)hipGetLastError();
+			}
 		}
 	}
 	return f;
 }
 
+// Dynamic LDS of a k_gs_flow launch: the row ring.
+inline size_t flow_lds_bytes(const SkewView &S, int nsw, size_t elem) { return (size_t)nsw * 4 * S.R * elem; }
+
 // Most sweeps per launch of k_gs_flow for a level (0: the level cannot
 // take it): the block must fit kFlowMaxThreads, and every plane's workgroup
 // must be resident at once (see k_gs_flow).
 template <typename T>
-int flow_max_sweeps(const SkewView &S) {
+int flow_max_sweeps(const SkewView &S, bool prd) {
 	const int rpad = flow_row_pad(S);
 	if (rpad + 64 > kFlowMaxThreads || S.o < 1) return 0;
 	// x is addressed with 32-bit byte offsets
@@ -1242,21 +1271,33 @@ int flow_max_sweeps(const SkewView &S) {
 	// take long against a sweep); initialised once even with several
 	// host threads
 	static const FlowOccupancy occ = flow_occupancy<T>();
+	auto resident = [&](int nsw) {
+		// workgroups per CU by threads and by LDS (rounded up to 1 KiB)
+		long long per = occ.per_cu[prd ? 1 : 0][nsw * rpad / 64 + 1];
+		const long long lds = ((long long)flow_lds_bytes(S, nsw, sizeof(T)) + 64 + 1023) / 1024 * 1024;
+		if (occ.lds > 0) per = min(per, occ.lds / lds);
+		return per * occ.cus;
+	};
 	int nsw = min(16, (kFlowMaxThreads - 64) / rpad);
-	while (nsw > 0 && (long long)occ.per_cu[nsw * rpad / 64 + 1] * occ.cus < S.o) nsw--;
+	while (nsw > 0 && resident(nsw) < S.o) nsw--;
 	return nsw;
 }
 
 template <typename T>
-bool launch_gs_flow(hipStream_t s, const SkewView &S, const T *As, T *xs, const T *rs, int nsweeps, int *prog,
-		    int *err, int limit) {
-	const int smax = flow_max_sweeps<T>(S);
+bool launch_gs_flow(hipStream_t s, const SkewView &S, bool prd, const T *As, T *xs, const T *rs, int nsweeps,
+		    int *prog, int *err, int limit) {
+	const int smax = flow_max_sweeps<T>(S, prd);
 	if (smax < 1) return false;
 	const int rpad = flow_row_pad(S);
 	for (int done = 0; done < nsweeps; done += smax) {
 		const int nsw = min(smax, nsweeps - done);
-		hipLaunchKernelGGL(k_gs_flow<T>, dim3(S.o), dim3(nsw * rpad + 64), 0, s, S, As, xs, rs, nsw, rpad, prog, err,
-				   limit);
+		const size_t lds = flow_lds_bytes(S, nsw, sizeof(T));
+		if (prd)
+			hipLaunchKernelGGL((k_gs_flow<T, true>), dim3(S.o), dim3(nsw * rpad + 64), lds, s, S, As, xs, rs, nsw,
+					   rpad, prog, err, limit);
+		else
+			hipLaunchKernelGGL((k_gs_flow<T, false>), dim3(S.o), dim3(nsw * rpad + 64), lds, s, S, As, xs, rs, nsw,
+					   rpad, prog, err, limit);
 		HIPCHECK_LAUNCH();
 	}
 	return true;
@@ -3252,8 +3293,8 @@ void launch_b_add_x(hipStream_t s, const SlotDev *slots, WorkList wl, int chunk)
 	template void launch_gs_plane<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int); \
 	template void launch_gs_planes<T>(hipStream_t, const LevelView &, const T *, T *, const T *, int); \
 	template void launch_gs_planes_skew<T>(hipStream_t, const SkewView &, const T *, T *, const T *, int); \
-	template bool launch_gs_flow<T>(hipStream_t, const SkewView &, const T *, T *, const T *, int, int *, int *, int); \
-	template int flow_max_sweeps<T>(const SkewView &); \
+	template bool launch_gs_flow<T>(hipStream_t, const SkewView &, bool, const T *, T *, const T *, int, int *, int *, int); \
+	template int flow_max_sweeps<T>(const SkewView &, bool); \
 	template void launch_skew_pack_op<T>(hipStream_t, const LevelView &, const SkewView &, const T *, T *); \
 	template void launch_skew_vec<T>(hipStream_t, const LevelView &, const SkewView &, const T *, T *, bool); \
 	template void launch_skew_halo_wrap<T>(hipStream_t, const LevelView &, const SkewView &, T *); \
