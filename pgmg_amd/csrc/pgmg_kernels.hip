@@ -2032,13 +2032,19 @@ void launch_reduce(hipStream_t s, const double *partial, int n, double *out) {
 // (fine M-1) also belongs to coarse 0, which the reference reaches through
 // its ghost strip.  In cube-scan order that cube comes after cube 0, so for
 // coarse 0 the axis visits cube 0 first and the wrapped last cube second.
+// A periodic axis of size 2 has one cube, which is also the last: its local 1
+// reaches coarse 0 both as that point's own share and through the wrap, and
+// the scatter adds the share twice in a row as it visits the cube (twice per
+// such axis, so four times with two of them), not in a second pass.
 template <typename T>
 __device__ __forceinline__ void restrict_point(const LevelView &F, const LevelView &C, const T *__restrict__ res,
 					       T *__restrict__ rc, T *__restrict__ xc, int I, int J, int K) {
 	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
 	const bool px = F.prd & 1, py = (F.prd >> 1) & 1, pz = (F.prd >> 2) & 1;
 	const bool evx = !(F.m & 1) && !px, evy = !(F.n & 1) && !py, evz = !(F.o & 1) && !pz;
-	const bool wx = px && !(F.m & 1) && I == 0, wy = py && !(F.n & 1) && J == 0, wz = pz && !(F.o & 1) && K == 0;
+	const bool dx = px && F.m == 2, dy = py && F.n == 2, dz = pz && F.o == 2;
+	const bool wx = px && !(F.m & 1) && I == 0 && !dx, wy = py && !(F.n & 1) && J == 0 && !dy,
+		   wz = pz && !(F.o & 1) && K == 0 && !dz;
 	T acc = (T)0;
 	for (int oz = 0; oz < 2; oz++) {
 		const int cz = wz ? 1 - oz : oz;
@@ -2078,7 +2084,10 @@ __device__ __forceinline__ void restrict_point(const LevelView &F, const LevelVi
 							else if (cx == 0) wx = sx ? (T)1 : (T)0.5;
 							else { if (sx) continue; wx = (T)0.5; }
 							const T w = (T)0.125 * wx * wy * wz;
-							acc = acc + w * res[idx3(F, fi, fj, fk)];
+							const T t = w * res[idx3(F, fi, fj, fk)];
+							acc = acc + t;
+							const int times = (dx && lx ? 2 : 1) * (dy && ly ? 2 : 1) * (dz && lz ? 2 : 1);
+							for (int q = 1; q < times; q++) acc = acc + t;
 						}
 					}
 				}

@@ -46,6 +46,10 @@ PFIXTURES = [
     ("prd_f7log_20x17x30_p6", 2, 9, 20, 17, 30, 6),
     ("prd_q1_10x14x12_p7", 3, 4, 10, 14, 12, 7),
     ("prd_f7u_16x16x16_p7", 1, 5, 16, 16, 16, 7),
+    # a periodic axis of size 2 on a restricting level (two on (261, 2, 2))
+    ("prd_f7u_18x29x2_p4", 1, 6, 18, 29, 2, 4),
+    ("prd_q1_2x24x29_p1", 3, 8, 2, 24, 29, 1),
+    ("prd_f7log_520x4x4_p6", 2, 10, 520, 4, 4, 6),
 ]
 # (name, family, seed, m, n, o, bottom, tol)
 SOLVES = [
