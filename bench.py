@@ -581,6 +581,9 @@ def main():
         log("pcie-inclusive:", out["pcie_inclusive_mixed"])
         out["q1_513"] = q1_513(P)
         out["batch_256_mixed"] = batch_256(P, P.PREC_MIXED)
+        # DOF-cycles per second of the batch against the lone 257^3 mixed solve
+        out["batch_256_mixed"]["vs_lone_257"] = round(out["batch_256_mixed"]["dof_cycles_per_s"]
+                                                      / out["dof_per_s_per_vcycle_mixed"], 3)
         log("batch mixed:", out["batch_256_mixed"])
         out["batch_256_fp64"] = batch_256(P, P.PREC_FP64)
         log("batch fp64:", out["batch_256_fp64"])

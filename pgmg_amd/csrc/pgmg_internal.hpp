@@ -167,7 +167,7 @@ struct WorkList {
 // Unit kinds: the colour-pair row groups of a level, one unit per slot,
 // residual row groups, 64x4-point blocks of a level, 64x4 blocks of its
 // 2x2x2 cubes.
-enum { BK_PAIR0 = 0, BK_PAIR1, BK_PAIR2, BK_PAIR3, BK_ONE, BK_ROWS, BK_PTS, BK_CUBES };
+enum { BK_PAIR0 = 0, BK_PAIR1, BK_PAIR2, BK_PAIR3, BK_ONE, BK_ROWS, BK_PTS, BK_CUBES, BK_FLAT };
 // Work units of one slot for a batched kernel kind at a level; elem is the
 // size of the cycle's arithmetic type (4 or 8).
 int batch_units(int kind, const LevelView &L, int elem);

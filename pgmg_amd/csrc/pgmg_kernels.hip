@@ -3061,6 +3061,7 @@ int batch_units(int kind, const LevelView &L, int elem) {
 	case BK_ONE: return 1;
 	case BK_ROWS: return residual_blocks(L, elem);
 	case BK_CUBES: return (((L.m + 1) / 2 + 63) / 64) * (((L.n + 1) / 2 + 3) / 4) * ((L.o + 1) / 2);
+	case BK_FLAT: return (int)(((long long)L.m * L.n * L.o + 255) / 256);
 	default: return ((L.m + 63) / 64) * ((L.n + 3) / 4) * L.o;
 	}
 }
@@ -3094,6 +3095,19 @@ __device__ __forceinline__ bool pts_unit(const LevelView &L, int lu, int &i, int
 	j = ((lu / gx) % gy) * 4 + (int)threadIdx.y;
 	k = lu / (gx * gy);
 	return i < L.m && j < L.n;
+}
+
+// Unit lu of kind BK_FLAT: 256 consecutive grid points in (i, j, k) order,
+// every lane busy whatever the row length.
+__device__ __forceinline__ bool flat_unit(const LevelView &L, int lu, int &i, int &j, int &k) {
+	const long long p = (long long)lu * 256 + (int)threadIdx.y * 64 + (int)threadIdx.x;
+	const long long mn = (long long)L.m * L.n;
+	if (p >= mn * L.o) return false;
+	k = (int)(p / mn);
+	const int q = (int)(p - (long long)k * mn);
+	j = q / L.m;
+	i = q - j * L.m;
+	return true;
 }
 
 // Unit lu of kind BK_CUBES: the fine cubes (2ic.., 2jc.., 2kc..) of level L.
@@ -3219,7 +3233,7 @@ __global__ __launch_bounds__(256) void kb_add_x(const SlotDev *__restrict__ slot
 	BATCH_LOOP
 		const LevelView L = S->lv[0];
 		int i, j, k;
-		if (pts_unit(L, lu, i, j, k)) {
+		if (flat_unit(L, lu, i, j, k)) {
 			const long long p = idx3(L, i, j, k);
 			S->x[0][p] = S->x[0][p] + (double)S->x32[0][p];
 		}
