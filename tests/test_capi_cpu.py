@@ -63,12 +63,13 @@ def test_footprint_scales():
 def test_footprint_counts_exact_skewed_copies():
     # the exact smoother's plane-skewed copies: 29 arrays of
     # R * (m + 2n + 4) * (o + 2) elements per level whose x (halo included)
-    # does not fit the one workgroup's 64 KB of LDS in FP64 (FP64, plus FP32
-    # in mixed precision)
+    # does not fit the one workgroup's 64 KB of LDS in FP64, or that has more
+    # than 1024 points (FP64, plus FP32 in mixed precision)
     def skew_elems(m, n, o):
         R = min((m + 1) // 2 + 2, n + 2)
         return R * (m + 2 * n + 4) * (o + 2)
-    big = [d for d in pgmg_amd.hierarchy_shape(257, 257, 257) if (d[0] + 2) * (d[1] + 2) * (d[2] + 2) * 8 > 65536]
+    big = [d for d in pgmg_amd.hierarchy_shape(257, 257, 257)
+           if (d[0] + 2) * (d[1] + 2) * (d[2] + 2) * 8 > 65536 or d[0] * d[1] * d[2] > 1024]
     extra = sum(29 * skew_elems(*d) for d in big)
     for prec, word in [(pgmg_amd.PREC_FP64, 8), (pgmg_amd.PREC_MIXED, 12)]:
         base = pgmg_amd.footprint(257, 257, 257, precision=prec)
