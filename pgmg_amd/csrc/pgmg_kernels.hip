@@ -1519,7 +1519,21 @@ __device__ __forceinline__ void gs_lds_run(const LevelView &L, const T *__restri
 	const int npair = L.n * L.o;
 	// forward colour sweeps: classes (0,cj,ck) and (1,cj,ck) as one pass
 	const bool staged = !resident && !exact && !mode && small_pairs_per_pass(L) <= nt;
-	if (staged) {
+	if (resident && exact && !mode && !L.prd) {
+		// Forward sweeps pipelined eight hyperplanes apart: sweep s updates
+		// hyperplane t at step t + 8s.  A neighbour moves t by 1 to 7, so
+		// when a point updates, its earlier neighbours hold this sweep's
+		// values and its later ones the previous sweep's, as in separate
+		// sweeps.  (The ghosts of a periodic level must hold each sweep's
+		// starting values, so periodic levels sweep one at a time.)
+		const int steps = tmax + 1 + 8 * (nsweeps - 1);
+#pragma unroll 1
+		for (int st = 0; st < steps; st++) {
+			const int d = st - ot;
+			if (own && d >= 0 && (d & 7) == 0 && (d >> 3) < nsweeps) gs_lds_update<T>(ca, cr, xs, olq, hx, hxy);
+			__syncthreads();
+		}
+	} else if (staged) {
 #pragma unroll 1
 		for (int sw = 0; sw < nsweeps; sw++) {
 			halo_wrap_block<T>(L, xs, 1 + hx + hxy, hx, hxy);

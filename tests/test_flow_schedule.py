@@ -161,3 +161,38 @@ def test_flow_schedule_needs_the_sweep_lag():
         except AssertionError as e:
             stuck += "waits" in str(e)
     assert stuck > 0
+
+
+def pipelined_hyperplanes(A, x0, r, nsw, lag=8):
+    """The one-workgroup schedule: sweep s updates hyperplane t = i+2j+4k at
+    step t + lag*s, every point of a step from the values before the step."""
+    o, n, m = x0.shape
+    tmax = (m - 1) + 2 * (n - 1) + 4 * (o - 1)
+    x = x0.copy()
+    for st in range(tmax + 1 + lag * (nsw - 1)):
+        new = []
+        for k in range(o):
+            for j in range(n):
+                for i in range(m):
+                    d = st - (i + 2 * j + 4 * k)
+                    if d >= 0 and d % lag == 0 and d // lag < nsw:
+                        new.append((k, j, i, gs_value(A, x, r, i, j, k)))
+        for k, j, i, v in new:
+            x[k, j, i] = v
+    return x
+
+
+@pytest.mark.parametrize("dims,nsw", [((9, 9, 9), 4), ((5, 7, 6), 3), ((8, 3, 10), 5)])
+def test_pipelined_hyperplanes_equal_lexicographic_sweeps(dims, nsw):
+    m, n, o = dims
+    A = O.gen_operator(3, 70, m, n, o)  # 27-point Q1: corner neighbours too
+    rng = np.random.default_rng(nsw)
+    x0 = rng.uniform(-1, 1, size=(o, n, m))
+    r = rng.uniform(-1, 1, size=(o, n, m))
+    ref = x0.copy()
+    for _ in range(nsw):
+        ref = O.gs(A, ref, r, 0)
+    assert np.array_equal(pipelined_hyperplanes(A, x0, r, nsw), ref)
+    # seven hyperplanes apart is too close: the corner neighbour (1, 1, 1),
+    # seven planes later, would be read before its previous-sweep update
+    assert not np.array_equal(pipelined_hyperplanes(A, x0, r, nsw, lag=7), ref)
