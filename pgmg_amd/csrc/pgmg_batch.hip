@@ -347,13 +347,25 @@ int run_cycle(pgmg_batch *b) {
 	auto eager = [&]() { if (b->mixed) batched_cycle<float>(b); else batched_cycle<double>(b); };
 	if (!b->opts.use_graph || !b->tune.done) { eager(); return PGMG_OK; }
 	if (!b->cycle_graph && ++b->plan_steps >= 2) {
-		hipGraph_t g;
-		BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-		eager();
-		BHIP(hipStreamEndCapture(b->stream, &g));
-		const hipError_t e = hipGraphInstantiate(&b->cycle_graph, g, nullptr, nullptr, 0);
-		hipGraphDestroy(g);
-		if (e != hipSuccess) { b->cycle_graph = nullptr; return bfail(PGMG_ERR_HIP, "batch graph instantiate: %s", hipGetErrorString(e)); }
+		// A capture only records: if it cannot be made (the stream already
+		// capturing, instantiation refused), the batch runs eagerly from
+		// here on, this cycle included.
+		hipGraph_t g = nullptr;
+		hipGraphExec_t ge = nullptr;
+		bool ok = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+		if (ok) {
+			eager();
+			ok = hipStreamEndCapture(b->stream, &g) == hipSuccess && g;
+			ok = ok && hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) == hipSuccess;
+			if (g) hipGraphDestroy(g);
+		}
+		if (!ok) {
+			(void)hipGetLastError();
+			b->opts.use_graph = 0;
+			eager();
+			return PGMG_OK;
+		}
+		b->cycle_graph = ge;
 	}
 	if (b->cycle_graph) {
 		BHIP(hipGraphLaunch(b->cycle_graph, b->stream));
