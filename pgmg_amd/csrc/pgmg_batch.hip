@@ -178,10 +178,6 @@ void drop_graph(pgmg_batch *b) {
 	b->plan_steps = 0;
 }
 
-}  // namespace
-
-namespace {
-
 int list_index(int level, int kind) { return L_TOP_COUNT + level * L_PER_LEVEL + kind; }
 int n_lists() { return L_TOP_COUNT + kBatchMaxLevels * L_PER_LEVEL; }
 
