@@ -1625,17 +1625,22 @@ void launch_gs_small(hipStream_t s, const LevelView &L, const T *A, T *x, const 
 
 // ------------------------------------------------------ residual, norms
 
-// Deterministic block sum of 256 values (4 wavefronts of 64).
+// Deterministic block sum of 256 values (4 wavefronts of 64): a fixed
+// shuffle tree inside each wavefront (lane l adds lane l + w, w = 32 .. 1),
+// then the four wavefront sums as (s0 + s1) + (s2 + s3).  Two barriers
+// instead of the nine of a 256-slot LDS tree: the 256-problem batch, whose
+// residual blocks hold few points, 14.3 -> 13.8 s.  Every thread gets the
+// sum, and the LDS slots may be reused as soon as it returns.
 __device__ __forceinline__ double block_sum_256(double v) {
-	__shared__ double red[256];
+	__shared__ double red[4];
 	const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-	red[tid] = v;
+#pragma unroll
+	for (int w = 32; w > 0; w >>= 1) v = v + __shfl_down(v, w, 64);
+	if ((tid & 63) == 0) red[tid >> 6] = v;
 	__syncthreads();
-	for (int w = 128; w > 0; w >>= 1) {
-		if (tid < w) red[tid] = red[tid] + red[tid + w];
-		__syncthreads();
-	}
-	return red[0];
+	const double s = (red[0] + red[1]) + (red[2] + red[3]);
+	__syncthreads();
+	return s;
 }
 
 // Residual r - A x and per-block sums of its square.  One wavefront per grid
