@@ -1627,10 +1627,9 @@ void launch_gs_small(hipStream_t s, const LevelView &L, const T *A, T *x, const 
 
 // Deterministic block sum of 256 values (4 wavefronts of 64): a fixed
 // shuffle tree inside each wavefront (lane l adds lane l + w, w = 32 .. 1),
-// then the four wavefront sums as (s0 + s1) + (s2 + s3).  Two barriers
-// instead of the nine of a 256-slot LDS tree: the 256-problem batch, whose
-// residual blocks hold few points, 14.3 -> 13.8 s.  Every thread gets the
-// sum, and the LDS slots may be reused as soon as it returns.
+// then the four wavefront sums as (s0 + s1) + (s2 + s3): two barriers
+// instead of the nine of a 256-slot LDS tree.  Every thread gets the sum,
+// and the LDS slots may be reused as soon as it returns.
 __device__ __forceinline__ double block_sum_256(double v) {
 	__shared__ double red[4];
 	const int tid = threadIdx.x + threadIdx.y * blockDim.x;
