@@ -2171,6 +2171,74 @@ __device__ __forceinline__ void restrict_any(const LevelView &F, const LevelView
 		restrict_point<T>(F, C, res, rc, xc, I, J, K);
 }
 
+// The lone hierarchy restricts onto a coarse level of at least
+// kRestrictSplitPoints points in two launches: the coarse points whose eight
+// surrounding cubes are interior (restrict_interior, every lane on the same
+// path), and the shell of the others (restrict_point).  One launch over all
+// points put a boundary point into a third of the wavefronts (I = 0 and the
+// last I), which then ran both paths: 257^3 -> 129^3 in FP32 87 us, split
+// 37 us; onto 65^3 30 -> 22 us; onto 33^3 the second launch cost more (18 ->
+// 21 us).
+const long long kRestrictSplitPoints = 100000;
+//
+// Coarse indices on an axis of coarse size n whose last full cube is `last`:
+// interior [1, last), shell {0} and [max(last, 1), n).
+struct ShellAxis {
+	int n, nin, nsh;  // points, interior points, shell points
+	int hi;           // first index of the upper shell
+	__host__ __device__ ShellAxis(int n_, int last) : n(n_) {
+		nin = last > 1 ? min(last, n_) - 1 : 0;
+		nsh = n_ - nin;
+		hi = max(last, 1);
+	}
+	__host__ __device__ int shell(int p) const { return p < 1 ? 0 : hi + (p - 1); }
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_restrict_interior(LevelView F, LevelView C, const T *__restrict__ res,
+							   T *__restrict__ rc, T *__restrict__ xc) {
+	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1;
+	const int I = 1 + (int)(blockIdx.x * 64 + threadIdx.x), J = 1 + (int)(blockIdx.y * 4 + threadIdx.y);
+	const int K = 1 + (int)blockIdx.z;
+	if (I >= lastx || J >= lasty) return;
+	restrict_interior<T>(F, C, res, rc, xc, I, J, K);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_restrict_shell(LevelView F, LevelView C, const T *__restrict__ res,
+							T *__restrict__ rc, T *__restrict__ xc, long long total) {
+	const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+	if (q >= total) return;
+	const ShellAxis ax(C.m, (F.m - 1) >> 1), ay(C.n, (F.n - 1) >> 1), az(C.o, (F.o - 1) >> 1);
+	const long long plane = (long long)C.m * C.n;
+	const long long faces = (long long)az.nsh * plane;
+	const long long ring = (long long)ay.nsh * C.m + (long long)ay.nin * ax.nsh;
+	int I, J, K;
+	if (q < faces) {
+		// a whole plane of the shell in K
+		K = az.shell((int)(q / plane));
+		const long long rem = q % plane;
+		J = (int)(rem / C.m);
+		I = (int)(rem % C.m);
+	} else {
+		const long long q2 = q - faces;
+		K = 1 + (int)(q2 / ring);
+		const long long r = q2 % ring;
+		if (r < (long long)ay.nsh * C.m) {
+			// a whole row of the shell in J
+			J = ay.shell((int)(r / C.m));
+			I = (int)(r % C.m);
+		} else {
+			const long long r2 = r - (long long)ay.nsh * C.m;
+			J = 1 + (int)(r2 / ax.nsh);
+			I = ax.shell((int)(r2 % ax.nsh));
+		}
+	}
+	restrict_point<T>(F, C, res, rc, xc, I, J, K);
+}
+
+// One launch over all points, for small coarse levels, where a second launch
+// costs more than the divergence it saves.
 template <typename T>
 __global__ __launch_bounds__(256) void k_restrict(LevelView F, LevelView C, const T *__restrict__ res,
 						  T *__restrict__ rc, T *__restrict__ xc) {
@@ -2181,9 +2249,25 @@ __global__ __launch_bounds__(256) void k_restrict(LevelView F, LevelView C, cons
 
 template <typename T>
 void launch_restrict(hipStream_t s, const LevelView &F, const LevelView &C, const T *res, T *rc, T *xc) {
-	dim3 grid((C.m + 63) / 64, (C.n + 3) / 4, C.o), block(64, 4, 1);
-	hipLaunchKernelGGL(k_restrict<T>, grid, block, 0, s, F, C, res, rc, xc);
-	HIPCHECK_LAUNCH();
+	if ((long long)C.m * C.n * C.o < kRestrictSplitPoints) {
+		dim3 grid((C.m + 63) / 64, (C.n + 3) / 4, C.o), block(64, 4, 1);
+		hipLaunchKernelGGL(k_restrict<T>, grid, block, 0, s, F, C, res, rc, xc);
+		HIPCHECK_LAUNCH();
+		return;
+	}
+	const ShellAxis ax(C.m, (F.m - 1) >> 1), ay(C.n, (F.n - 1) >> 1), az(C.o, (F.o - 1) >> 1);
+	if (ax.nin > 0 && ay.nin > 0 && az.nin > 0) {
+		dim3 grid((ax.nin + 63) / 64, (ay.nin + 3) / 4, az.nin), block(64, 4, 1);
+		hipLaunchKernelGGL(k_restrict_interior<T>, grid, block, 0, s, F, C, res, rc, xc);
+		HIPCHECK_LAUNCH();
+	}
+	const long long total = (long long)az.nsh * C.m * C.n +
+				(long long)az.nin * ((long long)ay.nsh * C.m + (long long)ay.nin * ax.nsh);
+	if (total > 0) {
+		hipLaunchKernelGGL(k_restrict_shell<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, F, C, res, rc, xc,
+				   total);
+		HIPCHECK_LAUNCH();
+	}
 }
 
 // Interpolation x_f += P x_c.  Fine point (2ic+li, 2jc+lj, 2kc+lk) takes its
