@@ -220,7 +220,7 @@ int build_plan(pgmg_batch *b) {
 					plan_add(b, list_index(l, L_SMALL), s, 1);
 				else for (int q = 0; q < 4; q++) plan_add(b, list_index(l, L_PAIR0 + q), s, batch_units(BK_PAIR0 + q, L, el));
 				plan_add(b, list_index(l, L_RES), s, batch_units(BK_ROWS, L, el));
-				plan_add(b, list_index(l + 1, L_RESTRICT), s, batch_units(BK_PTS, D.lv[l + 1], el));
+				plan_add(b, list_index(l + 1, L_RESTRICT), s, batch_restrict_units(L, D.lv[l + 1]));
 				plan_add(b, list_index(l + 1, L_INTERP), s, batch_units(BK_CUBES, L, el));
 			} else if (D.lu_active) {
 				plan_add(b, list_index(l, L_LU), s, 1);

@@ -171,6 +171,9 @@ enum { BK_PAIR0 = 0, BK_PAIR1, BK_PAIR2, BK_PAIR3, BK_ONE, BK_ROWS, BK_PTS, BK_C
 // Work units of one slot for a batched kernel kind at a level; elem is the
 // size of the cycle's arithmetic type (4 or 8).
 int batch_units(int kind, const LevelView &L, int elem);
+// Work units of a batched restriction onto C from F (interior tiles, then
+// shell blocks).
+int batch_restrict_units(const LevelView &F, const LevelView &C);
 
 // Batched launchers: one launch covers the level `level` of every slot in
 // the list; blocks walk the units grid-stride.  chunk: work units per

@@ -2194,6 +2194,56 @@ struct ShellAxis {
 	__host__ __device__ int shell(int p) const { return p < 1 ? 0 : hi + (p - 1); }
 };
 
+// Shell point q (0 <= q < shell_count) of the coarse level: the planes of
+// the shell in K whole, then in every interior plane the shell rows in J
+// whole and the shell points of the interior rows.
+__host__ __device__ inline long long shell_count(const ShellAxis &ax, const ShellAxis &ay, const ShellAxis &az) {
+	return (long long)az.nsh * ax.n * ay.n + (long long)az.nin * ((long long)ay.nsh * ax.n + (long long)ay.nin * ax.nsh);
+}
+__device__ __forceinline__ void shell_point(const ShellAxis &ax, const ShellAxis &ay, const ShellAxis &az, long long q,
+					   int &I, int &J, int &K) {
+	const long long plane = (long long)ax.n * ay.n;
+	const long long faces = (long long)az.nsh * plane;
+	const long long ring = (long long)ay.nsh * ax.n + (long long)ay.nin * ax.nsh;
+	if (q < faces) {
+		K = az.shell((int)(q / plane));
+		const long long rem = q % plane;
+		J = (int)(rem / ax.n);
+		I = (int)(rem % ax.n);
+	} else {
+		const long long q2 = q - faces;
+		K = 1 + (int)(q2 / ring);
+		const long long r = q2 % ring;
+		if (r < (long long)ay.nsh * ax.n) {
+			J = ay.shell((int)(r / ax.n));
+			I = (int)(r % ax.n);
+		} else {
+			const long long r2 = r - (long long)ay.nsh * ax.n;
+			J = 1 + (int)(r2 / ax.nsh);
+			I = ax.shell((int)(r2 % ax.nsh));
+		}
+	}
+}
+
+// Work units of a batched restriction onto C from F: 64 x 4 tiles of the
+// interior points first, then blocks of 256 shell points, so that no
+// workgroup mixes the two gathers.
+struct RestrictUnits {
+	ShellAxis ax, ay, az;
+	int gx, gy;
+	long long tiles, shell;
+	__host__ __device__ RestrictUnits(const LevelView &F, const LevelView &C)
+	    : ax(C.m, (F.m - 1) >> 1), ay(C.n, (F.n - 1) >> 1), az(C.o, (F.o - 1) >> 1) {
+		gx = (ax.nin + 63) / 64;
+		gy = (ay.nin + 3) / 4;
+		tiles = (ax.nin > 0 && ay.nin > 0 && az.nin > 0) ? (long long)gx * gy * az.nin : 0;
+		shell = shell_count(ax, ay, az);
+	}
+	__host__ __device__ long long units() const { return tiles + (shell + 255) / 256; }
+};
+
+int batch_restrict_units(const LevelView &F, const LevelView &C) { return (int)RestrictUnits(F, C).units(); }
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_restrict_interior(LevelView F, LevelView C, const T *__restrict__ res,
 							   T *__restrict__ rc, T *__restrict__ xc) {
@@ -2210,30 +2260,8 @@ __global__ __launch_bounds__(256) void k_restrict_shell(LevelView F, LevelView C
 	const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
 	if (q >= total) return;
 	const ShellAxis ax(C.m, (F.m - 1) >> 1), ay(C.n, (F.n - 1) >> 1), az(C.o, (F.o - 1) >> 1);
-	const long long plane = (long long)C.m * C.n;
-	const long long faces = (long long)az.nsh * plane;
-	const long long ring = (long long)ay.nsh * C.m + (long long)ay.nin * ax.nsh;
 	int I, J, K;
-	if (q < faces) {
-		// a whole plane of the shell in K
-		K = az.shell((int)(q / plane));
-		const long long rem = q % plane;
-		J = (int)(rem / C.m);
-		I = (int)(rem % C.m);
-	} else {
-		const long long q2 = q - faces;
-		K = 1 + (int)(q2 / ring);
-		const long long r = q2 % ring;
-		if (r < (long long)ay.nsh * C.m) {
-			// a whole row of the shell in J
-			J = ay.shell((int)(r / C.m));
-			I = (int)(r % C.m);
-		} else {
-			const long long r2 = r - (long long)ay.nsh * C.m;
-			J = 1 + (int)(r2 / ax.nsh);
-			I = ax.shell((int)(r2 % ax.nsh));
-		}
-	}
+	shell_point(ax, ay, az, q, I, J, K);
 	restrict_point<T>(F, C, res, rc, xc, I, J, K);
 }
 
@@ -2261,8 +2289,7 @@ void launch_restrict(hipStream_t s, const LevelView &F, const LevelView &C, cons
 		hipLaunchKernelGGL(k_restrict_interior<T>, grid, block, 0, s, F, C, res, rc, xc);
 		HIPCHECK_LAUNCH();
 	}
-	const long long total = (long long)az.nsh * C.m * C.n +
-				(long long)az.nin * ((long long)ay.nsh * C.m + (long long)ay.nin * ax.nsh);
+	const long long total = shell_count(ax, ay, az);
 	if (total > 0) {
 		hipLaunchKernelGGL(k_restrict_shell<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, F, C, res, rc, xc,
 				   total);
@@ -3306,10 +3333,22 @@ __global__ __launch_bounds__(1024) void kb_reduce(const SlotDev *__restrict__ sl
 template <typename T>
 __global__ __launch_bounds__(256) void kb_restrict(const SlotDev *__restrict__ slots, WorkList wl, int level) {
 	BATCH_LOOP
-		const LevelView C = S->lv[level];
-		int I, J, K;
-		if (pts_unit(C, lu, I, J, K))
-			restrict_any<T>(S->lv[level - 1], C, sres<T>(S), sr<T>(S, level), sx<T>(S, level), I, J, K);
+		const LevelView F = S->lv[level - 1], C = S->lv[level];
+		const RestrictUnits R(F, C);
+		if (lu < R.tiles) {
+			const int I = 1 + (int)(lu % R.gx) * 64 + (int)threadIdx.x;
+			const int J = 1 + (int)((lu / R.gx) % R.gy) * 4 + (int)threadIdx.y;
+			const int K = 1 + (int)(lu / ((long long)R.gx * R.gy));
+			if (I <= R.ax.nin && J <= R.ay.nin)
+				restrict_interior<T>(F, C, sres<T>(S), sr<T>(S, level), sx<T>(S, level), I, J, K);
+		} else {
+			const long long q = (lu - R.tiles) * 256 + (int)threadIdx.y * 64 + (int)threadIdx.x;
+			if (q < R.shell) {
+				int I, J, K;
+				shell_point(R.ax, R.ay, R.az, q, I, J, K);
+				restrict_point<T>(F, C, sres<T>(S), sr<T>(S, level), sx<T>(S, level), I, J, K);
+			}
+		}
 	}
 }
 
