@@ -1321,17 +1321,6 @@ void launch_skew_pack_op(hipStream_t s, const LevelView &L, const SkewView &S, c
 	HIPCHECK_LAUNCH();
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_skew_vec(LevelView L, SkewView S, const T *__restrict__ v,
-						  T *__restrict__ vs, int to_skew) {
-	const long long N = (long long)L.m * L.n * L.o;
-	for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < N; p += (long long)gridDim.x * 256) {
-		const int i = (int)(p % L.m), j = (int)((p / L.m) % L.n), k = (int)(p / ((long long)L.m * L.n));
-		if (to_skew) vs[sidx3(S, i, j, k)] = v[idx3(L, i, j, k)];
-		else vs[idx3(L, i, j, k)] = v[sidx3(S, i, j, k)];
-	}
-}
-
 // Ghost copies of a periodic level's halo on its skewed copy (as
 // k_halo_wrap on the level layout).
 template <typename T>
@@ -1349,10 +1338,54 @@ void launch_skew_halo_wrap(hipStream_t s, const LevelView &L, const SkewView &S,
 	HIPCHECK_LAUNCH();
 }
 
+// A vector between the level layout and the plane-skewed one, through LDS:
+// a tile of 64 i by 16 j of one k-plane per workgroup.  Level rows are read
+// (or written) 64 points at a time, and the tile's part of each skewed row
+// u = i + 2j, up to 16 consecutive j, is written (or read) as one run.
+// (Copied point by point, consecutive i are a skewed row apart, so every
+// access on that side touched a line of its own.)
+constexpr int kSkewTileI = 64, kSkewTileJ = 16;
+template <typename T>
+__global__ __launch_bounds__(256) void k_skew_tile(LevelView L, SkewView S, const T *__restrict__ v, T *__restrict__ vs,
+						   int to_skew) {
+	__shared__ T tile[kSkewTileJ][kSkewTileI + 1];
+	const int i0 = (int)blockIdx.x * kSkewTileI, j0 = (int)blockIdx.y * kSkewTileJ, k = (int)blockIdx.z;
+	const int tid = (int)threadIdx.x;
+	// skewed rows the tile touches, and the (u, jj) slots of the skewed side
+	const int ulo = i0 + 2 * j0;
+	const int slots = (kSkewTileI - 1 + 2 * (kSkewTileJ - 1) + 1) * kSkewTileJ;
+	if (to_skew) {
+		for (int q = tid; q < kSkewTileI * kSkewTileJ; q += 256) {
+			const int jj = q / kSkewTileI, ii = q % kSkewTileI;
+			const int i = i0 + ii, j = j0 + jj;
+			if (i < L.m && j < L.n) tile[jj][ii] = v[idx3(L, i, j, k)];
+		}
+		__syncthreads();
+		for (int q = tid; q < slots; q += 256) {
+			const int u = ulo + q / kSkewTileJ, jj = q % kSkewTileJ;
+			const int j = j0 + jj, i = u - 2 * j;
+			if (i >= i0 && i < i0 + kSkewTileI && i < L.m && j < L.n) vs[sidx3(S, i, j, k)] = tile[jj][i - i0];
+		}
+	} else {
+		for (int q = tid; q < slots; q += 256) {
+			const int u = ulo + q / kSkewTileJ, jj = q % kSkewTileJ;
+			const int j = j0 + jj, i = u - 2 * j;
+			if (i >= i0 && i < i0 + kSkewTileI && i < L.m && j < L.n) tile[jj][i - i0] = v[sidx3(S, i, j, k)];
+		}
+		__syncthreads();
+		for (int q = tid; q < kSkewTileI * kSkewTileJ; q += 256) {
+			const int jj = q / kSkewTileI, ii = q % kSkewTileI;
+			const int i = i0 + ii, j = j0 + jj;
+			if (i < L.m && j < L.n) vs[idx3(L, i, j, k)] = tile[jj][ii];
+		}
+	}
+}
+
 // to_skew: vs (skewed) = v (level layout); else vs (level layout) = v (skewed).
 template <typename T>
 void launch_skew_vec(hipStream_t s, const LevelView &L, const SkewView &S, const T *v, T *vs, bool to_skew) {
-	hipLaunchKernelGGL(k_skew_vec<T>, dim3(2048), dim3(256), 0, s, L, S, v, vs, to_skew ? 1 : 0);
+	const dim3 grid((L.m + kSkewTileI - 1) / kSkewTileI, (L.n + kSkewTileJ - 1) / kSkewTileJ, L.o);
+	hipLaunchKernelGGL(k_skew_tile<T>, grid, dim3(256), 0, s, L, S, v, vs, to_skew ? 1 : 0);
 	HIPCHECK_LAUNCH();
 }
 
