@@ -240,14 +240,20 @@ def test_solve_257_exact_mode_matches_reference():
     h.close()
 
 
-@pytest.mark.parametrize("precision,family", [(P.PREC_FP64, P.FAM_FACE7_UNIFORM), (P.PREC_MIXED, P.FAM_FACE7_LOGUNI)],
-                         ids=["fp64", "mixed"])
-def test_colour8_129_bitwise_vs_oracle(precision, family):
+@pytest.mark.parametrize("dims,precision,family",
+                         [((129, 129, 129), P.PREC_FP64, P.FAM_FACE7_UNIFORM),
+                          ((129, 129, 129), P.PREC_MIXED, P.FAM_FACE7_LOGUNI),
+                          ((140, 100, 60), P.PREC_FP64, P.FAM_Q1_ANISO),
+                          ((142, 98, 64), P.PREC_MIXED, P.FAM_FACE7_UNIFORM)],
+                         ids=["129-fp64", "129-mixed", "140x100x60-fp64-q1", "142x98x64-mixed"])
+def test_colour8_129_bitwise_vs_oracle(dims, precision, family):
     """At 129^3 (every level on the colour-pair, residual and transfer
-    kernels down to 17^3): two cycles of the 8-colour solver, FP64 or mixed,
-    bit-identical to the oracle's."""
-    m = n = o = 129
-    say("case 129^3 colour8", precision)
+    kernels down to 17^3), and on even non-cubic grids whose first coarse
+    level is restricted in two launches (interior and shell, partial cubes
+    and the injection flags on the shell): two cycles of the 8-colour solver,
+    FP64 or mixed, bit-identical to the oracle's."""
+    m, n, o = dims
+    say("case colour8", dims, precision)
     A = O.gen_operator(family, 31, m, n, o)
     b = O.gen_rhs(31, m, n, o)
     h = P.Hierarchy(m, n, o, precision=precision, smoother=P.SMOOTH_COLOUR8)
@@ -258,7 +264,7 @@ def test_colour8_129_bitwise_vs_oracle(precision, family):
     c, hist = h.solve(1e-30, 2, check=False)
     c2, hist2 = mg.solve(1e-30, 2, mixed=precision == P.PREC_MIXED)
     assert c == c2 == 2
-    # the oracle sums 2.1e6 squares in sequence: error bound N*u = 2.3e-10
+    # the oracle sums up to 2.1e6 squares in sequence: error bound N*u = 2.3e-10
     np.testing.assert_allclose(hist, hist2, rtol=2.3e-10)
     assert np.array_equal(h.get_x(0), mg.get_x(0))
     h.close()
