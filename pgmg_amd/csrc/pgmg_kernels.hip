@@ -224,6 +224,32 @@ __device__ __forceinline__ bool zero_row(int cj, int ck, int dj, int dk) {
 // which feed the even point that lane takes from the hand-over.  Per-point
 // arithmetic and the colour order are those of k_gs_colour, so the result is
 // bit-identical.  j and k must be wavefront-uniform.
+//
+// The point out's 27 coefficients and the x of the point after it in each of
+// its 9 rows are loaded by lanes 0..26 and 0..8, one load each, and lane 63
+// reads them from those lanes (v_readlane) instead of issuing 36 loads of its
+// own: those doubled the load instructions of the segment (level-0 FP64
+// smoother 216 -> 206 us).
+template <typename T>
+__device__ __forceinline__ T read_lane(T v, int l) {
+	if constexpr (sizeof(T) == 8) {
+		unsigned long long u;
+		__builtin_memcpy(&u, &v, 8);
+		const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u & 0xffffffffu), l);
+		const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+		u = ((unsigned long long)hi << 32) | lo;
+		T w;
+		__builtin_memcpy(&w, &u, 8);
+		return w;
+	} else {
+		unsigned u;
+		__builtin_memcpy(&u, &v, 4);
+		u = (unsigned)__builtin_amdgcn_readlane((int)u, l);
+		T w;
+		__builtin_memcpy(&w, &u, 4);
+		return w;
+	}
+}
 template <typename T>
 __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restrict__ A, T *__restrict__ x,
 					    const T *__restrict__ r, int j, int k) {
@@ -246,8 +272,18 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 		const bool doE = live && !(lane == 0 && s0 > 0);
 		const bool doO = live && i0 + 1 < L.m;
 		const bool ext = lane == 63 && i0 + 2 < L.m;  // point out s0+128
+		const bool wext = s0 + 128 < L.m;              // the wavefront has one
 		const uint32_t ob = (uint32_t)i0 * (uint32_t)sizeof(T);
 		const uint32_t oz = ob + 2u * (uint32_t)sizeof(T);
+		// lanes 0..26: coefficient `lane` of the point out; lanes 0..8: x of
+		// the point after it (s0+129, the halo where the row ends) in row
+		// (dj, dk) = (lane % 3 - 1, lane / 3 - 1)
+		T zco = (T)0, zx3 = (T)0;
+		if (wext) {
+			const uint32_t oz0 = (uint32_t)(s0 + 128) * (uint32_t)sizeof(T);
+			if (lane < 27) zco = ld_row<T>(Ar + lane * px, oz0);
+			if (lane < 9) zx3 = ld_row<T>(xr + (lane % 3 - 1) * px + (lane / 3 - 1) * pxy + 1, oz0);
+		}
 		// Streamed row by row (dj,dk): the even point's sum is formed as the
 		// loads arrive; the odd point's terms before its own row are summed
 		// at once, its own-row coefficients are kept, and its 12 terms after
@@ -292,10 +328,10 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 				}
 				if (ext) {
 					// point out i0+2: neighbours i0+1 (xv.y), i0+2 (xp), i0+3
-					const T zm = ld_row<T>(Ar + (3 * rq) * px, oz);
-					const T z0 = ld_row<T>(Ar + (3 * rq + 1) * px, oz);
-					const T zp = ld_row<T>(Ar + (3 * rq + 2) * px, oz);
-					const T x3 = ld_row<T>(xq + 3, ob);
+					const T zm = read_lane(zco, 3 * rq);
+					const T z0 = read_lane(zco, 3 * rq + 1);
+					const T zp = read_lane(zco, 3 * rq + 2);
+					const T x3 = read_lane(zx3, rq);
 					az = az + zm * xv.y;
 					if (rq == 4) ac_z = z0;
 					else az = az + z0 * xp;
