@@ -325,9 +325,11 @@ def batch_256(P, precision, tol=1e-8, count=256, max_cycles=1000, use_graph=True
     conv = sum(1 for s in st if s[0] == P.SLOT_CONVERGED)
     pts = sum(m * n * o for (m, n, o) in sizes)
     cyc = [s[1] for s in st]
+    # each slot's grid points times the cycles it ran: the V-cycle work done
+    work = sum(m * n * o * c for (m, n, o), c in zip(sizes, cyc))
     return {"problems": count, "converged": conv, "solve_s": round(t, 3), "steps": steps,
             "cycles_mean": round(float(np.mean(cyc)), 1), "cycles_max": int(max(cyc)),
-            "points_total": pts, "dof_cycles_per_s": round(pts * float(np.mean(cyc)) / t, 1),
+            "points_total": pts, "point_cycles": int(work), "dof_cycles_per_s": round(work / t, 1),
             "arena_gb": round(need / 1e9, 2), "tiles": tiles, "graph_replays": replays}
 
 

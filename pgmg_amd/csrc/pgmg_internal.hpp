@@ -23,6 +23,7 @@ namespace pgmg {
 struct LevelView {
 	int m, n, o;       // grid points per axis
 	int px;            // row pitch (elements)
+	int pa;            // coefficient row pitch (elements): m rounded up to 4
 	long long pxy;     // plane pitch (elements)
 	long long base;    // index of point (0,0,0)
 	long long npad;    // elements per array, halo included
@@ -36,6 +37,7 @@ inline LevelView make_level_view(int m, int n, int o) {
 	LevelView v;
 	v.m = m; v.n = n; v.o = o;
 	v.px = ((m + 7 + 15) / 16) * 16;  // i = -1 .. m+2 fit: a quad at i0 <= m-1 reads i0+3
+	v.pa = ((m + 3) / 4) * 4;
 	v.pxy = (long long)v.px * (n + 2);
 	v.base = 4 + v.px + v.pxy;
 	v.npad = v.pxy * (o + 2);
@@ -94,16 +96,21 @@ __host__ __device__ inline long long idx3(const LevelView &L, int i, int j, int 
 	return L.base + i + (long long)L.px * j + L.pxy * k;
 }
 // Coefficients are interleaved by grid row: the 27 planes' rows of grid row
-// (j,k) are one contiguous block of 27*px elements, plane e's row at offset
-// e*px in it, entry i at +4+i as in the per-point arrays.  A wavefront that
-// streams one grid row therefore reads one contiguous 27*px block (one or
-// two memory pages), not 27 rows a plane apart.  The array keeps the size of
-// 27 per-point arrays; the halo rows' blocks are unused.
+// (j,k) are one contiguous block of 27*pa elements, plane e's row at offset
+// e*pa in it, entry i at +i.  A wavefront that streams one grid row therefore
+// reads one contiguous 27*pa block (one or two memory pages), not 27 rows a
+// plane apart.  The coefficient pitch pa is m rounded up to 4, not the x
+// rows' px: coefficients have no halo, and a block of 27 px-rows would be
+// streamed whole (1.45x the bytes used at m = 33, 1.05x at 257; with pa
+// 1.09x and 1.01x).  A vector load at i0 <= m-1 may run past m-1 into the
+// next row of the block; those entries only feed sums that are discarded.
+// Row blocks are numbered as the x rows, halo rows included, so the array
+// fits in 27 per-point arrays; the halo rows' blocks are unused.
 __host__ __device__ inline long long arow0(const LevelView &L, int j, int k) {  // plane 0, i = 0
-	return 27 * (idx3(L, 0, j, k) - 4) + 4;
+	return 27LL * L.pa * ((long long)(j + 1) + (long long)(L.n + 2) * (k + 1));
 }
 __host__ __device__ inline long long aidx3(const LevelView &L, int e, int i, int j, int k) {
-	return arow0(L, j, k) + (long long)e * L.px + i;
+	return arow0(L, j, k) + (long long)e * L.pa + i;
 }
 
 // Lane layout of the row kernels (smoother and residual).  A lane holds

@@ -42,9 +42,9 @@ __device__ __forceinline__ void gs_point(const LevelView &L, const T *__restrict
 	for (int e = 0; e < 27; e++) {
 		if (e == 13) continue;
 		const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
-		ax = ax + Ap[e * px] * x[p + di + dj * px + dk * pxy];
+		ax = ax + Ap[e * L.pa] * x[p + di + dj * px + dk * pxy];
 	}
-	x[p] = (r[p] - ax) * ((T)1 / Ap[13 * px]);
+	x[p] = (r[p] - ax) * ((T)1 / Ap[13 * L.pa]);
 }
 
 // ------------------------------------------------------- periodic halo
@@ -256,7 +256,7 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 	typedef typename Vec2<T>::type V2;
 	const int lane = threadIdx.x;
 	const long long row = idx3(L, 0, j, k);
-	const long long px = L.px, pxy = L.pxy;
+	const long long px = L.px, pxy = L.pxy, pa = L.pa;
 	const T *Ar = A + arow0(L, j, k);
 	const T *rr_ = r + row;
 	T *xr = x + row;
@@ -281,7 +281,7 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 		T zco = (T)0, zx3 = (T)0;
 		if (wext) {
 			const uint32_t oz0 = (uint32_t)(s0 + 128) * (uint32_t)sizeof(T);
-			if (lane < 27) zco = ld_row<T>(Ar + lane * px, oz0);
+			if (lane < 27) zco = ld_row<T>(Ar + lane * pa, oz0);
 			if (lane < 9) zx3 = ld_row<T>(xr + (lane % 3 - 1) * px + (lane / 3 - 1) * pxy + 1, oz0);
 		}
 		// Streamed row by row (dj,dk): the even point's sum is formed as the
@@ -301,9 +301,9 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 			for (int rq = 0; rq < 9; rq++) {
 				const int dj = rq % 3 - 1, dk = rq / 3 - 1;
 				const T *xq = xr + dj * px + dk * pxy;
-				const V2 am = ld_stream<V2>(Ar + (3 * rq) * px, ob);
-				const V2 a0 = ld_stream<V2>(Ar + (3 * rq + 1) * px, ob);
-				const V2 ap = ld_stream<V2>(Ar + (3 * rq + 2) * px, ob);
+				const V2 am = ld_stream<V2>(Ar + (3 * rq) * pa, ob);
+				const V2 a0 = ld_stream<V2>(Ar + (3 * rq + 1) * pa, ob);
+				const V2 ap = ld_stream<V2>(Ar + (3 * rq + 2) * pa, ob);
 				const V2 xv = ld_row<V2>(xq, ob);
 				const T xm = ld_row<T>(xq - 1, ob);
 				const T xp = doO ? ld_row<T>(xq + 2, ob) : (T)0;
@@ -389,7 +389,7 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 	const int cj = j & 1, ck = k & 1;
 	const int lane = threadIdx.x;
 	const long long row = idx3(L, 0, j, k);
-	const long long px = L.px, pxy = L.pxy;
+	const long long px = L.px, pxy = L.pxy, pa = L.pa;
 	const float *Ar = A + arow0(L, j, k);
 	const float *rr_ = r + row;
 	float *xr = x + row;
@@ -425,9 +425,9 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 			for (int rq = 0; rq < 9; rq++) {
 				const int dj = rq % 3 - 1, dk = rq / 3 - 1;
 				const float *xq = xr + dj * px + dk * pxy;
-				const float4 am = ld_stream<float4>(Ar + (3 * rq) * px, ob);
-				const float4 a0 = ld_stream<float4>(Ar + (3 * rq + 1) * px, ob);
-				const float4 ap = ld_stream<float4>(Ar + (3 * rq + 2) * px, ob);
+				const float4 am = ld_stream<float4>(Ar + (3 * rq) * pa, ob);
+				const float4 a0 = ld_stream<float4>(Ar + (3 * rq + 1) * pa, ob);
+				const float4 ap = ld_stream<float4>(Ar + (3 * rq + 2) * pa, ob);
 				const bool zr = Z && zero_row(cj, ck, dj, dk);
 				float4 xv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 				float xm = 0.0f, xp = 0.0f;
@@ -438,9 +438,9 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 				}
 				float bm = 0.0f, b0 = 0.0f, bp = 0.0f, x5 = 0.0f;
 				if (ext) {
-					bm = ld_row<float>(Ar + (3 * rq) * px, ob + 16u);
-					b0 = ld_row<float>(Ar + (3 * rq + 1) * px, ob + 16u);
-					bp = ld_row<float>(Ar + (3 * rq + 2) * px, ob + 16u);
+					bm = ld_row<float>(Ar + (3 * rq) * pa, ob + 16u);
+					b0 = ld_row<float>(Ar + (3 * rq + 1) * pa, ob + 16u);
+					bp = ld_row<float>(Ar + (3 * rq + 2) * pa, ob + 16u);
 					if (!zr) x5 = ld_row<float>(xq + 5, ob);  // the halo point when i0+4 = m-1
 				}
 				if (rq == 4) {
@@ -600,7 +600,7 @@ __device__ __forceinline__ void gs_lanes(const LevelView &L, const T *__restrict
 	const int nb = (L.n - cj + 1) >> 1;
 	const int j0 = 2 * (q0 % nb) + cj, k0 = 2 * (q0 / nb) + ck;
 	const long long xb0 = idx3(L, 0, j0, k0), ab0 = arow0(L, j0, k0);
-	const long long px = L.px, pxy = L.pxy;
+	const long long px = L.px, pxy = L.pxy, pa = L.pa;
 	const int split = coef_split<T>(L);
 	const T *Ab = A + ab0;
 	const T *rb = r + xb0;
@@ -631,7 +631,7 @@ __device__ __forceinline__ void gs_lanes(const LevelView &L, const T *__restrict
 			const T *xq = xb + dj * px + dk * pxy;
 			T a[3][PPL], xv[PPL];
 #pragma unroll
-			for (int d = 0; d < 3; d++) ld_vec_coef<T, PPL>(Ab + (3 * rq + d) * px, oa, a[d], 3 * rq + d >= split);
+			for (int d = 0; d < 3; d++) ld_vec_coef<T, PPL>(Ab + (3 * rq + d) * pa, oa, a[d], 3 * rq + d >= split);
 			T xm = (T)0, xp = (T)0;
 			if (Z && zero_row(cj, ck, dj, dk)) {
 #pragma unroll
@@ -644,7 +644,7 @@ __device__ __forceinline__ void gs_lanes(const LevelView &L, const T *__restrict
 			T b[3] = {(T)0, (T)0, (T)0};
 			if (ext) {
 #pragma unroll
-				for (int d = 0; d < 3; d++) b[d] = ld_row<T>(Ab + (3 * rq + d) * px, oa + oe);
+				for (int d = 0; d < 3; d++) b[d] = ld_row<T>(Ab + (3 * rq + d) * pa, oa + oe);
 			}
 			if (rq == 4) {
 				// own-row value at m, where the row ends inside this lane
@@ -744,7 +744,7 @@ __device__ __forceinline__ double residual_lanes(const LevelView &L, const T *__
 	const bool live = rr < W.rpw && q < nrows;
 	const int j0 = q0 % L.n, k0 = q0 / L.n;
 	const long long xb0 = idx3(L, 0, j0, k0), ab0 = arow0(L, j0, k0);
-	const long long px = L.px, pxy = L.pxy;
+	const long long px = L.px, pxy = L.pxy, pa = L.pa;
 	const int split = coef_split<T>(L);
 	double v2 = 0.0;
 	if (!live) return v2;
@@ -769,7 +769,7 @@ __device__ __forceinline__ double residual_lanes(const LevelView &L, const T *__
 		const T *xq = xb + dj * px + dk * pxy;
 		T a[3][PPL], xv[PPL];
 #pragma unroll
-		for (int d = 0; d < 3; d++) ld_vec_coef<T, PPL>(Ab + (3 * rq + d) * px, oa, a[d], 3 * rq + d >= split);
+		for (int d = 0; d < 3; d++) ld_vec_coef<T, PPL>(Ab + (3 * rq + d) * pa, oa, a[d], 3 * rq + d >= split);
 		ld_vec<T, PPL>(xq, ox, xv);
 		const T xm = ld_row<T>(xq - 1, ox);
 		const T xp = needxp ? ld_row<T>(xq + PPL, ox) : (T)0;
@@ -782,9 +782,9 @@ __device__ __forceinline__ double residual_lanes(const LevelView &L, const T *__
 			s[t] = s[t] + a[2][t] * vp;
 		}
 		if (ext) {
-			sx = sx + ld_row<T>(Ab + (3 * rq) * px, oa + oe) * xv[PPL - 1];
-			sx = sx + ld_row<T>(Ab + (3 * rq + 1) * px, oa + oe) * xp;
-			sx = sx + ld_row<T>(Ab + (3 * rq + 2) * px, oa + oe) * (T)0;  // the halo point i = m
+			sx = sx + ld_row<T>(Ab + (3 * rq) * pa, oa + oe) * xv[PPL - 1];
+			sx = sx + ld_row<T>(Ab + (3 * rq + 1) * pa, oa + oe) * xp;
+			sx = sx + ld_row<T>(Ab + (3 * rq + 2) * pa, oa + oe) * (T)0;  // the halo point i = m
 		}
 	}
 	TR o[PPL], z[PPL];
@@ -1493,7 +1493,7 @@ __device__ __forceinline__ void gs_lds_global(const LevelView &L, const T *__res
 	const T *Ap = A + aidx3(L, 0, i, j, k);
 	T c[27];
 #pragma unroll
-	for (int e = 0; e < 27; e++) c[e] = Ap[e * L.px];
+	for (int e = 0; e < 27; e++) c[e] = Ap[e * L.pa];
 	gs_lds_update<T>(c, r[p], xs, (i + 1) + hx * (j + 1) + hxy * (k + 1), hx, hxy);
 }
 
@@ -1531,7 +1531,7 @@ __device__ __forceinline__ void gs_lds_pair_pass(const LevelView &L, const T *__
 		T ax = (T)0, a13 = (T)1;
 #pragma unroll
 		for (int e = 0; e < 27; e++) {
-			const V2 a = *(const V2 *)(Ap + e * L.px);
+			const V2 a = *(const V2 *)(Ap + e * L.pa);
 			c2[e] = a.y;
 			if (e == 13) { a13 = a.x; continue; }
 			const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
@@ -1580,7 +1580,7 @@ __device__ __forceinline__ void gs_lds_run(const LevelView &L, const T *__restri
 		const long long p = idx3(L, i, j, k);
 		const T *Ap = A + aidx3(L, 0, i, j, k);
 #pragma unroll
-		for (int e = 0; e < 27; e++) ca[e] = Ap[e * L.px];
+		for (int e = 0; e < 27; e++) ca[e] = Ap[e * L.pa];
 		cr = r[p];
 	}
 	__syncthreads();
@@ -1724,7 +1724,7 @@ __device__ __forceinline__ double residual_row_pair(const LevelView &L, const T 
 	typedef typename Vec2<T>::type V2;
 	typedef typename Vec2<TR>::type R2;
 	const int lane = threadIdx.x;
-	const long long px = L.px, pxy = L.pxy;
+	const long long px = L.px, pxy = L.pxy, pa = L.pa;
 	double v2 = 0.0;
 	if (q < L.n * L.o) {
 		const int j = q % L.n, k = q / L.n;
@@ -1768,9 +1768,9 @@ __device__ __forceinline__ double residual_row_pair(const LevelView &L, const T 
 				for (int rq = 0; rq < 9; rq++) {
 					const int dj = rq % 3 - 1, dk = rq / 3 - 1;
 					const T *xq = xr + dj * px + dk * pxy;
-					const V2 am = ld_stream<V2>(Ar + (3 * rq) * px, ob);
-					const V2 a0 = ld_stream<V2>(Ar + (3 * rq + 1) * px, ob);
-					const V2 ap = ld_stream<V2>(Ar + (3 * rq + 2) * px, ob);
+					const V2 am = ld_stream<V2>(Ar + (3 * rq) * pa, ob);
+					const V2 a0 = ld_stream<V2>(Ar + (3 * rq + 1) * pa, ob);
+					const V2 ap = ld_stream<V2>(Ar + (3 * rq + 2) * pa, ob);
 					const V2 xv = ld_row<V2>(xq, ob);
 					const T xm = ld_row<T>(xq - 1, ob);
 					const T xp = two ? ld_row<T>(xq + 2, ob) : (T)0;
@@ -1782,9 +1782,9 @@ __device__ __forceinline__ double residual_row_pair(const LevelView &L, const T 
 					ay = ay + ap.y * xp;
 					if (ext) {
 						const uint32_t oz = ob + 2u * (uint32_t)sizeof(T);
-						az = az + ld_row<T>(Ar + (3 * rq) * px, oz) * xv.y;
-						az = az + ld_row<T>(Ar + (3 * rq + 1) * px, oz) * xp;
-						az = az + ld_row<T>(Ar + (3 * rq + 2) * px, oz) * (T)0;  // the halo point i = m
+						az = az + ld_row<T>(Ar + (3 * rq) * pa, oz) * xv.y;
+						az = az + ld_row<T>(Ar + (3 * rq + 1) * pa, oz) * xp;
+						az = az + ld_row<T>(Ar + (3 * rq + 2) * pa, oz) * (T)0;  // the halo point i = m
 					}
 				}
 			}
@@ -1811,7 +1811,7 @@ __device__ __forceinline__ double residual_row_quad_f32(const LevelView &L, cons
 							const float *__restrict__ x, const float *__restrict__ r,
 							float *__restrict__ res, int q) {
 	const int lane = threadIdx.x;
-	const long long px = L.px, pxy = L.pxy;
+	const long long px = L.px, pxy = L.pxy, pa = L.pa;
 	double v2 = 0.0;
 	if (q < L.n * L.o) {
 		const int j = q % L.n, k = q / L.n;
@@ -1849,9 +1849,9 @@ __device__ __forceinline__ double residual_row_quad_f32(const LevelView &L, cons
 				for (int rq = 0; rq < 9; rq++) {
 					const int dj = rq % 3 - 1, dk = rq / 3 - 1;
 					const float *xq = xr + dj * px + dk * pxy;
-					const float4 am = ld_stream<float4>(Ar + (3 * rq) * px, ob);
-					const float4 ac = ld_stream<float4>(Ar + (3 * rq + 1) * px, ob);
-					const float4 ap = ld_stream<float4>(Ar + (3 * rq + 2) * px, ob);
+					const float4 am = ld_stream<float4>(Ar + (3 * rq) * pa, ob);
+					const float4 ac = ld_stream<float4>(Ar + (3 * rq + 1) * pa, ob);
+					const float4 ap = ld_stream<float4>(Ar + (3 * rq + 2) * pa, ob);
 					const float4 xv = ld_row<float4>(xq, ob);
 					const float xm = ld_row<float>(xq - 1, ob);
 					const float xp = nv == 4 ? ld_row<float>(xq + 4, ob) : 0.0f;
@@ -1868,9 +1868,9 @@ __device__ __forceinline__ double residual_row_quad_f32(const LevelView &L, cons
 					a3s = a3s + ac.w * xv.w;
 					a3s = a3s + ap.w * xp;
 					if (ext) {
-						azs = azs + ld_row<float>(Ar + (3 * rq) * px, ob + 16u) * xv.w;
-						azs = azs + ld_row<float>(Ar + (3 * rq + 1) * px, ob + 16u) * xp;
-						azs = azs + ld_row<float>(Ar + (3 * rq + 2) * px, ob + 16u) * 0.0f;  // halo
+						azs = azs + ld_row<float>(Ar + (3 * rq) * pa, ob + 16u) * xv.w;
+						azs = azs + ld_row<float>(Ar + (3 * rq + 1) * pa, ob + 16u) * xp;
+						azs = azs + ld_row<float>(Ar + (3 * rq + 2) * pa, ob + 16u) * 0.0f;  // halo
 					}
 				}
 			}
@@ -1950,7 +1950,7 @@ __global__ __launch_bounds__(256) void k_residual_pt(LevelView L, const T *__res
 #pragma unroll
 		for (int e = 0; e < 27; e++) {
 			const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
-			ax = ax + Ap[e * px] * x[p + di + dj * px + dk * pxy];
+			ax = ax + Ap[e * L.pa] * x[p + di + dj * px + dk * pxy];
 		}
 		const T v = r[p] - ax;
 		if (res) res[p] = (TR)v;
@@ -2022,7 +2022,7 @@ __global__ __launch_bounds__(256) void k_apply(LevelView L, const double *__rest
 #pragma unroll
 	for (int e = 0; e < 27; e++) {
 		const int di = e % 3 - 1, dj = (e / 3) % 3 - 1, dk = e / 9 - 1;
-		ax = ax + Ap[e * px] * in[p + di + dj * px + dk * pxy];
+		ax = ax + Ap[e * L.pa] * in[p + di + dj * px + dk * pxy];
 	}
 	out[p] = ax;
 }
@@ -2538,7 +2538,7 @@ __device__ __forceinline__ double rat_entry(const LevelView &F, const LevelView 
 							const double *Afr = Af + aidx3(F, 0, ci, cj2, ck2);
 							double a[27];
 #pragma unroll
-							for (int ef = 0; ef < 27; ef++) a[ef] = Afr[ef * F.px];
+							for (int ef = 0; ef < 27; ef++) a[ef] = Afr[ef * F.pa];
 							if (valid) {
 								double inner = 0.0;
 #pragma unroll
@@ -2624,7 +2624,7 @@ __device__ __forceinline__ void fine_point(const LevelView &F, const double *__r
 	const double *Afr = Af + aidx3(F, 0, fi + DFX, fj + DFY, fk + DFZ);
 	double a8[27];
 #pragma unroll
-	for (int e = 0; e < 27; e++) a8[e] = 0.125 * Afr[e * F.px];
+	for (int e = 0; e < 27; e++) a8[e] = 0.125 * Afr[e * F.pa];
 	(entry<DFX, DFY, DFZ, J>(a8, wr, acc, std::make_integer_sequence<int, 27>()), ...);
 }
 // Cube q (cz = q>>2, cy, cx) and its local point l (lz = l>>2, ly, lx), in
@@ -2662,7 +2662,7 @@ __global__ __launch_bounds__(256) void k_rat_interior(LevelView F, LevelView C, 
 	rat::all<std::make_integer_sequence<int, 27>>(F, Af, 2 * I, 2 * Jy, 2 * K, acc, std::make_integer_sequence<int, 8>());
 	double *Acr = Ac + aidx3(C, 0, I, Jy, K);
 #pragma unroll
-	for (int e = 0; e < 27; e++) Acr[e * C.px] = acc[e];
+	for (int e = 0; e < 27; e++) Acr[e * C.pa] = acc[e];
 }
 
 // The same interior points one entry per thread (blockIdx.y = entry J, so a
@@ -2829,7 +2829,7 @@ __global__ __launch_bounds__(256) void k_rat_prd(LevelView F, LevelView C, const
 #pragma unroll
 			for (int e = 0; e < 27; e++) {
 				const double pw = ((1.0 * pg[0][fx][e % 3]) * pg[1][fy][(e / 3) % 3]) * pg[2][fz][e / 9];
-				if (pw != 0.0) sum = sum + 0.125 * Arow[(long long)e * F.px] * pw;
+				if (pw != 0.0) sum = sum + 0.125 * Arow[(long long)e * F.pa] * pw;
 			}
 			acc = acc + sum * w;
 		}
@@ -3052,7 +3052,7 @@ __global__ __launch_bounds__(256) void k_generate(LevelView L, uint64_t seed, do
 	const long long p = idx3(L, i, j, k);
 	double *Ap = A + aidx3(L, 0, i, j, k);
 #pragma unroll
-	for (int e = 0; e < 27; e++) Ap[e * L.px] = row[e];
+	for (int e = 0; e < 27; e++) Ap[e * L.pa] = row[e];
 	r[p] = pgmg_rhs(seed, L.m, L.n, L.o, i, j, k);
 }
 
@@ -3124,7 +3124,7 @@ __global__ __launch_bounds__(256) void k_generate_q1(LevelView L, uint64_t seed,
 			gen_q1_cell<0>(hi, c + kGenCx + 1, i, j, k, L.m, L.n, L.o, row);
 			double *Ap = A + aidx3(L, 0, i, j, k);
 #pragma unroll
-			for (int e = 0; e < 27; e++) Ap[e * L.px] = row[e];
+			for (int e = 0; e < 27; e++) Ap[e * L.pa] = row[e];
 			r[idx3(L, i, j, k)] = pgmg_rhs(seed, L.m, L.n, L.o, i, j, k);
 		}
 		__syncthreads();
