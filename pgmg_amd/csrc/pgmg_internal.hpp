@@ -24,6 +24,9 @@ struct LevelView {
 	int m, n, o;       // grid points per axis
 	int px;            // row pitch (elements)
 	int pa;            // coefficient row pitch (elements): m rounded up to 4
+	int tail;          // 1: rows of 4q+1 points (q >= 1), whose coefficient
+	                   // blocks end with a copy of the last point's 27
+	                   // entries (see arow0)
 	long long pxy;     // plane pitch (elements)
 	long long base;    // index of point (0,0,0)
 	long long npad;    // elements per array, halo included
@@ -38,6 +41,7 @@ inline LevelView make_level_view(int m, int n, int o) {
 	v.m = m; v.n = n; v.o = o;
 	v.px = ((m + 7 + 15) / 16) * 16;  // i = -1 .. m+2 fit: a quad at i0 <= m-1 reads i0+3
 	v.pa = ((m + 3) / 4) * 4;
+	v.tail = (m >= 5 && (m & 3) == 1) ? 1 : 0;
 	v.pxy = (long long)v.px * (n + 2);
 	v.base = 4 + v.px + v.pxy;
 	v.npad = v.pxy * (o + 2);
@@ -104,10 +108,19 @@ __host__ __device__ inline long long idx3(const LevelView &L, int i, int j, int 
 // streamed whole (1.45x the bytes used at m = 33, 1.05x at 257; with pa
 // 1.09x and 1.01x).  A vector load at i0 <= m-1 may run past m-1 into the
 // next row of the block; those entries only feed sums that are discarded.
+// On levels with rows of 4q+1 points (tail), the block ends with 28 more
+// entries: a copy of the 27 coefficients of the row's last point i = m-1, in
+// plane order, and a zero.  The row kernels give that point to one extra
+// lane per row, which reads the copy with 16-byte loads (two 64-byte lines)
+// instead of 27 single entries a plane row apart (27 lines); the copies are
+// refreshed by every launcher that writes coefficients (launch_coef_tail).
 // Row blocks are numbered as the x rows, halo rows included, so the array
 // fits in 27 per-point arrays; the halo rows' blocks are unused.
+__host__ __device__ inline long long ablock(const LevelView &L) {
+	return 27LL * L.pa + (L.tail ? 28 : 0);
+}
 __host__ __device__ inline long long arow0(const LevelView &L, int j, int k) {  // plane 0, i = 0
-	return 27LL * L.pa * ((long long)(j + 1) + (long long)(L.n + 2) * (k + 1));
+	return ablock(L) * ((long long)(j + 1) + (long long)(L.n + 2) * (k + 1));
 }
 __host__ __device__ inline long long aidx3(const LevelView &L, int e, int i, int j, int k) {
 	return arow0(L, j, k) + (long long)e * L.pa + i;

@@ -579,6 +579,42 @@ __device__ __forceinline__ void st_vec(T *base, uint32_t off, const T (&v)[N]) {
 	st_row<V>(base, off, o);
 }
 
+// How the packed-row kernels take the coefficients of a row's point out (its
+// last point m-1, done by the row's last lane):
+//   PO_ROWS  FP64: where the level has point outs (a run-time test), the last
+//            lane loads them from the plane rows, a plane row apart;
+//   PO_NONE  FP32 levels without point outs (rows of other than 4q+1 points);
+//   PO_TAIL  FP32 levels with point outs: every lane loads its row's tail
+//            copy (see arow0), 16 bytes at a time, so the loads need no
+//            branch: the lanes of a row share the addresses, and only the
+//            last lane uses the values.  Loaded into registers inside a
+//            branch, they made the wavefront wait for its loads at every row
+//            step (nine round trips instead of one).
+// In FP64 the tail's loads, or a second inlined variant, took the kernels
+// past 256 VGPRs (one wavefront per SIMD instead of two).
+enum { PO_NONE = 0, PO_ROWS = 1, PO_TAIL = 2 };
+
+// Coefficients 3rq .. 3rq+2 of the row's point out from its tail copy (ot:
+// the copy's byte offset from Ab), read 16 bytes at a time into tb as the
+// planes are reached; plain loads, so a level that keeps its coefficients
+// cached keeps the copy cached too.  rq must be a compile-time constant (an
+// unrolled loop), so tb stays in registers.
+template <typename T>
+__device__ __forceinline__ void point_out_tail(const T *Ab, uint32_t ot, int rq, T (&tb)[28], T (&b)[3]) {
+	constexpr int VW = 16 / (int)sizeof(T);
+#pragma unroll
+	for (int d = 0; d < 3; d++) {
+		const int e = 3 * rq + d;
+		if (e % VW == 0) {
+			T v[VW];
+			ld_vec<T, VW>(Ab, ot + (uint32_t)(e * (int)sizeof(T)), v);
+#pragma unroll
+			for (int q = 0; q < VW; q++) tb[e + q] = v[q];
+		}
+		b[d] = tb[e];
+	}
+}
+
 // Forward colour-pair update of rows that fit one wavefront's lanes
 // (row_lanes: rpw rows per wavefront, lpr lanes per row, ppl points per
 // lane, the last lane of a 2^k+1 row also doing the point out).  The
@@ -589,7 +625,7 @@ __device__ __forceinline__ void st_vec(T *base, uint32_t off, const T (&v)[N]) {
 // at a row's last lane the point out (or the halo).  Only the wavefront's
 // rows are written by the launch, so each lane stores its points once, at
 // the end.  Per-point arithmetic and order as in gs_point.
-template <typename T, int PPL, bool Z = false>
+template <typename T, int PPL, bool Z, int PO>
 __device__ __forceinline__ void gs_lanes(const LevelView &L, const T *__restrict__ A, T *__restrict__ x,
 					 const T *__restrict__ r, int cj, int ck, int q0, int nrows) {
 	const RowLanes W = row_lanes(L.m, (int)sizeof(T));
@@ -613,10 +649,12 @@ __device__ __forceinline__ void gs_lanes(const LevelView &L, const T *__restrict
 		ox = (uint32_t)((idx3(L, 0, j, k) - xb0 + i0) * (long long)sizeof(T));
 		oa = (uint32_t)((arow0(L, j, k) - ab0 + i0) * (long long)sizeof(T));
 	}
-	const bool ext = live && W.ext && li == W.lpr - 1;  // even point i0+PPL = m-1
+	const bool ext = (PO == PO_TAIL || (PO == PO_ROWS && W.ext)) && live && li == W.lpr - 1;  // even point i0+PPL = m-1
 	const bool lastlane = li == W.lpr - 1;
 	const bool needxp = live && i0 + PPL <= L.m;
 	const uint32_t oe = (uint32_t)(PPL * sizeof(T));
+	const uint32_t ot = oa + (uint32_t)((27 * pa - i0) * (long long)sizeof(T));  // the tail copy
+	T tb[28];
 	T s[PPL], cc[PPL], wm[PPL], wp[PPL], rv[PPL];
 	T prod[PPL / 2][12];
 	T sx = (T)0, cx = (T)1, rx = (T)0, xhalo = (T)0;
@@ -642,9 +680,13 @@ __device__ __forceinline__ void gs_lanes(const LevelView &L, const T *__restrict
 				xp = needxp ? ld_row<T>(xq + PPL, ox) : (T)0;
 			}
 			T b[3] = {(T)0, (T)0, (T)0};
-			if (ext) {
+			if constexpr (PO == PO_TAIL) {
+				point_out_tail<T>(Ab, ot, rq, tb, b);
+			} else if constexpr (PO == PO_ROWS) {
+				if (ext) {
 #pragma unroll
-				for (int d = 0; d < 3; d++) b[d] = ld_row<T>(Ab + (3 * rq + d) * pa, oa + oe);
+					for (int d = 0; d < 3; d++) b[d] = ld_row<T>(Ab + (3 * rq + d) * pa, oa + oe);
+				}
 			}
 			if (rq == 4) {
 				// own-row value at m, where the row ends inside this lane
@@ -733,7 +775,7 @@ __device__ __forceinline__ void gs_lanes(const LevelView &L, const T *__restrict
 // Residual of rows that fit one wavefront's lanes (layout as gs_lanes);
 // returns this lane's sum of squares.  zx (may be null) is set to 0 at the
 // same points.
-template <typename T, typename TR, int PPL>
+template <typename T, typename TR, int PPL, int PO>
 __device__ __forceinline__ double residual_lanes(const LevelView &L, const T *__restrict__ A, const T *__restrict__ x,
 						 const T *__restrict__ r, TR *__restrict__ res, TR *__restrict__ zx,
 						 int q0, int nrows) {
@@ -753,9 +795,11 @@ __device__ __forceinline__ double residual_lanes(const LevelView &L, const T *__
 	const long long dx = idx3(L, 0, j, k) - xb0 + i0;
 	const uint32_t ox = (uint32_t)(dx * (long long)sizeof(T));
 	const uint32_t oa = (uint32_t)((arow0(L, j, k) - ab0 + i0) * (long long)sizeof(T));
-	const bool ext = W.ext && li == W.lpr - 1;
+	const bool ext = (PO == PO_TAIL || (PO == PO_ROWS && W.ext)) && li == W.lpr - 1;
 	const bool needxp = i0 + PPL <= L.m;
 	const uint32_t oe = (uint32_t)(PPL * sizeof(T));
+	const uint32_t ot = oa + (uint32_t)((27 * pa - i0) * (long long)sizeof(T));  // the tail copy
+	T tb[28];
 	const T *Ab = A + ab0, *rb = r + xb0, *xb = x + xb0;
 	T s[PPL], rv[PPL];
 	T sx = (T)0, rx = (T)0;
@@ -781,10 +825,19 @@ __device__ __forceinline__ double residual_lanes(const LevelView &L, const T *__
 			s[t] = s[t] + a[1][t] * xv[t];
 			s[t] = s[t] + a[2][t] * vp;
 		}
-		if (ext) {
-			sx = sx + ld_row<T>(Ab + (3 * rq) * pa, oa + oe) * xv[PPL - 1];
-			sx = sx + ld_row<T>(Ab + (3 * rq + 1) * pa, oa + oe) * xp;
-			sx = sx + ld_row<T>(Ab + (3 * rq + 2) * pa, oa + oe) * (T)0;  // the halo point i = m
+		if constexpr (PO == PO_TAIL) {
+			T b[3];
+			point_out_tail<T>(Ab, ot, rq, tb, b);
+			// every lane sums; only the last lane's sx is used
+			sx = sx + b[0] * xv[PPL - 1];
+			sx = sx + b[1] * xp;
+			sx = sx + b[2] * (T)0;  // the halo point i = m
+		} else if constexpr (PO == PO_ROWS) {
+			if (ext) {
+				sx = sx + ld_row<T>(Ab + (3 * rq) * pa, oa + oe) * xv[PPL - 1];
+				sx = sx + ld_row<T>(Ab + (3 * rq + 1) * pa, oa + oe) * xp;
+				sx = sx + ld_row<T>(Ab + (3 * rq + 2) * pa, oa + oe) * (T)0;  // the halo point i = m
+			}
 		}
 	}
 	TR o[PPL], z[PPL];
@@ -844,7 +897,12 @@ __device__ __forceinline__ void gs_wave(const LevelView &L, const T *__restrict_
 	const RowLanes W = row_lanes(L.m, (int)sizeof(T));
 	if (W.lpr <= 64) {
 		const int q0 = w * W.rpw;
-		if (q0 < rows) gs_lanes<T, Ppl<T>::v, Z>(L, A, x, r, cj, ck, q0, rows);
+		if (q0 < rows) {
+			// FP32 point outs come with a tail copy (m = 4q+1)
+			if constexpr (sizeof(T) == 8) gs_lanes<T, Ppl<T>::v, Z, PO_ROWS>(L, A, x, r, cj, ck, q0, rows);
+			else if (W.ext) gs_lanes<T, Ppl<T>::v, Z, PO_TAIL>(L, A, x, r, cj, ck, q0, rows);
+			else gs_lanes<T, Ppl<T>::v, Z, PO_NONE>(L, A, x, r, cj, ck, q0, rows);
+		}
 	} else if (w < rows) {
 		gs_row<T, Z>(L, A, x, r, 2 * (w % nb) + cj, 2 * (w / nb) + ck);
 	}
@@ -1915,7 +1973,11 @@ __device__ __forceinline__ double residual_wave(const LevelView &L, const T *__r
 	const RowLanes W = row_lanes(L.m, (int)sizeof(T));
 	if (W.lpr <= 64) {
 		const int q0 = w * W.rpw;
-		return q0 < rows ? residual_lanes<T, TR, Ppl<T>::v>(L, A, x, r, res, zx, q0, rows) : 0.0;
+		if (q0 >= rows) return 0.0;
+		if constexpr (sizeof(T) == 8) return residual_lanes<T, TR, Ppl<T>::v, PO_ROWS>(L, A, x, r, res, zx, q0, rows);
+		// FP32 point outs come with a tail copy (m = 4q+1)
+		if (W.ext) return residual_lanes<T, TR, Ppl<T>::v, PO_TAIL>(L, A, x, r, res, zx, q0, rows);
+		return residual_lanes<T, TR, Ppl<T>::v, PO_NONE>(L, A, x, r, res, zx, q0, rows);
 	}
 	return residual_row<T, TR>(L, A, x, r, res, w, zx);
 }
@@ -2837,11 +2899,30 @@ __global__ __launch_bounds__(256) void k_rat_prd(LevelView F, LevelView C, const
 	Ac[aidx3(C, d, I[0], I[1], I[2])] = acc;
 }
 
+// Copies the 27 coefficients of the last point of every grid row into the
+// row block's tail (levels with tail; see arow0).  Run by every launcher that
+// writes coefficients, after its writes.
+__global__ void k_coef_tail(LevelView L, double *__restrict__ A) {
+	const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= (long long)L.n * L.o * 27) return;
+	const int e = (int)(t % 27);
+	const long long q = t / 27;
+	const int j = (int)(q % L.n), k = (int)(q / L.n);
+	A[arow0(L, j, k) + 27LL * L.pa + e] = A[aidx3(L, e, L.m - 1, j, k)];
+}
+static void launch_coef_tail(hipStream_t s, const LevelView &L, double *A) {
+	if (!L.tail) return;
+	const long long tot = (long long)L.n * L.o * 27;
+	hipLaunchKernelGGL(k_coef_tail, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, L, A);
+	HIPCHECK_LAUNCH();
+}
+
 void launch_rat(hipStream_t s, const LevelView &F, const LevelView &C, const double *Af, double *Ac) {
 	if (F.prd) {
 		const long long tot = (long long)C.m * C.n * C.o * 27;
 		hipLaunchKernelGGL(k_rat_prd, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, F, C, Af, Ac);
 		HIPCHECK_LAUNCH();
+		launch_coef_tail(s, C, Ac);
 		return;
 	}
 	const int lastx = (F.m - 1) >> 1, lasty = (F.n - 1) >> 1, lastz = (F.o - 1) >> 1;
@@ -2859,6 +2940,7 @@ void launch_rat(hipStream_t s, const LevelView &F, const LevelView &C, const dou
 	const long long nbnd = (E.na + E.nb + E.nc) * 27;
 	hipLaunchKernelGGL(k_rat_boundary, dim3((unsigned)((nbnd + 255) / 256)), dim3(256), 0, s, F, C, Af, Ac, E);
 	HIPCHECK_LAUNCH();
+	launch_coef_tail(s, C, Ac);
 }
 
 // ---------------------------------------------------------- bottom LU
@@ -3152,6 +3234,7 @@ void launch_generate(hipStream_t s, const LevelView &L, int family, uint64_t see
 			hipLaunchKernelGGL(k_generate<PGMG_FAM_FACE7_UNIFORM>, grid, block, 0, s, L, seed, A, r);
 	}
 	HIPCHECK_LAUNCH();
+	launch_coef_tail(s, L, A);
 }
 
 __device__ __forceinline__ long long compact_to_padded(const LevelView &L, long long p) {
@@ -3200,6 +3283,7 @@ void launch_unpack_vec(hipStream_t s, const LevelView &L, const double *d, doubl
 void launch_pack_op(hipStream_t s, const LevelView &L, const double *c, double *A, long long p0, long long n) {
 	if (n > 0) hipLaunchKernelGGL(k_pack_op, dim3(nblk(n * 27, 256)), dim3(256), 0, s, L, c, A, p0, n);
 	HIPCHECK_LAUNCH();
+	launch_coef_tail(s, L, A);
 }
 void launch_unpack_op(hipStream_t s, const LevelView &L, const double *A, double *c, long long p0, long long n) {
 	if (n > 0) hipLaunchKernelGGL(k_unpack_op, dim3(nblk(n * 27, 256)), dim3(256), 0, s, L, A, c, p0, n);
