@@ -250,7 +250,9 @@ __device__ __forceinline__ T read_lane(T v, int l) {
 		return w;
 	}
 }
-template <typename T>
+// ALL: every lane forms the point out's sum, with no branch (see below); the
+// batched kernel keeps it in lane 63's branch, which holds it to 256 VGPRs.
+template <typename T, bool ALL = true>
 __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restrict__ A, T *__restrict__ x,
 					    const T *__restrict__ r, int j, int k) {
 	typedef typename Vec2<T>::type V2;
@@ -326,8 +328,12 @@ __device__ __forceinline__ void gs_pair_row(const LevelView &L, const T *__restr
 						prod[3 * (rq - 5) + 2] = ap.y * xp;
 					}
 				}
-				if (ext) {
-					// point out i0+2: neighbours i0+1 (xv.y), i0+2 (xp), i0+3
+				if (ALL || ext) {
+					// point out i0+2 (lane 63's): neighbours i0+1 (xv.y), i0+2
+					// (xp), i0+3.  With ALL every lane forms the sum, with no
+					// branch, and only lane 63's is used: inside a branch these
+					// products make the wavefront wait for each row step's loads
+					// (nine round trips a segment instead of one).
 					const T zm = read_lane(zco, 3 * rq);
 					const T z0 = read_lane(zco, 3 * rq + 1);
 					const T zp = read_lane(zco, 3 * rq + 2);
@@ -799,6 +805,7 @@ __device__ __forceinline__ double residual_lanes(const LevelView &L, const T *__
 	const bool needxp = i0 + PPL <= L.m;
 	const uint32_t oe = (uint32_t)(PPL * sizeof(T));
 	const uint32_t ot = oa + (uint32_t)((27 * pa - i0) * (long long)sizeof(T));  // the tail copy
+	const uint32_t oz = oa + (uint32_t)((L.m - 1 - i0) * (long long)sizeof(T));  // the row's point m-1
 	T tb[28];
 	const T *Ab = A + ab0, *rb = r + xb0, *xb = x + xb0;
 	T s[PPL], rv[PPL];
@@ -833,10 +840,13 @@ __device__ __forceinline__ double residual_lanes(const LevelView &L, const T *__
 			sx = sx + b[1] * xp;
 			sx = sx + b[2] * (T)0;  // the halo point i = m
 		} else if constexpr (PO == PO_ROWS) {
-			if (ext) {
-				sx = sx + ld_row<T>(Ab + (3 * rq) * pa, oa + oe) * xv[PPL - 1];
-				sx = sx + ld_row<T>(Ab + (3 * rq + 1) * pa, oa + oe) * xp;
-				sx = sx + ld_row<T>(Ab + (3 * rq + 2) * pa, oa + oe) * (T)0;  // the halo point i = m
+			// every lane loads its row's point-out entries (the lanes of a row
+			// share the addresses) and sums, with no branch; only the last
+			// lane's sx is used
+			if (W.ext) {
+				sx = sx + ld_row<T>(Ab + (3 * rq) * pa, oz) * xv[PPL - 1];
+				sx = sx + ld_row<T>(Ab + (3 * rq + 1) * pa, oz) * xp;
+				sx = sx + ld_row<T>(Ab + (3 * rq + 2) * pa, oz) * (T)0;  // the halo point i = m
 			}
 		}
 	}
@@ -873,15 +883,16 @@ __device__ __forceinline__ double residual_lanes(const LevelView &L, const T *__
 
 // Forward colour-pair update of one row: four points per lane in FP32,
 // two in FP64 (whose 16-byte pair loads already fill the registers).
-// The first-sweep-from-zero form (Z) exists in FP32 only.
-template <typename T, bool Z>
+// The first-sweep-from-zero form (Z) exists in FP32 only.  ALL: as in
+// gs_pair_row.
+template <typename T, bool Z, bool ALL = true>
 __device__ __forceinline__ void gs_row(const LevelView &L, const T *__restrict__ A, T *__restrict__ x,
 				       const T *__restrict__ r, int j, int k) {
 	if constexpr (sizeof(T) == 4) {
 		gs_quad_row_f32<Z>(L, A, x, r, j, k);
 	} else {
 		static_assert(!Z, "first sweep from zero: FP32 only");
-		gs_pair_row<T>(L, A, x, r, j, k);
+		gs_pair_row<T, ALL>(L, A, x, r, j, k);
 	}
 }
 
@@ -890,7 +901,7 @@ template <typename T> struct Ppl { static const int v = sizeof(T) == 4 ? 4 : 2; 
 // Wavefront w of a colour-pair launch: rows w*rpw .. of the pair when they
 // fit the lanes, else the single long row w, streamed in segments.  w must be
 // wavefront-uniform.
-template <typename T, bool Z = false>
+template <typename T, bool Z = false, bool ALL = true>
 __device__ __forceinline__ void gs_wave(const LevelView &L, const T *__restrict__ A, T *__restrict__ x,
 					const T *__restrict__ r, int cj, int ck, int w) {
 	const int nb = (L.n - cj + 1) >> 1, nd = (L.o - ck + 1) >> 1, rows = nb * nd;
@@ -904,7 +915,7 @@ __device__ __forceinline__ void gs_wave(const LevelView &L, const T *__restrict_
 			else gs_lanes<T, Ppl<T>::v, Z, PO_NONE>(L, A, x, r, cj, ck, q0, rows);
 		}
 	} else if (w < rows) {
-		gs_row<T, Z>(L, A, x, r, 2 * (w % nb) + cj, 2 * (w / nb) + ck);
+		gs_row<T, Z, ALL>(L, A, x, r, 2 * (w % nb) + cj, 2 * (w / nb) + ck);
 	}
 }
 
@@ -1813,11 +1824,18 @@ __device__ __forceinline__ double residual_row_pair(const LevelView &L, const T 
 		for (int s0 = 0;; s0 += 128) {
 			const bool last = L.m - s0 <= 129;
 			const int i0 = s0 + 2 * lane;
-			const bool ext = last && lane == 63 && L.m - s0 == 129;  // point i0+2 = m-1
+			const bool zseg = last && L.m - s0 == 129;  // the segment has a point out, m-1
+			const bool ext = zseg && lane == 63;         // lane 63 does it
 			const bool live = i0 < L.m;
 			const bool two = i0 + 1 < L.m;
 			T ax = (T)0, ay = (T)0, az = (T)0, r0 = (T)0, r1 = (T)0, rz = (T)0;
 			const uint32_t ob = (uint32_t)i0 * (uint32_t)sizeof(T);
+			// lanes 0..26: coefficient `lane` of the point out, read by every
+			// lane with v_readlane (its sum is formed with no branch; inside
+			// lane 63's branch the products made the wavefront wait for each
+			// row step's loads)
+			T zco = (T)0;
+			if (zseg && lane < 27) zco = ld_row<T>(Ar + lane * pa, (uint32_t)(L.m - 1) * (uint32_t)sizeof(T));
 			if (live) {
 				const V2 rr = ld_stream<V2>(rr_, ob);
 				r0 = rr.x; r1 = rr.y;
@@ -1838,12 +1856,10 @@ __device__ __forceinline__ double residual_row_pair(const LevelView &L, const T 
 					ay = ay + am.y * xv.x;
 					ay = ay + a0.y * xv.y;
 					ay = ay + ap.y * xp;
-					if (ext) {
-						const uint32_t oz = ob + 2u * (uint32_t)sizeof(T);
-						az = az + ld_row<T>(Ar + (3 * rq) * pa, oz) * xv.y;
-						az = az + ld_row<T>(Ar + (3 * rq + 1) * pa, oz) * xp;
-						az = az + ld_row<T>(Ar + (3 * rq + 2) * pa, oz) * (T)0;  // the halo point i = m
-					}
+					// the point out (lane 63's az is the one used)
+					az = az + read_lane(zco, 3 * rq) * xv.y;
+					az = az + read_lane(zco, 3 * rq + 1) * xp;
+					az = az + read_lane(zco, 3 * rq + 2) * (T)0;  // the halo point i = m
 				}
 			}
 			flush();
@@ -3421,7 +3437,7 @@ __global__ __launch_bounds__(256) void kb_gs_pair(const SlotDev *__restrict__ sl
 								   int level, int cj, int ck) {
 	BATCH_LOOP
 		const LevelView L = S->lv[level];
-		gs_wave<T, Z>(L, sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), cj, ck,
+		gs_wave<T, Z, false>(L, sA<T>(S, level), sx<T>(S, level), sr<T>(S, level), cj, ck,
 			      __builtin_amdgcn_readfirstlane(lu * 4 + (int)threadIdx.y));
 	}
 }
