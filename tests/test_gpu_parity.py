@@ -244,14 +244,22 @@ def test_solve_257_exact_mode_matches_reference():
                          [((129, 129, 129), P.PREC_FP64, P.FAM_FACE7_UNIFORM),
                           ((129, 129, 129), P.PREC_MIXED, P.FAM_FACE7_LOGUNI),
                           ((140, 100, 60), P.PREC_FP64, P.FAM_Q1_ANISO),
-                          ((142, 98, 64), P.PREC_MIXED, P.FAM_FACE7_UNIFORM)],
-                         ids=["129-fp64", "129-mixed", "140x100x60-fp64-q1", "142x98x64-mixed"])
+                          ((142, 98, 64), P.PREC_MIXED, P.FAM_FACE7_UNIFORM),
+                          ((517, 14, 12), P.PREC_MIXED, P.FAM_FACE7_LOGUNI),
+                          ((513, 10, 9), P.PREC_MIXED, P.FAM_Q1_ANISO),
+                          ((300, 11, 10), P.PREC_MIXED, P.FAM_FACE7_UNIFORM),
+                          ((517, 14, 12), P.PREC_FP64, P.FAM_FACE7_LOGUNI)],
+                         ids=["129-fp64", "129-mixed", "140x100x60-fp64-q1", "142x98x64-mixed",
+                              "517x14x12-mixed", "513x10x9-mixed-q1", "300x11x10-mixed", "517x14x12-fp64"])
 def test_colour8_129_bitwise_vs_oracle(dims, precision, family):
     """At 129^3 (every level on the colour-pair, residual and transfer
     kernels down to 17^3), and on even non-cubic grids whose first coarse
     level is restricted in two launches (interior and shell, partial cubes
     and the injection flags on the shell): two cycles of the 8-colour solver,
-    FP64 or mixed, bit-identical to the oracle's."""
+    FP64 or mixed, bit-identical to the oracle's.  Rows longer than one
+    wavefront's 256 FP32 points (or 128 FP64) take the segmented row kernels:
+    517 and 513 points have a point out at a segment boundary and at the row
+    end, 300 at a segment boundary only."""
     m, n, o = dims
     say("case colour8", dims, precision)
     A = O.gen_operator(family, 31, m, n, o)
