@@ -414,6 +414,19 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 		const bool doO3 = live && i0 + 3 < L.m;
 		const bool ext = lane == 63 && i0 + 4 < L.m;  // even point out i0+4 = s0+256
 		const uint32_t ob = (uint32_t)i0 * 4u;
+		// lanes 0..26: coefficient `lane` of the point out; lanes 0..8: x of
+		// the point after it (s0+257, the halo where the row ends) in row
+		// (dj, dk) = (lane % 3 - 1, lane / 3 - 1), 0 in rows known to be 0.
+		// Every lane reads them with v_readlane and forms the point out's sum,
+		// with no branch (lane 63's is the one used): loaded by lane 63 inside
+		// its branch, they made the wavefront wait for each row step's loads.
+		float zco = 0.0f, zx5 = 0.0f;
+		if (s0 + 256 < L.m) {
+			const uint32_t oz0 = (uint32_t)(s0 + 256) * 4u;
+			if (lane < 27) zco = ld_row<float>(Ar + lane * pa, oz0);
+			if (lane < 9 && !(Z && zero_row(cj, ck, lane % 3 - 1, lane / 3 - 1)))
+				zx5 = ld_row<float>(xr + (lane % 3 - 1) * px + (lane / 3 - 1) * pxy + 1, oz0);
+		}
 		// even sums e0 (i0), e2 (i0+2), ex (i0+4, ext); odd sums o1 (i0+1),
 		// o3 (i0+3) up to the own row; own-row coefficients kept; the odd
 		// points' 12 terms after the own row kept as products (the in-order
@@ -442,13 +455,10 @@ __device__ __forceinline__ void gs_quad_row_f32(const LevelView &L, const float 
 					xm = ld_row<float>(xq - 1, ob);
 					xp = doO3 ? ld_row<float>(xq + 4, ob) : 0.0f;
 				}
-				float bm = 0.0f, b0 = 0.0f, bp = 0.0f, x5 = 0.0f;
-				if (ext) {
-					bm = ld_row<float>(Ar + (3 * rq) * pa, ob + 16u);
-					b0 = ld_row<float>(Ar + (3 * rq + 1) * pa, ob + 16u);
-					bp = ld_row<float>(Ar + (3 * rq + 2) * pa, ob + 16u);
-					if (!zr) x5 = ld_row<float>(xq + 5, ob);  // the halo point when i0+4 = m-1
-				}
+				const float bm = read_lane(zco, 3 * rq);
+				const float b0 = read_lane(zco, 3 * rq + 1);
+				const float bp = read_lane(zco, 3 * rq + 2);
+				const float x5 = read_lane(zx5, rq);  // the halo point when i0+4 = m-1
 				if (rq == 4) {
 					e0 = e0 + am.x * xm;
 					e0 = e0 + ap.x * xv.y;
@@ -1910,9 +1920,15 @@ __device__ __forceinline__ double residual_row_quad_f32(const LevelView &L, cons
 		for (int s0 = 0;; s0 += 256) {
 			const bool last = L.m - s0 <= 257;
 			const int i0 = s0 + 4 * lane;
-			const bool ext = last && lane == 63 && L.m - s0 == 257;  // point i0+4 = m-1
+			const bool zseg = last && L.m - s0 == 257;  // the segment has a point out, m-1
+			const bool ext = zseg && lane == 63;         // lane 63 does it
 			const int nv = i0 < L.m ? min(4, L.m - i0) : 0;
 			const uint32_t ob = (uint32_t)i0 * 4u;
+			// lanes 0..26: coefficient `lane` of the point out, read by every
+			// lane with v_readlane (its sum is formed with no branch, as in
+			// residual_row_pair)
+			float zco = 0.0f;
+			if (zseg && lane < 27) zco = ld_row<float>(Ar + lane * pa, (uint32_t)(L.m - 1) * 4u);
 			float a0s = 0.0f, a1s = 0.0f, a2s = 0.0f, a3s = 0.0f, azs = 0.0f;
 			float4 rr = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 			float rz = 0.0f;
@@ -1941,11 +1957,10 @@ __device__ __forceinline__ double residual_row_quad_f32(const LevelView &L, cons
 					a3s = a3s + am.w * xv.z;
 					a3s = a3s + ac.w * xv.w;
 					a3s = a3s + ap.w * xp;
-					if (ext) {
-						azs = azs + ld_row<float>(Ar + (3 * rq) * pa, ob + 16u) * xv.w;
-						azs = azs + ld_row<float>(Ar + (3 * rq + 1) * pa, ob + 16u) * xp;
-						azs = azs + ld_row<float>(Ar + (3 * rq + 2) * pa, ob + 16u) * 0.0f;  // halo
-					}
+					// the point out (lane 63's azs is the one used)
+					azs = azs + read_lane(zco, 3 * rq) * xv.w;
+					azs = azs + read_lane(zco, 3 * rq + 1) * xp;
+					azs = azs + read_lane(zco, 3 * rq + 2) * 0.0f;  // halo
 				}
 			}
 			flush();
