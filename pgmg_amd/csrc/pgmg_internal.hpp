@@ -151,11 +151,19 @@ __host__ __device__ inline int row_waves(int rows, int m, int elem) {
 	return (rows + w.rpw - 1) / w.rpw;
 }
 
+// Segments of a long row (more than 64 lanes' worth) in the residual: 128
+// points each in FP64, 256 in FP32, the last one up to one more (its last
+// lane's point out).  Each segment is a wavefront of its own: no update runs
+// along the row, so the segments need no order, and one wavefront per row
+// made two or more dependent round trips of loads.
+__host__ __device__ inline int residual_segments(int m, int elem) {
+	return 1 + (m - 2) / (elem == 4 ? 256 : 128);
+}
 // Workgroups (4 wavefronts each) of a residual launch over all rows of L in
 // arithmetic of elem bytes; an FP64 residual has one partial sum per block.
 __host__ __device__ inline int residual_blocks(const LevelView &L, int elem) {
 	const int rows = L.n * L.o;
-	const int waves = row_lanes(L.m, elem).lpr <= 64 ? row_waves(rows, L.m, elem) : rows;
+	const int waves = row_lanes(L.m, elem).lpr <= 64 ? row_waves(rows, L.m, elem) : rows * residual_segments(L.m, elem);
 	return (waves + 3) / 4;
 }
 

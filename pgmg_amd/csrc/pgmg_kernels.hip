@@ -1790,15 +1790,16 @@ __device__ __forceinline__ double block_sum_256(double v) {
 	return s;
 }
 
-// Residual r - A x and per-block sums of its square.  One wavefront per grid
-// row, each lane a pair of points (FP64) or four (FP32) read with vector
-// loads (as in k_gs_pair, with the same uniform row bases); blocks of 4
-// rows, renumbered per XCD.
+// Residual r - A x and per-block sums of its square, on rows longer than
+// one wavefront's lanes: one wavefront per segment seg of grid row q (see
+// residual_segments), each lane a pair of points (FP64) or four (FP32) read
+// with vector loads (as in k_gs_pair, with the same uniform row bases);
+// blocks of 4 wavefronts, renumbered per XCD.
 // res (may be null) receives the residual in type TR, partial (may be null)
-// one sum per block.  q (the row) must be wavefront-uniform.
+// one sum per block.  q and seg must be wavefront-uniform.
 template <typename T, typename TR>
 __device__ __forceinline__ double residual_row_pair(const LevelView &L, const T *__restrict__ A, const T *__restrict__ x,
-						    const T *__restrict__ r, TR *__restrict__ res, int q,
+						    const T *__restrict__ r, TR *__restrict__ res, int q, int seg,
 						    TR *__restrict__ zx = nullptr) {
 	typedef typename Vec2<T>::type V2;
 	typedef typename Vec2<TR>::type R2;
@@ -1812,9 +1813,8 @@ __device__ __forceinline__ double residual_row_pair(const LevelView &L, const T 
 		TR *resr = res ? res + row : nullptr;
 		TR *zxr = zx ? zx + row : nullptr;
 		// segments of 128 points; the last one takes up to 129, lane 63
-		// doing the odd point out from scalar loads.  A segment's stores are
-		// issued after the next segment's loads, which then never wait for
-		// them (one counter tracks loads and stores in issue order).
+		// doing the odd point out.  The wavefront does segment seg only; its
+		// stores go out after its sum, through the same flush as before.
 		bool pend = false, ptwo = false, pext = false;
 		uint32_t pobr = 0;
 		T p0 = (T)0, p1 = (T)0, p2 = (T)0;
@@ -1831,7 +1831,7 @@ __device__ __forceinline__ double residual_row_pair(const LevelView &L, const T 
 				if (pext) st_row<TR>(zxr, pobr + 2u * (uint32_t)sizeof(TR), (TR)0);
 			}
 		};
-		for (int s0 = 0;; s0 += 128) {
+		for (int s0 = 128 * seg;; s0 += 128) {
 			const bool last = L.m - s0 <= 129;
 			const int i0 = s0 + 2 * lane;
 			const bool zseg = last && L.m - s0 == 129;  // the segment has a point out, m-1
@@ -1882,7 +1882,7 @@ __device__ __forceinline__ double residual_row_pair(const LevelView &L, const T 
 			}
 			pend = live; ptwo = two; pext = ext;
 			pobr = (uint32_t)i0 * (uint32_t)sizeof(TR);
-			if (last) break;
+			break;  // one segment per wavefront
 		}
 		flush();
 	}
@@ -1890,10 +1890,11 @@ __device__ __forceinline__ double residual_row_pair(const LevelView &L, const T 
 }
 
 // FP32 residual with four points per lane (16-byte loads): segments of 256
-// points, the last one up to 257 with lane 63 doing the point out.
+// points, the last one up to 257 with lane 63 doing the point out; the
+// wavefront does segment seg of row q.
 __device__ __forceinline__ double residual_row_quad_f32(const LevelView &L, const float *__restrict__ A,
 							const float *__restrict__ x, const float *__restrict__ r,
-							float *__restrict__ res, int q) {
+							float *__restrict__ res, int q, int seg) {
 	const int lane = threadIdx.x;
 	const long long px = L.px, pxy = L.pxy, pa = L.pa;
 	double v2 = 0.0;
@@ -1902,7 +1903,7 @@ __device__ __forceinline__ double residual_row_quad_f32(const LevelView &L, cons
 		const long long row = idx3(L, 0, j, k);
 		const float *Ar = A + arow0(L, j, k), *xr = x + row, *rr_ = r + row;
 		float *resr = res ? res + row : nullptr;
-		// a segment's stores are issued after the next segment's loads
+		// the segment's stores go out after its sum
 		int pnv = 0;
 		bool pext = false;
 		uint32_t pob = 0;
@@ -1917,7 +1918,7 @@ __device__ __forceinline__ double residual_row_quad_f32(const LevelView &L, cons
 			}
 			if (pext) st_row<float>(resr, pob + 16u, qz);
 		};
-		for (int s0 = 0;; s0 += 256) {
+		for (int s0 = 256 * seg;; s0 += 256) {
 			const bool last = L.m - s0 <= 257;
 			const int i0 = s0 + 4 * lane;
 			const bool zseg = last && L.m - s0 == 257;  // the segment has a point out, m-1
@@ -1974,7 +1975,7 @@ __device__ __forceinline__ double residual_row_quad_f32(const LevelView &L, cons
 				q0 = v0; q1 = v1; q2 = v2f; q3 = v3; qz = vz;
 			}
 			pnv = nv; pext = ext; pob = ob;
-			if (last) break;
+			break;  // one segment per wavefront
 		}
 		flush();
 	}
@@ -1985,18 +1986,20 @@ __device__ __forceinline__ double residual_row_quad_f32(const LevelView &L, cons
 // FP32 (residual into FP32), two in FP64.
 template <typename T, typename TR>
 __device__ __forceinline__ double residual_row(const LevelView &L, const T *__restrict__ A, const T *__restrict__ x,
-					       const T *__restrict__ r, TR *__restrict__ res, int q, TR *__restrict__ zx = nullptr) {
-	return residual_row_pair<T, TR>(L, A, x, r, res, q, zx);
+					       const T *__restrict__ r, TR *__restrict__ res, int q, int seg,
+					       TR *__restrict__ zx = nullptr) {
+	return residual_row_pair<T, TR>(L, A, x, r, res, q, seg, zx);
 }
 template <>
 __device__ __forceinline__ double residual_row<float, float>(const LevelView &L, const float *__restrict__ A,
 							     const float *__restrict__ x, const float *__restrict__ r,
-							     float *__restrict__ res, int q, float *__restrict__ zx) {
-	return residual_row_quad_f32(L, A, x, r, res, q);  // zx: not used in FP32
+							     float *__restrict__ res, int q, int seg, float *__restrict__ zx) {
+	return residual_row_quad_f32(L, A, x, r, res, q, seg);  // zx: not used in FP32
 }
 
 // Wavefront w of a residual launch: rows w*rpw .. when they fit the lanes,
-// else the single long row w.  w must be wavefront-uniform.
+// else one segment of a long row (residual_segments per row).  w must be
+// wavefront-uniform.
 template <typename T, typename TR>
 __device__ __forceinline__ double residual_wave(const LevelView &L, const T *__restrict__ A, const T *__restrict__ x,
 						const T *__restrict__ r, TR *__restrict__ res, TR *__restrict__ zx, int w) {
@@ -2010,7 +2013,8 @@ __device__ __forceinline__ double residual_wave(const LevelView &L, const T *__r
 		if (W.ext) return residual_lanes<T, TR, Ppl<T>::v, PO_TAIL>(L, A, x, r, res, zx, q0, rows);
 		return residual_lanes<T, TR, Ppl<T>::v, PO_NONE>(L, A, x, r, res, zx, q0, rows);
 	}
-	return residual_row<T, TR>(L, A, x, r, res, w, zx);
+	const int nseg = residual_segments(L.m, (int)sizeof(T));
+	return residual_row<T, TR>(L, A, x, r, res, w / nseg, w % nseg, zx);
 }
 
 template <typename T, typename TR>
